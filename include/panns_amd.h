@@ -1,0 +1,162 @@
+/*
+ * C-ABI of the MI355X NSG top-M search (the drop-in boundary).
+ *
+ * The reference boundary is the C++ class PANNS::Searching (no FFI); these
+ * entry points replace its search-path methods one for one and are what a
+ * foreign binding (ctypes, cgo, JNI) or the C++ drop-in in
+ * include/panns/Searching.h call.  Tags:
+ *   PSS5LG = core/Searching.202102031939.PSS_v5.large_graph.dist_thresh.profiling.{cpp,h}
+ *   TOPMLG = core/Searching.202011241615.top_m_pure_large_graph.{cpp,h}
+ *
+ * Conventions: every call returns PANNS_OK (0) or a negative panns_status;
+ * nothing calls exit().  panns_last_error() gives the message of the last
+ * failing call on the calling thread.  One handle owns one HIP stream and its
+ * device memory; calls on one handle are serialised by the caller; distinct
+ * handles are independent.  No torch or HIP types appear in the signatures.
+ */
+#ifndef PANNS_AMD_H
+#define PANNS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    PANNS_OK = 0,
+    PANNS_ERR_ARG = -1,         /* bad argument (K > L, L > N, M == 0, null pointer, ...) */
+    PANNS_ERR_IO = -2,          /* file cannot be opened / read */
+    PANNS_ERR_FORMAT = -3,      /* truncated or inconsistent file */
+    PANNS_ERR_DIM = -4,         /* base / query dimension mismatch or unsupported D */
+    PANNS_ERR_RANGE = -5,       /* neighbour id >= N, vertex-count mismatch, id space */
+    PANNS_ERR_ALLOC = -6,       /* host or device allocation failed */
+    PANNS_ERR_DEVICE = -7,      /* HIP runtime error */
+    PANNS_ERR_UNSUPPORTED = -8, /* valid in the reference but not offered here */
+    PANNS_ERR_STATE = -9        /* call order: e.g. search before the graph is loaded */
+} panns_status;
+
+typedef enum {
+    PANNS_MODE_TOP_M = 0,   /* para_search_with_top_m_pure_large_graph (TOPMLG.cpp:2022-2207) */
+    PANNS_MODE_PSS_V5 = 1,  /* para_search_PSS_v5_large_graph_dist_thresh_profiling (PSS5LG.cpp:1376-1724) */
+    PANNS_MODE_SCALE_M = 2  /* M = 1, 2, 4, ... up to M (the subsearch_with_top_m schedule, TOPMLG.cpp:1023-1071) */
+} panns_mode;
+
+typedef enum {
+    PANNS_STOP_FIXED = 0,   /* run until no unchecked candidate (the reference) */
+    PANNS_STOP_LEARNED = 1  /* stop when the loaded MLP predicts the top-K final */
+} panns_stop;
+
+typedef struct {
+    uint32_t K;            /* results per query, K <= L */
+    uint32_t L;            /* queue length, L <= N */
+    uint32_t M;            /* TOP_M: picks per iteration; SCALE_M: M_max; PSS_V5: X (sub-search steps) */
+    uint32_t mode;         /* panns_mode */
+    uint32_t num_threads;  /* PSS_V5: the reference's num_threads_; only 1 is offered */
+    uint32_t stop;         /* panns_stop */
+    float stop_threshold;  /* LEARNED: stop when sigmoid output >= threshold */
+    uint32_t hash_slots;   /* 0 = automatic; else power of two, LDS visited-hash slots per query */
+    uint32_t reserved[4];
+} panns_params;
+
+typedef struct {
+    uint64_t distance_computations; /* count_distance_computation_ (PSS5LG.h:170) */
+    uint64_t merges;                /* count_merge_ (PSS5LG.h:171) as the reference counts it */
+    uint64_t expansions;            /* candidates expanded */
+    uint64_t adjacency_slots;       /* neighbour ids read (sum of expanded degrees) */
+    uint64_t visited_spills;        /* vertices kept in the global bitmap, not the LDS hash */
+    double kernel_ms;               /* device time of the search kernel(s) */
+} panns_counters;
+
+/* Outputs of a search, host memory.  ids is required; the rest may be NULL.
+ * Per-query arrays are indexed by the position of the query in the call. */
+typedef struct {
+    uint32_t *ids;           /* nq * K: set_K */
+    float *dists;            /* nq * K */
+    uint32_t *queue_ids;     /* nq * L: final set_L ids */
+    float *queue_dists;      /* nq * L: final set_L distances */
+    uint8_t *queue_checked;  /* nq * L: final set_L is_checked_ */
+    uint32_t *ndist;         /* nq: distance computations per query */
+    uint32_t *nexp;          /* nq: expansions per query */
+    uint32_t *niter;         /* nq: iterations (picks rounds) per query */
+} panns_outputs;
+
+typedef struct panns_index panns_index;
+typedef struct panns_plan panns_plan;
+
+const char *panns_last_error(void);
+
+/* Handle on HIP device `device` (ordinal).  Replaces PANNS::Searching's
+ * constructor; destroy replaces ~Searching (PSS5LG.h:211-219). */
+int panns_create(int device, panns_index **out);
+int panns_destroy(panns_index *h);
+
+/* PSS5LG.cpp:14-30 load_data_load: fvecs base set; errors on a dimension
+ * that differs from already-loaded queries. */
+int panns_load_data_load(panns_index *h, const char *fvecs_path);
+/* PSS5LG.cpp:35-49 load_queries_load. */
+int panns_load_queries_load(panns_index *h, const char *fvecs_path);
+/* In-memory forms of the two loaders (row-major n x dim float32). */
+int panns_set_base(panns_index *h, const float *data, uint32_t n, uint32_t dim);
+int panns_set_queries(panns_index *h, const float *data, uint32_t nq, uint32_t dim);
+
+/* PSS5LG.cpp:138-251 load_common_nsg_graph: .nsg file; errors when the
+ * vertex count differs from the base set or a neighbour id is >= N. */
+int panns_load_common_nsg_graph(panns_index *h, const char *nsg_path);
+/* In-memory form: adj[v*stride + i] for i < deg[v]. */
+int panns_set_graph(panns_index *h, uint32_t width, uint32_t ep, uint32_t n, uint32_t stride,
+                    const uint32_t *adj, const uint32_t *deg);
+
+/* num_v_, dimension_, num_queries_, width_, ep_, num_e_ (PSS5LG.h:32-45). */
+int panns_get_info(panns_index *h, uint32_t *num_v, uint32_t *dim, uint32_t *num_queries,
+                   uint32_t *width, uint32_t *ep, uint64_t *num_e);
+
+/* PSS5LG.cpp:356-410 prepare_init_ids: L ids into init_ids. */
+int panns_prepare_init_ids(panns_index *h, uint32_t L, uint32_t *init_ids);
+
+/* One batch of queries (ids into the loaded query set) searched with the same
+ * parameters and the same init_ids (n_init >= L).  Covers, per query:
+ *   TOP_M   -> para_search_with_top_m_pure_large_graph (TOPMLG.cpp:2022-2207)
+ *   PSS_V5  -> para_search_PSS_v5_large_graph_dist_thresh_profiling at
+ *              num_threads_ = 1 (PSS5LG.cpp:1376-1724)
+ *   SCALE_M -> initialize_set_L_para then subsearch_with_top_m
+ * Results are identical in ids to the reference's num_threads_ = 1 run. */
+int panns_search(panns_index *h, const panns_params *p, const uint32_t *query_ids, uint32_t nq,
+                 const uint32_t *init_ids, uint32_t n_init, panns_outputs *out,
+                 panns_counters *cnt);
+
+/* TOPMLG.cpp:1023-1071 subsearch_with_top_m on a caller-held, already
+ * initialised queue.  queue_* hold local_L entries sorted by (dist, id);
+ * *queue_size must equal local_L (a full queue; the reference's callers
+ * always pass one).  visited_words holds num_v bits (boost::dynamic_bitset
+ * block order: bit v in word v/64); it is read and updated, and left set on
+ * return as the reference leaves it.  *count_distance is incremented. */
+int panns_subsearch_with_top_m(panns_index *h, uint32_t M_max, uint32_t query_id,
+                               uint32_t local_L, uint32_t *queue_ids, float *queue_dists,
+                               uint8_t *queue_checked, uint32_t *queue_size,
+                               uint64_t *visited_words, uint64_t *count_distance);
+
+/* Resident batches: inputs uploaded once, outputs kept in HBM until fetched,
+ * so repeated runs time only the device work.  L_per_query / K_per_query may
+ * be NULL (uniform p->L / p->K); otherwise each entry overrides them for its
+ * query (uneven batches) and init_ids must hold max L entries. */
+int panns_plan_create(panns_index *h, const panns_params *p, const uint32_t *query_ids,
+                      uint32_t nq, const uint32_t *L_per_query, const uint32_t *K_per_query,
+                      const uint32_t *init_ids, uint32_t n_init, panns_plan **out);
+/* Enqueue the batch; when sync != 0 wait for it.  *kernel_ms (nullable)
+ * receives the device time of this run when sync != 0. */
+int panns_plan_run(panns_plan *plan, int sync, float *kernel_ms);
+/* Copy results of the last run to host (waits for it).  Per-query K is the
+ * plan's max K: row q holds K_q valid ids followed by 0xFFFFFFFF. */
+int panns_plan_fetch(panns_plan *plan, panns_outputs *out, panns_counters *cnt);
+int panns_plan_destroy(panns_plan *plan);
+/* Launch geometry chosen for the plan: persistent workgroups and the LDS
+ * bytes each one uses. */
+int panns_plan_describe(panns_plan *plan, uint32_t *workgroups, uint32_t *lds_bytes,
+                        uint32_t *hash_slots, uint32_t *max_L);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,437 @@
+/*
+ * CPU oracle for the NSG top-M / PSS_v5 search path -- TEST INFRASTRUCTURE.
+ * See panns_oracle.h for scope, citations and how each function is pinned.
+ * Built with -ffp-contract=off so that the only fused operations are the
+ * explicit fmaf() calls that mirror the reference's compiled FMA chain.
+ */
+#include "panns_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static inline int bit_get(const uint64_t *bits, uint32_t i) { return (int)((bits[i >> 6] >> (i & 63)) & 1u); }
+static inline void bit_set(uint64_t *bits, uint32_t i) { bits[i >> 6] |= (uint64_t)1 << (i & 63); }
+
+/* PSS5LG.cpp:525-560.  D8 = (D+7)&~7, DR = D8 % 16, DD = D8 - DR.  Eight
+ * accumulators; the tail block [DD, DD+8) is added first when DR != 0, then
+ * blocks of 16 as two 8-wide halves.  Each lane: acc = fma(v-q, v-q, acc).
+ * Horizontal sum left to right. */
+float po_distance(const float *v, const float *q, uint32_t dim)
+{
+    const uint32_t D8 = (dim + 7u) & ~7u;
+    const uint32_t DR = D8 % 16u;
+    const uint32_t DD = D8 - DR;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (DR) {
+        for (int j = 0; j < 8; ++j) {
+            float t = v[DD + j] - q[DD + j];
+            acc[j] = fmaf(t, t, acc[j]);
+        }
+    }
+    for (uint32_t i = 0; i < DD; i += 16) {
+        for (int j = 0; j < 8; ++j) {
+            float t = v[i + j] - q[i + j];
+            acc[j] = fmaf(t, t, acc[j]);
+        }
+        for (int j = 0; j < 8; ++j) {
+            float t = v[i + 8 + j] - q[i + 8 + j];
+            acc[j] = fmaf(t, t, acc[j]);
+        }
+    }
+    float r = acc[0];
+    for (int j = 1; j < 8; ++j) r = r + acc[j];
+    return r;
+}
+
+int po_cand_less(const po_cand *a, const po_cand *b)
+{
+    if (a->dist != b->dist) return a->dist < b->dist;
+    return a->id < b->id;
+}
+
+/* std::lower_bound over [lo, hi) with Candidate::operator<. */
+static uint32_t lower_bound_cand(const po_cand *q, uint32_t lo, uint32_t hi, const po_cand *c)
+{
+    uint32_t count = hi - lo;
+    while (count > 0) {
+        uint32_t step = count / 2;
+        uint32_t it = lo + step;
+        if (po_cand_less(&q[it], c)) {
+            lo = it + 1;
+            count -= step + 1;
+        } else {
+            count = step;
+        }
+    }
+    return lo;
+}
+
+/* PSS5LG.cpp:565-608 */
+uint32_t po_add_into_queue(po_cand *queue, uint32_t queue_start, uint32_t *queue_size,
+                           uint32_t queue_capacity, po_cand cand)
+{
+    if (*queue_size == 0) {
+        queue[queue_start + (*queue_size)++] = cand;
+        return 0;
+    }
+    uint32_t queue_end = queue_start + *queue_size;
+    uint32_t insert_loc = lower_bound_cand(queue, queue_start, queue_end, &cand);
+    if (insert_loc != queue_end) {
+        if (cand.id == queue[insert_loc].id) return queue_capacity; /* duplicate */
+        if (*queue_size >= queue_capacity) { /* full: drop the last */
+            --*queue_size;
+            --queue_end;
+        }
+    } else {
+        if (*queue_size < queue_capacity) {
+            queue[insert_loc] = cand;
+            ++*queue_size;
+            return *queue_size - 1;
+        }
+        return queue_capacity;
+    }
+    memmove(queue + insert_loc + 1, queue + insert_loc, (size_t)(queue_end - insert_loc) * sizeof(po_cand));
+    queue[insert_loc] = cand;
+    ++*queue_size;
+    return insert_loc - queue_start;
+}
+
+/* PSS5LG.cpp:356-410: ep's neighbours in adjacency order (de-duplicated),
+ * then ep+1, ep+2, ... wrapping at num_v, skipping ids already chosen. */
+int po_prepare_init_ids(const po_graph *g, uint32_t L, uint32_t *init_ids)
+{
+    if (L > g->num_v) return -1;
+    uint64_t *sel = (uint64_t *)calloc((g->num_v + 63) / 64, sizeof(uint64_t));
+    if (!sel) return -2;
+    const uint32_t *out_edges = g->deg_ngbrs + g->vertex_base[g->ep];
+    uint32_t out_degree = *out_edges++;
+    uint32_t end = 0;
+    for (uint32_t e = 0; e < out_degree && end < L; ++e) {
+        uint32_t v = out_edges[e];
+        if (bit_get(sel, v)) continue;
+        bit_set(sel, v);
+        init_ids[end++] = v;
+    }
+    uint32_t tmp = g->ep + 1;
+    while (end < L) {
+        if (tmp == g->num_v) tmp = 0;
+        uint32_t v = tmp++;
+        if (bit_get(sel, v)) continue;
+        bit_set(sel, v);
+        init_ids[end++] = v;
+    }
+    free(sel);
+    return 0;
+}
+
+static int cmp_cand(const void *a, const void *b)
+{
+    const po_cand *x = (const po_cand *)a, *y = (const po_cand *)b;
+    if (po_cand_less(x, y)) return -1;
+    if (po_cand_less(y, x)) return 1;
+    return 0;
+}
+
+/* PSS5LG.cpp:1055-1096 */
+void po_initialize_set_L(const float *base, uint32_t dim, const float *query, uint32_t L,
+                         po_cand *set_L, uint32_t set_L_start, uint32_t *set_L_size,
+                         const uint32_t *init_ids, uint64_t *visited, uint64_t *count_distance)
+{
+    for (uint32_t c = 0; c < L; ++c) bit_set(visited, init_ids[c]);
+    for (uint32_t i = 0; i < L; ++i) {
+        uint32_t v = init_ids[i];
+        po_cand c;
+        memset(&c, 0, sizeof(c));
+        c.id = v;
+        c.dist = po_distance(base + (size_t)v * dim, query, dim);
+        c.checked = 0;
+        set_L[set_L_start + i] = c;
+    }
+    *count_distance += L;
+    qsort(set_L + set_L_start, L, sizeof(po_cand), cmp_cand);
+    *set_L_size = L;
+}
+
+/* TOPMLG.cpp:1262-1330 (and PSS5LG.cpp:984-1053 whose extra dist_thresh
+ * test is implied by the live bound, which never increases).  dist_bound
+ * points into the queue so that it is read live, as the reference's
+ * `const distf &` parameter is.  Returns the lowest insert index. */
+static uint32_t expand_one_candidate(const po_graph *g, const float *base, uint32_t dim,
+                                     uint32_t cand_id, const float *query,
+                                     const float *dist_bound, po_cand *set_L,
+                                     uint32_t queue_start, uint32_t *queue_size,
+                                     uint32_t queue_capacity, uint64_t *visited,
+                                     uint64_t *count_distance)
+{
+    const uint32_t *out_edges = g->deg_ngbrs + g->vertex_base[cand_id];
+    uint32_t out_degree = *out_edges++;
+    uint32_t nk = queue_capacity;
+    uint64_t cnt = 0;
+    for (uint32_t e = 0; e < out_degree; ++e) {
+        uint32_t nb = out_edges[e];
+        if (bit_get(visited, nb)) continue;
+        bit_set(visited, nb);
+        ++cnt;
+        float d = po_distance(base + (size_t)nb * dim, query, dim);
+        if (d > *dist_bound) continue;
+        po_cand c;
+        memset(&c, 0, sizeof(c));
+        c.id = nb;
+        c.dist = d;
+        c.checked = 0;
+        uint32_t r = po_add_into_queue(set_L, queue_start, queue_size, queue_capacity, c);
+        if (r < nk) nk = r;
+    }
+    *count_distance += cnt;
+    return nk;
+}
+
+/* TOPMLG.cpp:1172-1198 */
+static void pick_top_m_unchecked(uint32_t M, uint32_t k_uc, po_cand *set_L, uint32_t start,
+                                 uint32_t size, uint32_t *top_m, uint32_t *top_m_size,
+                                 uint32_t *last_k)
+{
+    uint32_t tmp_last_k = size;
+    uint32_t n = 0;
+    for (uint32_t c = start + k_uc; c < start + size && n < M; ++c) {
+        if (set_L[c].checked) continue;
+        tmp_last_k = c - start;
+        set_L[c].checked = 1;
+        top_m[n++] = set_L[c].id;
+    }
+    *last_k = tmp_last_k;
+    *top_m_size = n;
+}
+
+/* TOPMLG.cpp:2022-2207 at num_threads_ = 1: the only queue is the master,
+ * so merge_all_queues_to_master (PSS5LG.cpp:811-842) merges nothing. */
+void po_search_top_m(const po_graph *g, const float *base, uint32_t dim, const float *query,
+                     uint32_t K, uint32_t L, uint32_t M, const uint32_t *init_ids,
+                     po_cand *set_L, uint32_t *set_K, uint64_t *visited, po_counters *cnt)
+{
+    uint32_t size = 0;
+    uint64_t ndist = 0;
+    po_initialize_set_L(base, dim, query, L, set_L, 0, &size, init_ids, visited, &ndist);
+    const float *last_dist = &set_L[L - 1].dist;
+    uint32_t *top_m = (uint32_t *)malloc(sizeof(uint32_t) * (M ? M : 1));
+    uint32_t k = 0, last_k = 0, n_top = 0;
+    uint64_t merges = 0, exps = 0;
+    while (1) {
+        last_k = L;
+        pick_top_m_unchecked(M, k, set_L, 0, size, top_m, &n_top, &last_k);
+        if (!n_top) break;
+        uint32_t nk = L;
+        for (uint32_t c = 0; c < n_top; ++c) {
+            uint32_t r = expand_one_candidate(g, base, dim, top_m[c], query, last_dist, set_L, 0,
+                                              &size, L, visited, &ndist);
+            if (r < nk) nk = r;
+        }
+        exps += n_top;
+        ++merges;
+        /* merge_all_queues_to_master returns L here (no worker queues) */
+        k = (nk <= last_k) ? nk : last_k + 1;
+    }
+    for (uint32_t i = 0; i < K; ++i) set_K[i] = set_L[i].id;
+    memset(visited, 0, sizeof(uint64_t) * ((g->num_v + 63) / 64));
+    free(top_m);
+    if (cnt) {
+        cnt->distance_computations += ndist;
+        cnt->merges += merges;
+        cnt->expansions += exps;
+    }
+}
+
+/* PSS5LG.cpp:1376-1724 at num_threads_ = 1.  pick_top_m_to_workers
+ * (PSS5LG.cpp:917-947) then only counts unchecked entries (there is no
+ * worker queue), the master runs up to X expansions from k_master, and the
+ * merge loop (PSS5LG.cpp:821) is empty. */
+void po_search_pss_v5_t1(const po_graph *g, const float *base, uint32_t dim, const float *query,
+                         uint32_t K, uint32_t L, uint32_t X, const uint32_t *init_ids,
+                         po_cand *set_L, uint32_t *set_K, uint64_t *visited, po_counters *cnt)
+{
+    uint32_t size = 0;
+    uint64_t ndist = 0, merges = 0, exps = 0;
+    po_initialize_set_L(base, dim, query, L, set_L, 0, &size, init_ids, visited, &ndist);
+    const float *last_dist = &set_L[size - 1].dist;
+    uint32_t k_master = 0;
+    int no_need = 0;
+    /* sequential start: one step (seq_iter_bound = 1), PSS5LG.cpp:1478-1534 */
+    if (k_master == L) {
+        no_need = 1;
+    } else {
+        uint32_t r;
+        po_cand *c = &set_L[k_master];
+        if (!c->checked) {
+            c->checked = 1;
+            r = expand_one_candidate(g, base, dim, c->id, query, last_dist, set_L, 0, &size, L,
+                                     visited, &ndist);
+            ++exps;
+        } else {
+            r = L;
+        }
+        k_master = (r <= k_master) ? r : k_master + 1;
+    }
+    while (!no_need) {
+        /* pick_top_m_to_workers: counts unchecked from k_master */
+        uint32_t count = 0;
+        for (uint32_t c = k_master; c < size; ++c)
+            if (!set_L[c].checked) ++count;
+        if (!count) break;
+        /* master expands up to X candidates, PSS5LG.cpp:1584-1646 */
+        uint32_t k_uc = k_master, it = 0;
+        while (it < X && k_uc < size) {
+            po_cand *c = &set_L[k_uc];
+            if (!c->checked) {
+                c->checked = 1;
+                ++it;
+                uint32_t r = expand_one_candidate(g, base, dim, c->id, query, last_dist, set_L, 0,
+                                                  &size, L, visited, &ndist);
+                ++exps;
+                k_uc = (r <= k_uc) ? r : k_uc + 1;
+            } else {
+                ++k_uc;
+            }
+        }
+        k_master = k_uc;
+        ++merges;
+        /* merge_all_queues_to_master returns L > k_master: no change */
+    }
+    for (uint32_t i = 0; i < K; ++i) set_K[i] = set_L[i].id;
+    memset(visited, 0, sizeof(uint64_t) * ((g->num_v + 63) / 64));
+    if (cnt) {
+        cnt->distance_computations += ndist;
+        cnt->merges += merges;
+        cnt->expansions += exps;
+    }
+}
+
+/* TOPMLG.cpp:1077-1131: the bound passed to expand_one_candidate is the
+ * element at index set_L_size-1 at the time of each call (a live reference). */
+static void subsearch_one_iteration(const po_graph *g, const float *base, uint32_t dim,
+                                    const float *query, uint32_t *k_uc, uint32_t M, uint32_t L,
+                                    po_cand *set_L, uint32_t start, uint32_t *size,
+                                    uint32_t *top_m, uint64_t *visited, uint64_t *ndist,
+                                    uint64_t *exps)
+{
+    uint32_t n_top = 0, last_k = L;
+    pick_top_m_unchecked(M, *k_uc, set_L, start, *size, top_m, &n_top, &last_k);
+    uint32_t nk = L;
+    for (uint32_t c = 0; c < n_top; ++c) {
+        const float *bound = &set_L[*size - 1 + start].dist;
+        uint32_t r = expand_one_candidate(g, base, dim, top_m[c], query, bound, set_L, start, size,
+                                          L, visited, ndist);
+        if (r < nk) nk = r;
+    }
+    *exps += n_top;
+    *k_uc = (nk <= last_k) ? nk : last_k + 1;
+}
+
+/* TOPMLG.cpp:1023-1071: M = 1, doubled after every iteration while
+ * M < value_M_max; loop while k < local_L. */
+void po_subsearch_with_top_m(const po_graph *g, const float *base, uint32_t dim,
+                             const float *query, uint32_t M_max, uint32_t local_L,
+                             po_cand *set_L, uint32_t set_L_start, uint32_t *set_L_size,
+                             uint64_t *visited, uint64_t *count_distance)
+{
+    uint32_t cap = 1;
+    while (cap < M_max) cap <<= 1;
+    uint32_t *top_m = (uint32_t *)malloc(sizeof(uint32_t) * (cap + 1));
+    uint32_t k = 0, M = 1;
+    uint64_t exps = 0;
+    while (k < local_L) {
+        subsearch_one_iteration(g, base, dim, query, &k, M, local_L, set_L, set_L_start,
+                                set_L_size, top_m, visited, count_distance, &exps);
+        if (M < M_max) M <<= 1;
+    }
+    free(top_m);
+}
+
+uint64_t po_search_batch(const po_graph *g, const float *base, uint32_t dim,
+                         const float *queries, uint32_t nq, uint32_t K, uint32_t L,
+                         uint32_t M, int mode, const uint32_t *init_ids, uint32_t *out_ids,
+                         float *out_dists, uint32_t *out_ndist, int nthreads)
+{
+    uint64_t total = 0;
+    const size_t words = (g->num_v + 63) / 64;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads) reduction(+ : total)
+#endif
+    {
+        po_cand *set_L = (po_cand *)malloc(sizeof(po_cand) * L);
+        uint64_t *visited = (uint64_t *)calloc(words, sizeof(uint64_t));
+        uint32_t *set_K = (uint32_t *)malloc(sizeof(uint32_t) * K);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 4)
+#endif
+        for (int64_t qi = 0; qi < (int64_t)nq; ++qi) {
+            po_counters c = {0, 0, 0};
+            const float *q = queries + (size_t)qi * dim;
+            if (mode == 1)
+                po_search_pss_v5_t1(g, base, dim, q, K, L, M, init_ids, set_L, set_K, visited, &c);
+            else
+                po_search_top_m(g, base, dim, q, K, L, M, init_ids, set_L, set_K, visited, &c);
+            for (uint32_t i = 0; i < K; ++i) {
+                out_ids[(size_t)qi * K + i] = set_L[i].id;
+                if (out_dists) out_dists[(size_t)qi * K + i] = set_L[i].dist;
+            }
+            if (out_ndist) out_ndist[qi] = (uint32_t)c.distance_computations;
+            total += c.distance_computations;
+        }
+        free(set_L);
+        free(visited);
+        free(set_K);
+    }
+    return total;
+}
+
+void po_brute_force_knn(const float *base, uint32_t n, uint32_t dim, const float *queries,
+                        uint32_t nq, uint32_t K, uint32_t *out_ids, float *out_dists,
+                        int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        po_cand *q = (po_cand *)malloc(sizeof(po_cand) * K);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int64_t qi = 0; qi < (int64_t)nq; ++qi) {
+            uint32_t size = 0;
+            const float *qd = queries + (size_t)qi * dim;
+            for (uint32_t v = 0; v < n; ++v) {
+                po_cand c;
+                memset(&c, 0, sizeof(c));
+                c.id = v;
+                c.dist = po_distance(base + (size_t)v * dim, qd, dim);
+                po_add_into_queue(q, 0, &size, K, c);
+            }
+            for (uint32_t i = 0; i < K; ++i) {
+                out_ids[(size_t)qi * K + i] = i < size ? q[i].id : 0xFFFFFFFFu;
+                if (out_dists) out_dists[(size_t)qi * K + i] = i < size ? q[i].dist : INFINITY;
+            }
+        }
+        free(q);
+    }
+}
+
+double po_recall(const uint32_t *true_ids, uint32_t true_k, const uint32_t *res_ids,
+                 uint32_t res_k, uint32_t nq, uint32_t L, uint32_t R)
+{
+    const uint32_t set_K_size = L < R ? L : R;
+    double hits = 0.0;
+    for (uint32_t q = 0; q < nq; ++q) {
+        for (uint32_t t = 0; t < R; ++t) {
+            uint32_t tid = true_ids[(size_t)q * true_k + t];
+            for (uint32_t n = 0; n < set_K_size && n < res_k; ++n)
+                if (res_ids[(size_t)q * res_k + n] == tid) hits += 1.0;
+        }
+    }
+    return hits / ((double)R * (double)nq);
+}

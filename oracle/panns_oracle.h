@@ -1,0 +1,135 @@
+/*
+ * CPU oracle for the NSG top-M / PSS_v5 search path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of the reference's
+ * algorithm used as the checker for the HIP path.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (parallel_anns_amd) never links or calls it.
+ *
+ * Every function names the reference file:line it follows.  Tags:
+ *   PSS5LG = core/Searching.202102031939.PSS_v5.large_graph.dist_thresh.profiling.{cpp,h}
+ *   TOPMLG = core/Searching.202011241615.top_m_pure_large_graph.{cpp,h}
+ *
+ * Pinning (what checks each function against something other than itself):
+ *   po_distance            AVX2 intrinsics build with GCC contraction (tests),
+ *                          fp64 closeness, v==q -> +0, one-hot exact values.
+ *   po_add_into_queue      brute force over tiny queues (sorted insert model).
+ *   po_prepare_init_ids    hand-made graphs with known answers.
+ *   po_search_top_m        L == N reduces to exact brute force; complete graph;
+ *                          an independent set-semantics (top-L of union)
+ *                          restatement in numpy; invariants
+ *                          (#dist = L + |visited \ init|, final queue all checked).
+ *   po_search_pss_v5_t1    equals po_search_top_m with M = 1 for every X.
+ *   po_subsearch_with_top_m   scale-M equals the same set semantics with the
+ *                          doubling M schedule.
+ *   po_brute_force_knn     tiny inputs against numpy float64 ranking with ties
+ *                          checked by id; equals L == N search.
+ * The reference itself cannot be built here (it needs Boost, which the image
+ * lacks), so no golden vector comes from a compiled reference.
+ */
+#ifndef PANNS_ORACLE_H
+#define PANNS_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* include/Candidate.h:11-28 : {id, distance, is_checked}, ordered by (distance, id). */
+typedef struct {
+    uint32_t id;
+    float dist;
+    uint8_t checked;
+    uint8_t pad_[3];
+} po_cand;
+
+/* CSR graph as built by load_common_nsg_graph (PSS5LG.cpp:138-251):
+ * vertex_base[v] indexes deg_ngbrs, where deg_ngbrs[vertex_base[v]] = degree
+ * followed by the neighbour ids in file order. */
+typedef struct {
+    uint32_t num_v;
+    uint32_t width;
+    uint32_t ep;
+    uint64_t num_e;
+    const uint64_t *vertex_base;
+    const uint32_t *deg_ngbrs;
+} po_graph;
+
+typedef struct {
+    uint64_t distance_computations; /* count_distance_computation_ (PSS5LG.h:170) */
+    uint64_t merges;                /* count_merge_ (PSS5LG.h:171) */
+    uint64_t expansions;            /* candidates expanded (hops) */
+} po_counters;
+
+/* PSS5LG.cpp:525-560 compute_distance as compiled with -O3 -march=native:
+ * vsubps + vfmadd per 8-lane accumulator, tail block first, then a sequential
+ * horizontal sum.  dim must be a multiple of 8. */
+float po_distance(const float *v, const float *q, uint32_t dim);
+
+/* include/Candidate.h:19-27 operator< */
+int po_cand_less(const po_cand *a, const po_cand *b);
+
+/* PSS5LG.cpp:565-608 add_into_queue.  Returns the insert index relative to
+ * queue_start, or queue_capacity when rejected (duplicate or full). */
+uint32_t po_add_into_queue(po_cand *queue, uint32_t queue_start, uint32_t *queue_size,
+                           uint32_t queue_capacity, po_cand cand);
+
+/* PSS5LG.cpp:356-410 prepare_init_ids.  Returns 0, or -1 when L > num_v. */
+int po_prepare_init_ids(const po_graph *g, uint32_t L, uint32_t *init_ids);
+
+/* TOPMLG.cpp:2022-2207 para_search_with_top_m_pure_large_graph with
+ * num_threads_ == 1.  visited must hold num_v bits, all clear; it is cleared
+ * again on return (PSS5LG.cpp:1691).  set_L receives the final queue (L
+ * entries), set_K the first K ids. */
+void po_search_top_m(const po_graph *g, const float *base, uint32_t dim, const float *query,
+                     uint32_t K, uint32_t L, uint32_t M, const uint32_t *init_ids,
+                     po_cand *set_L, uint32_t *set_K, uint64_t *visited, po_counters *cnt);
+
+/* PSS5LG.cpp:1376-1724 para_search_PSS_v5_large_graph_dist_thresh_profiling
+ * with num_threads_ == 1 (X = subsearch_iterations). */
+void po_search_pss_v5_t1(const po_graph *g, const float *base, uint32_t dim, const float *query,
+                         uint32_t K, uint32_t L, uint32_t X, const uint32_t *init_ids,
+                         po_cand *set_L, uint32_t *set_K, uint64_t *visited, po_counters *cnt);
+
+/* TOPMLG.cpp:1023-1071 subsearch_with_top_m + :1077-1131
+ * subsearch_top_m_for_one_iteration.  Works in place on set_L[set_L_start ..
+ * set_L_start + *set_L_size) with capacity local_L and the caller's visited
+ * bits (left set on return, as in the reference). */
+void po_subsearch_with_top_m(const po_graph *g, const float *base, uint32_t dim,
+                             const float *query, uint32_t M_max, uint32_t local_L,
+                             po_cand *set_L, uint32_t set_L_start, uint32_t *set_L_size,
+                             uint64_t *visited, uint64_t *count_distance);
+
+/* PSS5LG.cpp:1055-1096 initialize_set_L_para (marks init ids visited,
+ * computes L distances, sorts).  Exposed for subsearch tests. */
+void po_initialize_set_L(const float *base, uint32_t dim, const float *query, uint32_t L,
+                         po_cand *set_L, uint32_t set_L_start, uint32_t *set_L_size,
+                         const uint32_t *init_ids, uint64_t *visited, uint64_t *count_distance);
+
+/* Batch driver used for the CPU baseline: queries [q0, q0+nq) searched
+ * independently, one per OpenMP thread (inter-query parallel, num_t = 1 per
+ * query, the PSS_v5_multi_queries pattern with inner = 1).  mode 0 = top-M,
+ * 1 = PSS_v5 T=1 with X = M.  out_ids is nq*K.  Returns total distances. */
+uint64_t po_search_batch(const po_graph *g, const float *base, uint32_t dim,
+                         const float *queries, uint32_t nq, uint32_t K, uint32_t L,
+                         uint32_t M, int mode, const uint32_t *init_ids, uint32_t *out_ids,
+                         float *out_dists, uint32_t *out_ndist, int nthreads);
+
+/* tools/save_true_nearest_neighbors.cpp:16-35 get_true_NN: every base point
+ * offered to add_into_queue with capacity K, so ties go to the lower id. */
+void po_brute_force_knn(const float *base, uint32_t n, uint32_t dim, const float *queries,
+                        uint32_t nq, uint32_t K, uint32_t *out_ids, float *out_dists,
+                        int nthreads);
+
+/* PSS5LG.cpp:309-348 get_recall_for_all_queries, recall@100 only, generalised
+ * to recall@R for R <= min(K_true, K_res):  sum_q |GT_R ∩ res[0..min(L,R))| / (R*Q). */
+double po_recall(const uint32_t *true_ids, uint32_t true_k, const uint32_t *res_ids,
+                 uint32_t res_k, uint32_t nq, uint32_t L, uint32_t R);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,184 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, and only as the checker / the timed CPU baseline.  The product
+package never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "liboracle.so")
+
+u32, u64, cint, cf, cd, vp = (ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_float,
+                              ctypes.c_double, ctypes.c_void_p)
+
+
+class Cand(ctypes.Structure):
+    _fields_ = [("id", u32), ("dist", cf), ("checked", ctypes.c_uint8), ("pad_", ctypes.c_uint8 * 3)]
+
+
+CAND_DT = np.dtype([("id", "<u4"), ("dist", "<f4"), ("checked", "u1"), ("pad", "u1", (3,))])
+
+
+class Graph(ctypes.Structure):
+    _fields_ = [("num_v", u32), ("width", u32), ("ep", u32), ("num_e", u64),
+                ("vertex_base", vp), ("deg_ngbrs", vp)]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [("distance_computations", u64), ("merges", u64), ("expansions", u64)]
+
+
+_API = {
+    "po_distance": (cf, [vp, vp, u32]),
+    "po_add_into_queue": (u32, [vp, u32, ctypes.POINTER(u32), u32, Cand]),
+    "po_prepare_init_ids": (cint, [ctypes.POINTER(Graph), u32, vp]),
+    "po_search_top_m": (None, [ctypes.POINTER(Graph), vp, u32, vp, u32, u32, u32, vp, vp, vp, vp,
+                               ctypes.POINTER(Counters)]),
+    "po_search_pss_v5_t1": (None, [ctypes.POINTER(Graph), vp, u32, vp, u32, u32, u32, vp, vp, vp, vp,
+                                   ctypes.POINTER(Counters)]),
+    "po_subsearch_with_top_m": (None, [ctypes.POINTER(Graph), vp, u32, vp, u32, u32, vp, u32,
+                                       ctypes.POINTER(u32), vp, ctypes.POINTER(u64)]),
+    "po_initialize_set_L": (None, [vp, u32, vp, u32, vp, u32, ctypes.POINTER(u32), vp, vp,
+                                   ctypes.POINTER(u64)]),
+    "po_search_batch": (u64, [ctypes.POINTER(Graph), vp, u32, vp, u32, u32, u32, u32, cint, vp, vp,
+                              vp, vp, cint]),
+    "po_brute_force_knn": (None, [vp, u32, u32, vp, u32, u32, vp, vp, cint]),
+    "po_recall": (cd, [vp, u32, vp, u32, u32, u32, u32]),
+}
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SO):
+            build()
+        _lib = ctypes.CDLL(SO)
+        for n, (r, a) in _API.items():
+            f = getattr(_lib, n)
+            f.restype = r
+            f.argtypes = a
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(vp)
+
+
+class CsrGraph:
+    """The reference's CSR (vertex_base + [deg, ids...]) built from padded rows."""
+
+    def __init__(self, adj, deg, ep, width=None):
+        adj = np.asarray(adj, dtype=np.uint32)
+        deg = np.asarray(deg, dtype=np.uint32)
+        n = adj.shape[0]
+        self.vertex_base = np.zeros(n, dtype=np.uint64)
+        total = int(n + deg.sum())
+        self.deg_ngbrs = np.empty(total, dtype=np.uint32)
+        off = 0
+        for v in range(n):
+            d = int(deg[v])
+            self.vertex_base[v] = off
+            self.deg_ngbrs[off] = d
+            self.deg_ngbrs[off + 1:off + 1 + d] = adj[v, :d]
+            off += 1 + d
+        self.g = Graph(n, int(width if width is not None else (deg.max() if n else 0)), int(ep),
+                       int(deg.sum()), _p(self.vertex_base), _p(self.deg_ngbrs))
+        self.n = n
+
+
+def distance(v, q):
+    v = np.ascontiguousarray(v, np.float32)
+    q = np.ascontiguousarray(q, np.float32)
+    return float(lib().po_distance(_p(v), _p(q), v.shape[-1]))
+
+
+def prepare_init_ids(graph, L):
+    out = np.empty(L, np.uint32)
+    rc = lib().po_prepare_init_ids(ctypes.byref(graph.g), L, _p(out))
+    if rc:
+        raise ValueError("prepare_init_ids failed")
+    return out
+
+
+def search_one(graph, base, query, K, L, M, init_ids, mode="top_m", X=1):
+    base = np.ascontiguousarray(base, np.float32)
+    query = np.ascontiguousarray(query, np.float32)
+    init_ids = np.ascontiguousarray(init_ids, np.uint32)
+    set_L = np.zeros(L, CAND_DT)
+    set_K = np.zeros(K, np.uint32)
+    visited = np.zeros((graph.n + 63) // 64, np.uint64)
+    c = Counters()
+    f = lib().po_search_pss_v5_t1 if mode == "pss_v5" else lib().po_search_top_m
+    f(ctypes.byref(graph.g), _p(base), base.shape[1], _p(query), K, L, X if mode == "pss_v5" else M,
+      _p(init_ids), _p(set_L), _p(set_K), _p(visited), ctypes.byref(c))
+    return {"ids": set_K, "queue": set_L, "ndist": c.distance_computations, "merges": c.merges,
+            "nexp": c.expansions, "visited_clear": not visited.any()}
+
+
+def initialize_set_L(base, query, L, init_ids, n):
+    base = np.ascontiguousarray(base, np.float32)
+    query = np.ascontiguousarray(query, np.float32)
+    init_ids = np.ascontiguousarray(init_ids, np.uint32)
+    set_L = np.zeros(L, CAND_DT)
+    size = u32(0)
+    visited = np.zeros((n + 63) // 64, np.uint64)
+    cnt = u64(0)
+    lib().po_initialize_set_L(_p(base), base.shape[1], _p(query), L, _p(set_L), 0, ctypes.byref(size),
+                              _p(init_ids), _p(visited), ctypes.byref(cnt))
+    return set_L, visited, cnt.value
+
+
+def subsearch_with_top_m(graph, base, query, M_max, local_L, set_L, visited):
+    """In place on set_L (CAND_DT, local_L entries) and visited (uint64 words)."""
+    base = np.ascontiguousarray(base, np.float32)
+    query = np.ascontiguousarray(query, np.float32)
+    size = u32(local_L)
+    cnt = u64(0)
+    lib().po_subsearch_with_top_m(ctypes.byref(graph.g), _p(base), base.shape[1], _p(query), M_max,
+                                  local_L, _p(set_L), 0, ctypes.byref(size), _p(visited),
+                                  ctypes.byref(cnt))
+    return cnt.value
+
+
+def search_batch(graph, base, queries, K, L, M, init_ids, mode="top_m", nthreads=0):
+    base = np.ascontiguousarray(base, np.float32)
+    queries = np.ascontiguousarray(queries, np.float32)
+    init_ids = np.ascontiguousarray(init_ids, np.uint32)
+    nq = queries.shape[0]
+    ids = np.empty((nq, K), np.uint32)
+    dists = np.empty((nq, K), np.float32)
+    nd = np.empty(nq, np.uint32)
+    total = lib().po_search_batch(ctypes.byref(graph.g), _p(base), base.shape[1], _p(queries), nq, K, L, M,
+                                  1 if mode == "pss_v5" else 0, _p(init_ids), _p(ids), _p(dists), _p(nd),
+                                  nthreads)
+    return ids, dists, nd, int(total)
+
+
+def brute_force_knn(base, queries, K, nthreads=0):
+    base = np.ascontiguousarray(base, np.float32)
+    queries = np.ascontiguousarray(queries, np.float32)
+    nq = queries.shape[0]
+    ids = np.empty((nq, K), np.uint32)
+    dists = np.empty((nq, K), np.float32)
+    lib().po_brute_force_knn(_p(base), base.shape[0], base.shape[1], _p(queries), nq, K, _p(ids), _p(dists),
+                             nthreads)
+    return ids, dists
+
+
+def add_into_queue(queue, size, capacity, cand_id, cand_dist):
+    """queue: CAND_DT array of length >= capacity; returns (ret, new_size)."""
+    s = u32(size)
+    c = Cand(cand_id, cand_dist, 0)
+    r = lib().po_add_into_queue(_p(queue), 0, ctypes.byref(s), capacity, c)
+    return int(r), int(s.value)

@@ -1,0 +1,729 @@
+// C-ABI of the MI355X NSG top-M search: handle, loaders, plans, searches.
+// Device memory is owned by the handle; each handle has its own HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "panns_amd.h"
+#include "panns_tools.h"
+#include "search_kernel.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(PANNS_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t ensure(size_t count)
+    {
+        if (count <= n && p) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+};
+
+uint32_t next_pow2(uint32_t x)
+{
+    uint32_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+uint32_t log2u(uint32_t x)
+{
+    uint32_t l = 0;
+    while ((1u << l) < x) ++l;
+    return l;
+}
+
+constexpr uint32_t kMaxL = 4096;
+constexpr uint32_t kMaxM = 1024;
+constexpr uint32_t kLogCap = 8192;
+
+}  // namespace
+
+struct panns_index {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int num_cu = 0;
+    // base and queries
+    uint32_t n = 0, dim = 0, nq = 0;
+    DevBuf<float> base, queries;
+    // graph
+    bool has_graph = false;
+    uint32_t width = 0, ep = 0, max_deg = 0, stride_log2 = 5;
+    uint64_t num_e = 0;
+    DevBuf<uint32_t> adj;
+    std::vector<uint32_t> ep_row;
+    // scratch for visited spills: one bitmap and log per persistent workgroup
+    DevBuf<uint32_t> bitmaps, spill_log;
+    uint32_t bitmap_slots = 0;
+    uint64_t bitmap_words = 0;
+    DevBuf<uint32_t> counter;
+    // learned stop model (device copy), 8-64-64-1
+    DevBuf<float> model;
+    bool has_model = false;
+};
+
+struct panns_plan {
+    panns_index *h = nullptr;
+    panns_params p{};
+    uint32_t nq = 0, maxL = 0, maxK = 0;
+    DevBuf<uint32_t> qids, qL, qK, init_ids, out_ids, out_stats, out_qids;
+    DevBuf<float> out_dists, out_qdists;
+    DevBuf<uint8_t> out_qchk;
+    DevBuf<uint64_t> init_queue;
+    DevBuf<float> trace_feat;
+    DevBuf<uint32_t> trace_topk;
+    uint32_t trace_cap = 0;
+    bool want_queue = false;
+    uint32_t *shared_bitmap = nullptr;
+    uint32_t grid = 0, lds = 0, hash_slots = 0;
+    panns::KArgs args{};
+    bool ran = false;
+};
+
+namespace {
+
+template <int D>
+hipError_t launch_search(const panns::KArgs &a, uint32_t grid, uint32_t lds, hipStream_t s)
+{
+    hipLaunchKernelGGL(panns::search_kernel<D>, dim3(grid), dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+
+using LaunchFn = hipError_t (*)(const panns::KArgs &, uint32_t, uint32_t, hipStream_t);
+using KernelPtr = const void *;
+
+bool kernel_for_dim(uint32_t dim, LaunchFn *fn, KernelPtr *kp)
+{
+    switch (dim) {
+#define PANNS_DIM(DD)                                                  \
+    case DD:                                                           \
+        *fn = launch_search<DD>;                                       \
+        *kp = reinterpret_cast<const void *>(&panns::search_kernel<DD>); \
+        return true;
+        PANNS_DIM(16)
+        PANNS_DIM(32)
+        PANNS_DIM(64)
+        PANNS_DIM(96)
+        PANNS_DIM(128)
+#undef PANNS_DIM
+    default:
+        return false;
+    }
+}
+
+int check_ready(panns_index *h)
+{
+    if (!h) return fail(PANNS_ERR_ARG, "null handle");
+    if (!h->n) return fail(PANNS_ERR_STATE, "base set not loaded");
+    if (!h->has_graph) return fail(PANNS_ERR_STATE, "graph not loaded");
+    return PANNS_OK;
+}
+
+int check_params(panns_index *h, const panns_params *p, uint32_t L, uint32_t K)
+{
+    if (!p) return fail(PANNS_ERR_ARG, "null params");
+    if (p->mode > 2) return fail(PANNS_ERR_ARG, "unknown mode");
+    if (p->mode == PANNS_MODE_PSS_V5 && p->num_threads != 1)
+        return fail(PANNS_ERR_UNSUPPORTED,
+                    "PSS_v5 with num_threads_ > 1 is nondeterministic in the reference; only num_threads_ = 1 is offered");
+    if (L == 0 || K == 0) return fail(PANNS_ERR_ARG, "K and L must be positive");
+    if (K > L) return fail(PANNS_ERR_ARG, "K > L");
+    if (L > h->n) return fail(PANNS_ERR_ARG, "L > number of base vectors");
+    if (L > kMaxL) return fail(PANNS_ERR_UNSUPPORTED, "L above 4096 is not offered");
+    if (p->M == 0) return fail(PANNS_ERR_ARG, "M must be positive");
+    if (p->M > kMaxM) return fail(PANNS_ERR_UNSUPPORTED, "M above 1024 is not offered");
+    if (p->stop == PANNS_STOP_LEARNED && !h->has_model) return fail(PANNS_ERR_STATE, "no stop model loaded");
+    if (p->hash_slots && ((p->hash_slots & (p->hash_slots - 1)) || p->hash_slots < 64 || p->hash_slots > 16384))
+        return fail(PANNS_ERR_ARG, "hash_slots must be a power of two in [64, 16384]");
+    return PANNS_OK;
+}
+
+// Layout of one workgroup's LDS; must match search_kernel's carving.
+uint32_t lds_bytes(uint32_t Lcap, uint32_t nb_cap, uint32_t hash_slots, uint32_t Mcap)
+{
+    return 8u * (Lcap + nb_cap + 64u) + 4u * (hash_slots + nb_cap + Mcap + 4u) + 4u * 64u;
+}
+
+int ensure_scratch(panns_index *h, uint32_t grid)
+{
+    const uint64_t words = ((uint64_t)h->n + 31) / 32;
+    if (h->bitmap_slots < grid || h->bitmap_words != words) {
+        h->bitmaps.release();
+        h->spill_log.release();
+        HIP_TRY(h->bitmaps.ensure((size_t)grid * words));
+        HIP_TRY(hipMemsetAsync(h->bitmaps.p, 0, (size_t)grid * words * 4, h->stream));
+        HIP_TRY(h->spill_log.ensure((size_t)grid * kLogCap));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        h->bitmap_slots = grid;
+        h->bitmap_words = words;
+    }
+    if (!h->counter.p) HIP_TRY(h->counter.ensure(4));
+    return PANNS_OK;
+}
+
+int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, const uint32_t *Kq,
+               const uint32_t *init_ids, uint32_t n_init)
+{
+    panns_index *h = pl->h;
+    const panns_params &p = pl->p;
+    LaunchFn fn;
+    KernelPtr kp;
+    if (!kernel_for_dim(h->dim, &fn, &kp))
+        return fail(PANNS_ERR_DIM, "dimension " + std::to_string(h->dim) + " has no kernel (16, 32, 64, 96, 128)");
+    pl->maxL = p.L;
+    pl->maxK = p.K;
+    if (Lq || Kq) {
+        pl->maxL = 0;
+        pl->maxK = 0;
+        for (uint32_t i = 0; i < pl->nq; ++i) {
+            const uint32_t L = Lq ? Lq[i] : p.L, K = Kq ? Kq[i] : p.K;
+            int rc = check_params(h, &p, L, K);
+            if (rc) return rc;
+            pl->maxL = std::max(pl->maxL, L);
+            pl->maxK = std::max(pl->maxK, K);
+        }
+    }
+    int rc = check_params(h, &p, pl->maxL, pl->maxK);
+    if (rc) return rc;
+    if (!pl->init_queue.p && n_init < pl->maxL) return fail(PANNS_ERR_ARG, "init_ids shorter than L");
+    for (uint32_t i = 0; i < pl->nq; ++i)
+        if (query_ids[i] >= h->nq) return fail(PANNS_ERR_RANGE, "query id out of range");
+    if (!pl->init_queue.p)
+        for (uint32_t i = 0; i < pl->maxL; ++i)
+            if (init_ids[i] >= h->n) return fail(PANNS_ERR_RANGE, "init id out of range");
+
+    const uint32_t stride = 1u << h->stride_log2;
+    const uint32_t nb_cap = std::max<uint32_t>(256u, stride);
+    const uint32_t Mcap = std::max<uint32_t>(4u, (p.M + 3u) & ~3u);
+    uint32_t hs = p.hash_slots;
+    if (!hs) hs = std::min<uint32_t>(8192u, std::max<uint32_t>(2048u, next_pow2(16u * pl->maxL)));
+    if (pl->shared_bitmap) hs = 64;
+    pl->hash_slots = hs;
+    // Lcap rounded to 8 keys keeps every LDS region 16-byte aligned.
+    const uint32_t Lcap = (pl->maxL + 7u) & ~7u;
+    pl->lds = lds_bytes(Lcap, nb_cap, hs, Mcap);
+    if (pl->lds > 65536) {
+        HIP_TRY(hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->lds));
+    }
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, 64, pl->lds));
+    if (per_cu <= 0) return fail(PANNS_ERR_UNSUPPORTED, "kernel does not fit on a CU with this L");
+    pl->grid = std::min<uint32_t>(pl->nq, (uint32_t)per_cu * (uint32_t)h->num_cu);
+    if (pl->grid == 0) pl->grid = 1;
+    rc = ensure_scratch(h, pl->grid);
+    if (rc) return rc;
+
+    HIP_TRY(pl->qids.ensure(pl->nq));
+    HIP_TRY(hipMemcpyAsync(pl->qids.p, query_ids, pl->nq * 4ull, hipMemcpyHostToDevice, h->stream));
+    if (Lq) {
+        HIP_TRY(pl->qL.ensure(pl->nq));
+        HIP_TRY(hipMemcpyAsync(pl->qL.p, Lq, pl->nq * 4ull, hipMemcpyHostToDevice, h->stream));
+    }
+    if (Kq) {
+        HIP_TRY(pl->qK.ensure(pl->nq));
+        HIP_TRY(hipMemcpyAsync(pl->qK.p, Kq, pl->nq * 4ull, hipMemcpyHostToDevice, h->stream));
+    }
+    if (!pl->init_queue.p) {
+        HIP_TRY(pl->init_ids.ensure(pl->maxL));
+        HIP_TRY(hipMemcpyAsync(pl->init_ids.p, init_ids, pl->maxL * 4ull, hipMemcpyHostToDevice, h->stream));
+    }
+    HIP_TRY(pl->out_ids.ensure((size_t)pl->nq * pl->maxK));
+    HIP_TRY(pl->out_dists.ensure((size_t)pl->nq * pl->maxK));
+    HIP_TRY(pl->out_stats.ensure((size_t)pl->nq * 5));
+    if (pl->want_queue) {
+        HIP_TRY(pl->out_qids.ensure((size_t)pl->nq * pl->maxL));
+        HIP_TRY(pl->out_qdists.ensure((size_t)pl->nq * pl->maxL));
+        HIP_TRY(pl->out_qchk.ensure((size_t)pl->nq * pl->maxL));
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+
+    panns::KArgs &a = pl->args;
+    a = panns::KArgs{};
+    a.base = h->base.p;
+    a.adj = h->adj.p;
+    a.n = h->n;
+    a.stride_log2 = h->stride_log2;
+    a.queries = h->queries.p;
+    a.qids = pl->qids.p;
+    a.qL = Lq ? pl->qL.p : nullptr;
+    a.qK = Kq ? pl->qK.p : nullptr;
+    a.n_work = pl->nq;
+    a.L = p.L;
+    a.K = p.K;
+    a.M = p.M;
+    a.mode = p.mode;
+    a.Kstride = pl->maxK;
+    a.Lstride = pl->maxL;
+    a.init_ids = pl->init_ids.p;
+    a.work_counter = h->counter.p;
+    a.out_ids = pl->out_ids.p;
+    a.out_dists = pl->out_dists.p;
+    a.out_qids = pl->want_queue ? pl->out_qids.p : nullptr;
+    a.out_qdists = pl->want_queue ? pl->out_qdists.p : nullptr;
+    a.out_qchk = pl->want_queue ? pl->out_qchk.p : nullptr;
+    a.out_stats = pl->out_stats.p;
+    a.bitmaps = h->bitmaps.p;
+    a.bitmap_words = h->bitmap_words;
+    a.spill_log = h->spill_log.p;
+    a.log_cap = kLogCap;
+    a.Lcap = Lcap;
+    a.hash_slots = hs;
+    a.hash_shift = 32u - log2u(hs);
+    a.hash_limit = pl->shared_bitmap ? 0u : (hs / 4u) * 3u;
+    a.Mcap = Mcap;
+    a.nb_cap = nb_cap;
+    a.group_picks = std::max<uint32_t>(1u, nb_cap / stride);
+    a.init_queue = pl->init_queue.p;
+    a.shared_bitmap = pl->shared_bitmap;
+    a.stop_learned = p.stop == PANNS_STOP_LEARNED ? 1u : 0u;
+    a.stop_threshold = p.stop_threshold;
+    if (h->has_model) {
+        const float *m = h->model.p;
+        a.model.w1 = m;
+        a.model.b1 = m + 512;
+        a.model.w2 = m + 576;
+        a.model.b2 = m + 576 + 4096;
+        a.model.w3 = m + 576 + 4096 + 64;
+        a.model.b3 = m + 576 + 4096 + 128;
+        a.model.mu = m + 576 + 4096 + 132;
+        a.model.sd = m + 576 + 4096 + 140;
+    }
+    a.trace_feat = pl->trace_cap ? pl->trace_feat.p : nullptr;
+    a.trace_topk = pl->trace_cap ? pl->trace_topk.p : nullptr;
+    a.trace_cap = pl->trace_cap;
+    return PANNS_OK;
+}
+
+int run_plan(panns_plan *pl, int sync, float *ms)
+{
+    panns_index *h = pl->h;
+    LaunchFn fn;
+    KernelPtr kp;
+    kernel_for_dim(h->dim, &fn, &kp);
+    hipLaunchKernelGGL(panns::reset_counter_kernel, dim3(1), dim3(64), 0, h->stream, h->counter.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(h->ev0, h->stream));
+    HIP_TRY(fn(pl->args, pl->grid, pl->lds, h->stream));
+    HIP_TRY(hipEventRecord(h->ev1, h->stream));
+    pl->ran = true;
+    if (sync) {
+        HIP_TRY(hipEventSynchronize(h->ev1));
+        if (ms) HIP_TRY(hipEventElapsedTime(ms, h->ev0, h->ev1));
+    }
+    return PANNS_OK;
+}
+
+int fetch_plan(panns_plan *pl, panns_outputs *out, panns_counters *cnt, float kernel_ms)
+{
+    panns_index *h = pl->h;
+    if (!pl->ran) return fail(PANNS_ERR_STATE, "plan has not run");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    std::vector<uint32_t> st((size_t)pl->nq * 5);
+    HIP_TRY(hipMemcpy(st.data(), pl->out_stats.p, st.size() * 4, hipMemcpyDeviceToHost));
+    if (out) {
+        if (out->ids)
+            HIP_TRY(hipMemcpy(out->ids, pl->out_ids.p, (size_t)pl->nq * pl->maxK * 4, hipMemcpyDeviceToHost));
+        if (out->dists)
+            HIP_TRY(hipMemcpy(out->dists, pl->out_dists.p, (size_t)pl->nq * pl->maxK * 4, hipMemcpyDeviceToHost));
+        if (pl->want_queue) {
+            if (out->queue_ids)
+                HIP_TRY(hipMemcpy(out->queue_ids, pl->out_qids.p, (size_t)pl->nq * pl->maxL * 4, hipMemcpyDeviceToHost));
+            if (out->queue_dists)
+                HIP_TRY(hipMemcpy(out->queue_dists, pl->out_qdists.p, (size_t)pl->nq * pl->maxL * 4, hipMemcpyDeviceToHost));
+            if (out->queue_checked)
+                HIP_TRY(hipMemcpy(out->queue_checked, pl->out_qchk.p, (size_t)pl->nq * pl->maxL, hipMemcpyDeviceToHost));
+        }
+        for (uint32_t i = 0; i < pl->nq; ++i) {
+            if (out->ndist) out->ndist[i] = st[(size_t)i * 5 + 0];
+            if (out->nexp) out->nexp[i] = st[(size_t)i * 5 + 1];
+            if (out->niter) out->niter[i] = st[(size_t)i * 5 + 2];
+        }
+    }
+    if (cnt) {
+        for (uint32_t i = 0; i < pl->nq; ++i) {
+            const uint32_t *s = &st[(size_t)i * 5];
+            cnt->distance_computations += s[0];
+            cnt->expansions += s[1];
+            if (pl->p.mode == PANNS_MODE_TOP_M) {
+                cnt->merges += s[2];
+            } else if (pl->p.mode == PANNS_MODE_PSS_V5) {
+                // PSS5LG.cpp:1543-1673 at T=1: one sequential expansion, then
+                // rounds of up to X expansions, one count_merge_ per round.
+                const uint32_t X = pl->p.M;
+                if (s[1] > 1) cnt->merges += (s[1] - 1 + X - 1) / X;
+            }
+            cnt->visited_spills += s[3];
+            cnt->adjacency_slots += s[4];
+        }
+        cnt->kernel_ms += kernel_ms;
+    }
+    return PANNS_OK;
+}
+
+int upload_graph(panns_index *h, uint32_t width, uint32_t ep, uint32_t n, uint32_t stride_in,
+                 const uint32_t *adj, const uint32_t *deg)
+{
+    if (n != h->n)
+        return fail(PANNS_ERR_RANGE, "graph has " + std::to_string(n) + " vertices, base set has " +
+                                         std::to_string(h->n));
+    if (ep >= n) return fail(PANNS_ERR_RANGE, "entry point out of range");
+    uint32_t md = 0;
+    uint64_t ne = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        if (deg[v] > stride_in) return fail(PANNS_ERR_RANGE, "degree exceeds the row stride");
+        md = std::max(md, deg[v]);
+        ne += deg[v];
+        for (uint32_t i = 0; i < deg[v]; ++i)
+            if (adj[(size_t)v * stride_in + i] >= n) return fail(PANNS_ERR_RANGE, "neighbour id >= N");
+    }
+    const uint32_t stride = next_pow2(std::max<uint32_t>(32u, md));
+    std::vector<uint32_t> rows((size_t)n * stride, panns::kSent);
+    for (uint32_t v = 0; v < n; ++v)
+        std::memcpy(&rows[(size_t)v * stride], adj + (size_t)v * stride_in, deg[v] * 4ull);
+    HIP_TRY(h->adj.ensure(rows.size()));
+    HIP_TRY(hipMemcpy(h->adj.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
+    h->width = width;
+    h->ep = ep;
+    h->max_deg = md;
+    h->num_e = ne;
+    h->stride_log2 = log2u(stride);
+    h->ep_row.assign(adj + (size_t)ep * stride_in, adj + (size_t)ep * stride_in + deg[ep]);
+    h->has_graph = true;
+    return PANNS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *panns_last_error(void) { return g_err.c_str(); }
+
+int panns_create(int device, panns_index **out)
+{
+    if (!out) return fail(PANNS_ERR_ARG, "null out");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(PANNS_ERR_DEVICE, "no such HIP device");
+    HIP_TRY(hipSetDevice(device));
+    std::unique_ptr<panns_index> h(new panns_index());
+    h->device = device;
+    HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&h->ev0));
+    HIP_TRY(hipEventCreate(&h->ev1));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    h->num_cu = prop.multiProcessorCount;
+    *out = h.release();
+    return PANNS_OK;
+}
+
+int panns_destroy(panns_index *h)
+{
+    if (!h) return PANNS_OK;
+    (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return PANNS_OK;
+}
+
+int panns_set_base(panns_index *h, const float *data, uint32_t n, uint32_t dim)
+{
+    if (!h || !data || !n || !dim) return fail(PANNS_ERR_ARG, "bad base set");
+    if (h->queries.p && h->dim && dim != h->dim)
+        return fail(PANNS_ERR_DIM, "data dimension " + std::to_string(dim) + " is not equal to query dimension " +
+                                       std::to_string(h->dim));
+    if (dim % 8) return fail(PANNS_ERR_DIM, "dimension must be a multiple of 8 (the reference reads 8-float blocks)");
+    if (n >= 0x80000000u) return fail(PANNS_ERR_RANGE, "N must be below 2^31");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(h->base.ensure((size_t)n * dim));
+    HIP_TRY(hipMemcpy(h->base.p, data, (size_t)n * dim * 4, hipMemcpyHostToDevice));
+    if (h->n != n) h->has_graph = false;
+    h->n = n;
+    h->dim = dim;
+    return PANNS_OK;
+}
+
+int panns_set_queries(panns_index *h, const float *data, uint32_t nq, uint32_t dim)
+{
+    if (!h || !data || !nq || !dim) return fail(PANNS_ERR_ARG, "bad query set");
+    if (h->n && dim != h->dim)
+        return fail(PANNS_ERR_DIM, "query dimension " + std::to_string(dim) + " is not equal to data dimension " +
+                                       std::to_string(h->dim));
+    if (dim % 8) return fail(PANNS_ERR_DIM, "dimension must be a multiple of 8");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(h->queries.ensure((size_t)nq * dim));
+    HIP_TRY(hipMemcpy(h->queries.p, data, (size_t)nq * dim * 4, hipMemcpyHostToDevice));
+    h->nq = nq;
+    h->dim = dim;
+    return PANNS_OK;
+}
+
+int panns_load_data_load(panns_index *h, const char *path)
+{
+    if (!h || !path) return fail(PANNS_ERR_ARG, "null argument");
+    uint32_t n = 0, d = 0;
+    if (pt_fvecs_info(path, &n, &d)) return fail(PANNS_ERR_IO, pt_last_error());
+    std::vector<float> buf((size_t)n * d);
+    if (pt_fvecs_read(path, buf.data(), n, d)) return fail(PANNS_ERR_FORMAT, pt_last_error());
+    return panns_set_base(h, buf.data(), n, d);
+}
+
+int panns_load_queries_load(panns_index *h, const char *path)
+{
+    if (!h || !path) return fail(PANNS_ERR_ARG, "null argument");
+    uint32_t n = 0, d = 0;
+    if (pt_fvecs_info(path, &n, &d)) return fail(PANNS_ERR_IO, pt_last_error());
+    std::vector<float> buf((size_t)n * d);
+    if (pt_fvecs_read(path, buf.data(), n, d)) return fail(PANNS_ERR_FORMAT, pt_last_error());
+    return panns_set_queries(h, buf.data(), n, d);
+}
+
+int panns_load_common_nsg_graph(panns_index *h, const char *path)
+{
+    if (!h || !path) return fail(PANNS_ERR_ARG, "null argument");
+    if (!h->n) return fail(PANNS_ERR_STATE, "load the base set before the graph");
+    uint32_t width, ep, n, md;
+    uint64_t ne;
+    const int rc = pt_nsg_info(path, &width, &ep, &n, &ne, &md);
+    if (rc == PT_ERR_IO) return fail(PANNS_ERR_IO, pt_last_error());
+    if (rc) return fail(PANNS_ERR_FORMAT, pt_last_error());
+    const uint32_t stride = std::max<uint32_t>(md, 1u);
+    std::vector<uint32_t> adj((size_t)n * stride), deg(n);
+    if (pt_nsg_read(path, n, stride, adj.data(), deg.data())) return fail(PANNS_ERR_FORMAT, pt_last_error());
+    HIP_TRY(hipSetDevice(h->device));
+    return upload_graph(h, width, ep, n, stride, adj.data(), deg.data());
+}
+
+int panns_set_graph(panns_index *h, uint32_t width, uint32_t ep, uint32_t n, uint32_t stride,
+                    const uint32_t *adj, const uint32_t *deg)
+{
+    if (!h || !adj || !deg) return fail(PANNS_ERR_ARG, "null argument");
+    if (!h->n) return fail(PANNS_ERR_STATE, "load the base set before the graph");
+    HIP_TRY(hipSetDevice(h->device));
+    return upload_graph(h, width, ep, n, stride, adj, deg);
+}
+
+int panns_get_info(panns_index *h, uint32_t *num_v, uint32_t *dim, uint32_t *num_queries,
+                   uint32_t *width, uint32_t *ep, uint64_t *num_e)
+{
+    if (!h) return fail(PANNS_ERR_ARG, "null handle");
+    if (num_v) *num_v = h->n;
+    if (dim) *dim = h->dim;
+    if (num_queries) *num_queries = h->nq;
+    if (width) *width = h->width;
+    if (ep) *ep = h->ep;
+    if (num_e) *num_e = h->num_e;
+    return PANNS_OK;
+}
+
+int panns_prepare_init_ids(panns_index *h, uint32_t L, uint32_t *init_ids)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!init_ids) return fail(PANNS_ERR_ARG, "null init_ids");
+    if (L > h->n) return fail(PANNS_ERR_ARG, "L > number of base vectors");
+    // PSS5LG.cpp:356-410
+    std::vector<uint64_t> sel((h->n + 63) / 64, 0);
+    auto get = [&](uint32_t v) { return (sel[v >> 6] >> (v & 63)) & 1u; };
+    auto set = [&](uint32_t v) { sel[v >> 6] |= 1ull << (v & 63); };
+    uint32_t end = 0;
+    for (size_t e = 0; e < h->ep_row.size() && end < L; ++e) {
+        const uint32_t v = h->ep_row[e];
+        if (get(v)) continue;
+        set(v);
+        init_ids[end++] = v;
+    }
+    uint32_t tmp = h->ep + 1;
+    while (end < L) {
+        if (tmp == h->n) tmp = 0;
+        const uint32_t v = tmp++;
+        if (get(v)) continue;
+        set(v);
+        init_ids[end++] = v;
+    }
+    return PANNS_OK;
+}
+
+int panns_plan_create(panns_index *h, const panns_params *p, const uint32_t *query_ids, uint32_t nq,
+                      const uint32_t *L_per_query, const uint32_t *K_per_query, const uint32_t *init_ids,
+                      uint32_t n_init, panns_plan **out)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!p || !query_ids || !init_ids || !out || nq == 0) return fail(PANNS_ERR_ARG, "bad plan arguments");
+    if (!h->nq) return fail(PANNS_ERR_STATE, "queries not loaded");
+    HIP_TRY(hipSetDevice(h->device));
+    std::unique_ptr<panns_plan> pl(new panns_plan());
+    pl->h = h;
+    pl->p = *p;
+    pl->nq = nq;
+    pl->want_queue = true;
+    rc = build_plan(pl.get(), query_ids, L_per_query, K_per_query, init_ids, n_init);
+    if (rc) return rc;
+    *out = pl.release();
+    return PANNS_OK;
+}
+
+int panns_plan_run(panns_plan *plan, int sync, float *kernel_ms)
+{
+    if (!plan) return fail(PANNS_ERR_ARG, "null plan");
+    HIP_TRY(hipSetDevice(plan->h->device));
+    return run_plan(plan, sync, kernel_ms);
+}
+
+int panns_plan_fetch(panns_plan *plan, panns_outputs *out, panns_counters *cnt)
+{
+    if (!plan) return fail(PANNS_ERR_ARG, "null plan");
+    HIP_TRY(hipSetDevice(plan->h->device));
+    float ms = 0.f;
+    if (plan->ran) {
+        HIP_TRY(hipEventSynchronize(plan->h->ev1));
+        (void)hipEventElapsedTime(&ms, plan->h->ev0, plan->h->ev1);
+    }
+    return fetch_plan(plan, out, cnt, ms);
+}
+
+int panns_plan_destroy(panns_plan *plan)
+{
+    if (!plan) return PANNS_OK;
+    (void)hipSetDevice(plan->h->device);
+    (void)hipStreamSynchronize(plan->h->stream);
+    delete plan;
+    return PANNS_OK;
+}
+
+int panns_plan_describe(panns_plan *plan, uint32_t *workgroups, uint32_t *lds, uint32_t *hash_slots,
+                        uint32_t *max_L)
+{
+    if (!plan) return fail(PANNS_ERR_ARG, "null plan");
+    if (workgroups) *workgroups = plan->grid;
+    if (lds) *lds = plan->lds;
+    if (hash_slots) *hash_slots = plan->hash_slots;
+    if (max_L) *max_L = plan->maxL;
+    return PANNS_OK;
+}
+
+int panns_search(panns_index *h, const panns_params *p, const uint32_t *query_ids, uint32_t nq,
+                 const uint32_t *init_ids, uint32_t n_init, panns_outputs *out, panns_counters *cnt)
+{
+    if (!out || !out->ids) return fail(PANNS_ERR_ARG, "outputs.ids is required");
+    panns_plan *pl = nullptr;
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!p || !query_ids || !init_ids || nq == 0) return fail(PANNS_ERR_ARG, "bad search arguments");
+    if (!h->nq) return fail(PANNS_ERR_STATE, "queries not loaded");
+    HIP_TRY(hipSetDevice(h->device));
+    std::unique_ptr<panns_plan> own(new panns_plan());
+    pl = own.get();
+    pl->h = h;
+    pl->p = *p;
+    pl->nq = nq;
+    pl->want_queue = out->queue_ids || out->queue_dists || out->queue_checked;
+    rc = build_plan(pl, query_ids, nullptr, nullptr, init_ids, n_init);
+    if (rc) return rc;
+    float ms = 0.f;
+    rc = run_plan(pl, 1, &ms);
+    if (rc) return rc;
+    return fetch_plan(pl, out, cnt, ms);
+}
+
+int panns_subsearch_with_top_m(panns_index *h, uint32_t M_max, uint32_t query_id, uint32_t local_L,
+                               uint32_t *queue_ids, float *queue_dists, uint8_t *queue_checked,
+                               uint32_t *queue_size, uint64_t *visited_words, uint64_t *count_distance)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!queue_ids || !queue_dists || !queue_checked || !queue_size || !visited_words)
+        return fail(PANNS_ERR_ARG, "null argument");
+    if (*queue_size != local_L)
+        return fail(PANNS_ERR_UNSUPPORTED, "subsearch_with_top_m needs a full queue (set_L_size == local_L)");
+    if (query_id >= h->nq) return fail(PANNS_ERR_RANGE, "query id out of range");
+    panns_params p{};
+    p.K = 1;
+    p.L = local_L;
+    p.M = M_max;
+    p.mode = PANNS_MODE_SCALE_M;
+    p.num_threads = 1;
+    rc = check_params(h, &p, local_L, 1);
+    if (rc) return rc;
+    std::vector<uint64_t> keys(local_L);
+    for (uint32_t i = 0; i < local_L; ++i) {
+        if (queue_ids[i] >= h->n) return fail(PANNS_ERR_RANGE, "queue id out of range");
+        uint32_t db;
+        std::memcpy(&db, &queue_dists[i], 4);
+        keys[i] = ((uint64_t)db << 32) | queue_ids[i] | (queue_checked[i] ? panns::kChk : 0ull);
+        if (i && (keys[i] & ~panns::kChk) < (keys[i - 1] & ~panns::kChk))
+            return fail(PANNS_ERR_ARG, "queue is not sorted by (distance, id)");
+        if (queue_dists[i] < 0.f || queue_dists[i] != queue_dists[i])
+            return fail(PANNS_ERR_ARG, "queue distances must be non-negative numbers");
+    }
+    HIP_TRY(hipSetDevice(h->device));
+    const size_t words32 = ((size_t)h->n + 31) / 32;
+    DevBuf<uint32_t> bm;
+    HIP_TRY(bm.ensure(words32 + 1));
+    HIP_TRY(hipMemset(bm.p, 0, (words32 + 1) * 4));
+    HIP_TRY(hipMemcpy(bm.p, visited_words, words32 * 4, hipMemcpyHostToDevice));
+    std::unique_ptr<panns_plan> pl(new panns_plan());
+    pl->h = h;
+    pl->p = p;
+    pl->nq = 1;
+    pl->want_queue = true;
+    pl->shared_bitmap = bm.p;
+    HIP_TRY(pl->init_queue.ensure(local_L));
+    HIP_TRY(hipMemcpy(pl->init_queue.p, keys.data(), local_L * 8ull, hipMemcpyHostToDevice));
+    rc = build_plan(pl.get(), &query_id, nullptr, nullptr, nullptr, 0);
+    if (rc) return rc;
+    float ms = 0.f;
+    rc = run_plan(pl.get(), 1, &ms);
+    if (rc) return rc;
+    std::vector<uint32_t> ids(1);
+    uint32_t nd = 0;
+    panns_outputs o{};
+    o.ids = ids.data();
+    o.queue_ids = queue_ids;
+    o.queue_dists = queue_dists;
+    o.queue_checked = queue_checked;
+    o.ndist = &nd;
+    rc = fetch_plan(pl.get(), &o, nullptr, ms);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(visited_words, bm.p, words32 * 4, hipMemcpyDeviceToHost));
+    if (count_distance) *count_distance += nd;
+    return PANNS_OK;
+}
+
+}  // extern "C"

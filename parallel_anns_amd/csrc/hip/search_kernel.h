@@ -1,0 +1,565 @@
+// Device code of the NSG top-M search for CDNA4 (gfx950), one 64-lane
+// wavefront per query, persistent workgroups pulling queries from a device
+// counter.
+//
+// Per-query state lives in LDS:
+//   Q       sorted candidate queue, L entries of 64-bit keys
+//           (float bits << 32 | id) ordered exactly like Candidate::operator<
+//           (include/Candidate.h:19-27: distance, then id); bit 31 of the low
+//           word is is_checked_ (ids < 2^31), masked off for comparisons.
+//   hash    open-addressing visited set (linear probing, never deleted);
+//           once it holds `hash_limit` ids or a probe run is exhausted, new
+//           ids go to a per-workgroup global bitmap whose touched words are
+//           logged and cleared when the query ends.  An id is claimed "new"
+//           exactly once, so visited = init ids plus every vertex whose
+//           distance was computed, as boost::dynamic_bitset gives it in
+//           PSS5LG.cpp:1017-1020.
+//   newids  neighbour ids newly claimed in the current pick group
+//   cand    (key) of new neighbours that beat the current L-th key
+//   bb      one sorted 64-key batch being merged
+//   picks   the ids picked this iteration
+//
+// One iteration (TOPMLG.cpp:2106-2172): pick the first M unchecked entries
+// (pick_top_m_unchecked, TOPMLG.cpp:1172-1198), claim their unvisited
+// neighbours, compute distances, and merge the survivors.  The queue after
+// the iteration is the L smallest keys of (old queue ∪ new neighbours) with
+// old checked flags kept -- the same state the reference's sequential
+// add_into_queue calls reach, because the visited set grows by the same union
+// and the L-th distance never increases.  Distances use the reference's
+// compiled summation order (PSS5LG.cpp:525-560), so every key is bit-equal.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace panns {
+
+constexpr uint32_t kSent = 0xFFFFFFFFu;
+constexpr uint64_t kChk = 0x80000000ull;
+constexpr uint64_t kKeyMax = ~0ull;
+constexpr uint32_t kIdMask = 0x7FFFFFFFu;
+constexpr int kProbe = 16;
+
+enum : uint32_t { kModeTopM = 0, kModePssV5 = 1, kModeScaleM = 2 };
+
+// Learned-stop MLP: 8 -> 64 -> 64 -> 1 (ReLU, ReLU, sigmoid), fp32.
+constexpr int kFeat = 8;
+constexpr int kHid = 64;
+struct StopModel {
+    const float *w1;  // [64][8]
+    const float *b1;  // [64]
+    const float *w2;  // [64][64]
+    const float *b2;  // [64]
+    const float *w3;  // [64]
+    const float *b3;  // [1]
+    const float *mu;  // [8] feature centring
+    const float *sd;  // [8] feature scaling
+};
+
+struct KArgs {
+    const float *base;
+    const uint32_t *adj;  // [n][stride], unused slots = kSent
+    uint32_t n;
+    uint32_t stride_log2;
+    const float *queries;
+    const uint32_t *qids;  // [n_work]
+    const uint32_t *qL;    // [n_work] or null
+    const uint32_t *qK;    // [n_work] or null
+    uint32_t n_work;
+    uint32_t L, K, M, mode;
+    uint32_t Kstride, Lstride;
+    const uint32_t *init_ids;
+    uint32_t *work_counter;
+    uint32_t *out_ids;
+    float *out_dists;
+    uint32_t *out_qids;
+    float *out_qdists;
+    uint8_t *out_qchk;
+    uint32_t *out_stats;  // [n_work][5]: ndist, nexp, niter, nspill, nadj
+    uint32_t *bitmaps;    // [grid][bitmap_words]
+    uint64_t bitmap_words;
+    uint32_t *spill_log;  // [grid][log_cap]
+    uint32_t log_cap;
+    uint32_t Lcap, hash_slots, hash_shift, hash_limit;
+    uint32_t Mcap, nb_cap, group_picks;
+    // subsearch on a caller-held queue and visited bitmap
+    const uint64_t *init_queue;  // [n_work][L] keys, or null
+    uint32_t *shared_bitmap;     // used instead of bitmaps when non-null; never cleared
+    // learned stop
+    uint32_t stop_learned;
+    float stop_threshold;
+    StopModel model;
+    // traces for learned-stop training: per iteration 8 features + top-K hash
+    float *trace_feat;      // [n_work][trace_cap][8] or null
+    uint32_t *trace_topk;   // [n_work][trace_cap] order-free hash of the top-K id set
+    uint32_t trace_cap;
+};
+
+__device__ __forceinline__ uint64_t mk_key(float d, uint32_t id)
+{
+    return ((uint64_t)__float_as_uint(d) << 32) | (uint64_t)id;
+}
+__device__ __forceinline__ uint64_t km(uint64_t e) { return e & ~kChk; }
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t x, int m)
+{
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    lo = __shfl_xor(lo, m);
+    hi = __shfl_xor(hi, m);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Bitonic sort of one key per lane, ascending across the 64 lanes.
+__device__ __forceinline__ uint64_t sort64(uint64_t x, int lane)
+{
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint64_t y = shfl_xor64(x, j);
+            const bool up = (lane & k) == 0;
+            const bool lower = (lane & j) == 0;
+            const uint64_t mn = x < y ? x : y;
+            const uint64_t mx = x < y ? y : x;
+            x = (lower == up) ? mn : mx;
+        }
+    }
+    return x;
+}
+
+// Block order of the reference's 8-wide accumulation: the tail block first
+// when (D/8) is odd, then blocks 0, 1, ...
+template <int D>
+__device__ __forceinline__ constexpr int blk_of(int i)
+{
+    constexpr int nb = D / 8;
+    return (nb & 1) ? (i == 0 ? nb - 1 : i - 1) : i;
+}
+
+template <int D>
+__device__ __forceinline__ void load_query(const float *q, int half, float4 (&qr)[D / 8])
+{
+    const float4 *q4 = reinterpret_cast<const float4 *>(q);
+#pragma unroll
+    for (int i = 0; i < D / 8; ++i) qr[i] = q4[2 * blk_of<D>(i) + half];
+}
+
+// Squared L2 of one base row against the query, computed by a lane pair:
+// the even lane owns accumulators 0-3, the odd lane 4-7 of every 8-wide
+// block; each accumulator is an fma chain in the reference's block order and
+// the horizontal sum is ((((((a0+a1)+a2)+a3)+a4)+a5)+a6)+a7.  The exact
+// value is returned in the odd lane.
+template <int D>
+__device__ __forceinline__ float pair_dist(const float *__restrict__ base, uint32_t id, bool valid,
+                                           int half, const float4 (&qr)[D / 8])
+{
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    if (valid) {
+        const float4 *row = reinterpret_cast<const float4 *>(base + (size_t)id * D);
+        float4 v[D / 8];
+#pragma unroll
+        for (int i = 0; i < D / 8; ++i) v[i] = row[2 * blk_of<D>(i) + half];
+#pragma unroll
+        for (int i = 0; i < D / 8; ++i) {
+            const float tx = v[i].x - qr[i].x;
+            const float ty = v[i].y - qr[i].y;
+            const float tz = v[i].z - qr[i].z;
+            const float tw = v[i].w - qr[i].w;
+            a0 = __builtin_fmaf(tx, tx, a0);
+            a1 = __builtin_fmaf(ty, ty, a1);
+            a2 = __builtin_fmaf(tz, tz, a2);
+            a3 = __builtin_fmaf(tw, tw, a3);
+        }
+    }
+    const float s = ((a0 + a1) + a2) + a3;
+    const float o = __shfl_xor(s, 1);
+    return (((o + a0) + a1) + a2) + a3;
+}
+
+__device__ __forceinline__ uint32_t lb_queue(const uint64_t *Q, uint32_t S, uint64_t x)
+{
+    uint32_t lo = 0, cnt = S;
+    while (cnt > 0) {
+        const uint32_t step = cnt >> 1;
+        const uint32_t mid = lo + step;
+        if (km(Q[mid]) < x) {
+            lo = mid + 1;
+            cnt -= step + 1;
+        } else {
+            cnt = step;
+        }
+    }
+    return lo;
+}
+
+// Merge a sorted batch (one key per lane, kKeyMax past nb) into Q[0, S),
+// capacity L: Q becomes the L smallest of both.  Old entries move right in
+// place, processed from the right end so nothing is overwritten before it is
+// read.  Returns the queue position of the smallest inserted key, or kSent.
+__device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32_t L, uint64_t x,
+                                                uint32_t nb, uint64_t *bb, int lane)
+{
+    uint32_t pos = kSent;
+    bool valid = false;
+    if ((uint32_t)lane < nb) {
+        pos = lb_queue(Q, S, x) + (uint32_t)lane;
+        valid = pos < L;
+    }
+    const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
+    const uint32_t p0 = __shfl(pos, 0);
+    if (nv == 0) return kSent;
+    if (valid) bb[lane] = x;
+    __syncthreads();
+    uint32_t hi = S;
+    while (hi > p0) {
+        const uint32_t lo = (hi - p0 > 64u) ? hi - 64u : p0;
+        const uint32_t j = lo + (uint32_t)lane;
+        uint64_t e = 0;
+        uint32_t np = kSent;
+        if (j < hi) {
+            e = Q[j];
+            const uint64_t ek = km(e);
+            uint32_t c = 0, cnt = nv;
+            while (cnt > 0) {
+                const uint32_t step = cnt >> 1;
+                const uint32_t mid = c + step;
+                if (bb[mid] < ek) {
+                    c = mid + 1;
+                    cnt -= step + 1;
+                } else {
+                    cnt = step;
+                }
+            }
+            np = j + c;
+        }
+        __syncthreads();
+        if (np < L) Q[np] = e;
+        __syncthreads();
+        hi = lo;
+    }
+    if (valid) Q[pos] = x;
+    S = min(L, S + nv);
+    __syncthreads();
+    return p0;
+}
+
+struct Visited {
+    uint32_t *hash;     // LDS
+    uint32_t *counts;   // LDS: [0] ids in hash, [1] ids spilled
+    uint32_t mask, shift, limit;
+    uint32_t *bitmap;   // global
+    uint32_t *log;      // global, may be null
+    uint32_t log_cap;
+
+    // Returns true iff id was not visited before; marks it visited.
+    __device__ __forceinline__ bool claim(uint32_t id) const
+    {
+        if (limit) {
+            uint32_t slot = (id * 2654435761u) >> shift;
+#pragma unroll 1
+            for (int p = 0; p < kProbe; ++p) {
+                const uint32_t cur = hash[slot];
+                if (cur == id) return false;
+                if (cur == kSent) {
+                    if (atomicAdd(&counts[0], 0u) >= limit) break;
+                    const uint32_t old = atomicCAS(&hash[slot], kSent, id);
+                    if (old == kSent) {
+                        atomicAdd(&counts[0], 1u);
+                        return true;
+                    }
+                    if (old == id) return false;
+                }
+                slot = (slot + 1) & mask;
+            }
+        }
+        const uint32_t bit = 1u << (id & 31);
+        const uint32_t old = atomicOr(&bitmap[id >> 5], bit);
+        if (old & bit) return false;
+        const uint32_t i = atomicAdd(&counts[1], 1u);
+        if (log && i < log_cap) __hip_atomic_store(&log[i], id >> 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return true;
+    }
+};
+
+// One-wave fp32 MLP: lane j computes hidden unit j.  Returns the sigmoid
+// output in every lane.
+__device__ __forceinline__ float mlp_eval(const StopModel &m, const float (&f)[kFeat], float *h1s,
+                                          int lane)
+{
+    float h = m.b1[lane];
+#pragma unroll
+    for (int i = 0; i < kFeat; ++i) h = __builtin_fmaf(m.w1[lane * kFeat + i], (f[i] - m.mu[i]) / m.sd[i], h);
+    h = h > 0.f ? h : 0.f;
+    h1s[lane] = h;
+    __syncthreads();
+    float g = m.b2[lane];
+    const float4 *w2r = reinterpret_cast<const float4 *>(m.w2 + lane * kHid);
+    const float4 *h1v = reinterpret_cast<const float4 *>(h1s);
+#pragma unroll 4
+    for (int i = 0; i < kHid / 4; ++i) {
+        const float4 w = w2r[i];
+        const float4 x = h1v[i];
+        g = __builtin_fmaf(w.x, x.x, g);
+        g = __builtin_fmaf(w.y, x.y, g);
+        g = __builtin_fmaf(w.z, x.z, g);
+        g = __builtin_fmaf(w.w, x.w, g);
+    }
+    g = g > 0.f ? g : 0.f;
+    float o = m.w3[lane] * g;
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) o += __shfl_xor(o, s);
+    __syncthreads();
+    o += m.b3[0];
+    return 1.f / (1.f + __expf(-o));
+}
+
+template <int D>
+__global__ __launch_bounds__(64) void search_kernel(KArgs a)
+{
+    extern __shared__ uint64_t smem[];
+    const int lane = threadIdx.x;
+    const int half = lane & 1;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    uint64_t *Q = smem;
+    uint64_t *cand = Q + a.Lcap;
+    uint64_t *bb = cand + a.nb_cap;
+    uint32_t *hash = reinterpret_cast<uint32_t *>(bb + 64);
+    uint32_t *newids = hash + a.hash_slots;
+    uint32_t *picks = newids + a.nb_cap;
+    uint32_t *counts = picks + a.Mcap;     // [0] hash fill, [1] spills, [2] work item
+    float *h1s = reinterpret_cast<float *>(counts + 4);
+
+    const uint32_t stride = 1u << a.stride_log2;
+    Visited vis;
+    vis.hash = hash;
+    vis.counts = counts;
+    vis.mask = a.hash_slots - 1;
+    vis.shift = a.hash_shift;
+    vis.limit = a.hash_limit;
+    vis.bitmap = a.shared_bitmap ? a.shared_bitmap : a.bitmaps + (size_t)blockIdx.x * a.bitmap_words;
+    vis.log = a.shared_bitmap ? nullptr : a.spill_log + (size_t)blockIdx.x * a.log_cap;
+    vis.log_cap = a.log_cap;
+
+    for (;;) {
+        if (lane == 0) counts[2] = atomicAdd(a.work_counter, 1u);
+        __syncthreads();
+        const uint32_t w = counts[2];
+        if (w >= a.n_work) break;
+        const uint32_t qid = a.qids[w];
+        const uint32_t L = a.qL ? a.qL[w] : a.L;
+        const uint32_t K = a.qK ? a.qK[w] : a.K;
+
+        float4 qr[D / 8];
+        load_query<D>(a.queries + (size_t)qid * D, half, qr);
+
+        for (uint32_t i = lane; i < a.hash_slots; i += 64) hash[i] = kSent;
+        if (lane == 0) {
+            counts[0] = 0;
+            counts[1] = 0;
+        }
+        __syncthreads();
+
+        uint32_t S = 0, ndist = 0, nexp = 0, niter = 0, nadj = 0;
+        uint32_t ins_last = 0;
+        float ep_dist = 0.f;
+        if (!a.init_queue) {
+            // initialize_set_L_para (PSS5LG.cpp:1055-1096)
+            for (uint32_t c0 = 0; c0 < L; c0 += 64) {
+                const uint32_t cnt = min(64u, L - c0);
+                const uint32_t id = (uint32_t)lane < cnt ? a.init_ids[c0 + lane] : 0u;
+                if ((uint32_t)lane < cnt) vis.claim(id);
+                for (uint32_t p = 0; p < cnt; p += 32) {
+                    const uint32_t idx = p + (uint32_t)(lane >> 1);
+                    const bool valid = idx < cnt;
+                    const uint32_t vid = __shfl(id, (int)(idx & 63));
+                    const float d = pair_dist<D>(a.base, vid, valid, half, qr);
+                    if (valid && half) cand[idx] = mk_key(d, vid);
+                }
+                __syncthreads();
+                uint64_t x = (uint32_t)lane < cnt ? cand[lane] : kKeyMax;
+                x = sort64(x, lane);
+                merge_batch(Q, S, L, x, cnt, bb, lane);
+            }
+            ndist = L;
+        } else {
+            const uint64_t *iq = a.init_queue + (size_t)w * a.Lstride;
+            for (uint32_t i = lane; i < L; i += 64) Q[i] = iq[i];
+            S = L;
+            __syncthreads();
+        }
+        ep_dist = __uint_as_float((uint32_t)(Q[0] >> 32));
+
+        uint32_t k = 0;
+        uint32_t Mcur = (a.mode == kModeScaleM) ? 1u : (a.mode == kModePssV5 ? 1u : a.M);
+        for (;;) {
+            // pick_top_m_unchecked (TOPMLG.cpp:1172-1198)
+            uint32_t m = 0, last_k = 0, pos = k;
+            while (m < Mcur && pos < S) {
+                const uint32_t j = pos + (uint32_t)lane;
+                const uint64_t e = j < S ? Q[j] : kChk;
+                const bool unc = j < S && !(e & kChk);
+                const uint64_t msk = __ballot(unc);
+                const uint32_t need = Mcur - m;
+                const uint32_t rank = (uint32_t)__popcll(msk & lt);
+                const bool take = unc && rank < need;
+                if (take) {
+                    Q[j] = e | kChk;
+                    picks[m + rank] = (uint32_t)e & kIdMask;
+                }
+                const uint64_t tm = __ballot(take);
+                if (tm) {
+                    last_k = pos + 63u - (uint32_t)__clzll(tm);
+                    m += (uint32_t)__popcll(tm);
+                }
+                pos += 64;
+            }
+            __syncthreads();
+            if (m == 0) break;
+            ++niter;
+            nexp += m;
+            uint32_t nk = kSent;
+            uint32_t ins = 0;
+            for (uint32_t g0 = 0; g0 < m; g0 += a.group_picks) {
+                const uint32_t gs = min(a.group_picks, m - g0);
+                const uint32_t slots = gs << a.stride_log2;
+                uint32_t nnew = 0;
+                for (uint32_t s0 = 0; s0 < slots; s0 += 64) {
+                    const uint32_t s = s0 + (uint32_t)lane;
+                    uint32_t nb = kSent;
+                    if (s < slots) {
+                        const uint32_t pi = s >> a.stride_log2;
+                        const uint32_t off = s & (stride - 1u);
+                        nb = a.adj[((size_t)picks[g0 + pi] << a.stride_log2) + off];
+                    }
+                    bool isnew = false;
+                    if (nb != kSent) {
+                        ++nadj;
+                        isnew = vis.claim(nb);
+                    }
+                    const uint64_t msk = __ballot(isnew);
+                    if (isnew) newids[nnew + (uint32_t)__popcll(msk & lt)] = nb;
+                    nnew += (uint32_t)__popcll(msk);
+                }
+                __syncthreads();
+                ndist += nnew;
+                uint32_t ncand = 0;
+                const uint64_t bound = km(Q[S - 1]);
+                for (uint32_t p = 0; p < nnew; p += 32) {
+                    const uint32_t idx = p + (uint32_t)(lane >> 1);
+                    const bool valid = idx < nnew;
+                    const uint32_t vid = valid ? newids[idx] : 0u;
+                    const float d = pair_dist<D>(a.base, vid, valid, half, qr);
+                    const uint64_t key = mk_key(d, vid);
+                    const bool surv = valid && half && key < bound;
+                    const uint64_t msk = __ballot(surv);
+                    if (surv) cand[ncand + (uint32_t)__popcll(msk & lt)] = key;
+                    ncand += (uint32_t)__popcll(msk);
+                }
+                __syncthreads();
+                for (uint32_t b0 = 0; b0 < ncand; b0 += 64) {
+                    const uint32_t cnt = min(64u, ncand - b0);
+                    uint64_t x = (uint32_t)lane < cnt ? cand[b0 + lane] : kKeyMax;
+                    x = sort64(x, lane);
+                    const uint32_t p0 = merge_batch(Q, S, L, x, cnt, bb, lane);
+                    if (p0 != kSent) {
+                        nk = min(nk, p0);
+                        ++ins;
+                    }
+                }
+            }
+            k = (nk <= last_k) ? nk : last_k + 1u;
+            ins_last = ins;
+            if (a.mode == kModeScaleM && Mcur < a.M) Mcur <<= 1;
+
+            if (a.stop_learned || a.trace_feat) {
+                // features: step, top-1 dist, K-th dist, ratio, insertions in
+                // the last step, first-init (entry) dist, query norm, unchecked count
+                const uint32_t j0 = (uint32_t)lane;
+                uint32_t unc = 0;
+                for (uint32_t j = j0; j < S; j += 64) unc += (Q[j] & kChk) ? 0u : 1u;
+#pragma unroll
+                for (int s = 32; s > 0; s >>= 1) unc += __shfl_xor(unc, s);
+                float qn = 0.f;
+#pragma unroll
+                for (int i = 0; i < D / 8; ++i)
+                    qn += qr[i].x * qr[i].x + qr[i].y * qr[i].y + qr[i].z * qr[i].z + qr[i].w * qr[i].w;
+                qn += __shfl_xor(qn, 1);
+                float f[kFeat];
+                const float d1 = __uint_as_float((uint32_t)(Q[0] >> 32));
+                const float dk = __uint_as_float((uint32_t)(Q[min(K, S) - 1] >> 32));
+                f[0] = (float)niter;
+                f[1] = d1;
+                f[2] = dk;
+                f[3] = dk / fmaxf(d1, 1e-12f);
+                f[4] = (float)ins_last;
+                f[5] = ep_dist;
+                f[6] = sqrtf(qn);
+                f[7] = (float)unc;
+                if (a.trace_feat && niter <= a.trace_cap) {
+                    float *tf = a.trace_feat + ((size_t)w * a.trace_cap + (niter - 1)) * kFeat;
+                    if (lane < kFeat) tf[lane] = f[lane];
+                    uint32_t hsh = 0;
+                    for (uint32_t j = j0; j < K && j < S; j += 64) {
+                        uint32_t x = ((uint32_t)Q[j] & kIdMask) * 0x9E3779B1u;
+                        x ^= x >> 15;
+                        hsh += x * 0x85EBCA77u;
+                    }
+#pragma unroll
+                    for (int s = 32; s > 0; s >>= 1) hsh += __shfl_xor(hsh, s);
+                    if (lane == 0) a.trace_topk[(size_t)w * a.trace_cap + (niter - 1)] = hsh;
+                }
+                if (a.stop_learned) {
+                    const float pstop = mlp_eval(a.model, f, h1s, lane);
+                    if (pstop >= a.stop_threshold) break;
+                }
+            }
+        }
+
+        // set_K and the final queue
+        for (uint32_t i = lane; i < a.Kstride; i += 64) {
+            const bool have = i < K && i < S;
+            const uint64_t e = have ? Q[i] : kKeyMax;
+            a.out_ids[(size_t)w * a.Kstride + i] = have ? ((uint32_t)e & kIdMask) : kSent;
+            if (a.out_dists) a.out_dists[(size_t)w * a.Kstride + i] = have ? __uint_as_float((uint32_t)(e >> 32)) : __int_as_float(0x7f800000);
+        }
+        if (a.out_qids) {
+            for (uint32_t i = lane; i < L; i += 64) {
+                const uint64_t e = Q[i];
+                a.out_qids[(size_t)w * a.Lstride + i] = (uint32_t)e & kIdMask;
+                a.out_qdists[(size_t)w * a.Lstride + i] = __uint_as_float((uint32_t)(e >> 32));
+                a.out_qchk[(size_t)w * a.Lstride + i] = (e & kChk) ? 1 : 0;
+            }
+        }
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1) nadj += __shfl_xor(nadj, s);
+        const uint32_t nsp = counts[1];
+        if (lane == 0) {
+            uint32_t *st = a.out_stats + (size_t)w * 5;
+            st[0] = ndist;
+            st[1] = nexp;
+            st[2] = niter;
+            st[3] = nsp;
+            st[4] = nadj;
+        }
+        // clear the spilled bitmap words so the next query starts empty
+        if (!a.shared_bitmap && nsp) {
+            if (nsp <= a.log_cap) {
+                for (uint32_t i = lane; i < nsp; i += 64) {
+                    const uint32_t wd = __hip_atomic_load(&vis.log[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    atomicAnd(&vis.bitmap[wd], 0u);
+                }
+            } else {
+                for (uint64_t i = lane; i < a.bitmap_words; i += 64) atomicAnd(&vis.bitmap[i], 0u);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void reset_counter_kernel(uint32_t *c)
+{
+    if (threadIdx.x == 0) *c = 0;
+}
+
+}  // namespace panns

@@ -1,0 +1,103 @@
+"""Synthetic workloads of the benchmark configurations and their preparation.
+
+Every workload is a seeded, L2-normalised 96-d Gaussian mixture (sigma 0.3);
+seeds: centres 0, base 1, queries 2, training queries 3.  The graph is built
+from exact k-NN lists (ties to the lower id) by tools.build_graph; the
+ground truth is exact brute force.  Prepared inputs can be cached on disk in
+the reference's own formats (fvecs, .nsg, truth).
+"""
+import hashlib
+import json
+import os
+import time
+
+import numpy as np
+
+from . import tools
+
+SEED_CENTERS, SEED_BASE, SEED_QUERY, SEED_TRAIN = 0, 1, 2, 3
+
+CONFIGS = {
+    # C0: 100k base, 1k queries, exact 64-NN -> R <= 32
+    "c0": dict(n=100_000, nq=1_000, dim=96, comps=64, sigma=0.3, knn=64, R=32, K=100, L=100),
+    # C1: 1M base, 10k queries, exact 100-NN -> R <= 50
+    "c1": dict(n=1_000_000, nq=10_000, dim=96, comps=64, sigma=0.3, knn=100, R=50, K=100, L=100),
+    # small cases for tests
+    "t20k": dict(n=20_000, nq=200, dim=96, comps=64, sigma=0.3, knn=64, R=32, K=100, L=100),
+    "t5k": dict(n=5_000, nq=64, dim=96, comps=16, sigma=0.3, knn=32, R=24, K=50, L=64),
+}
+
+
+def log(msg):
+    print(f"[datasets {time.strftime('%H:%M:%S')}] {msg}", flush=True)
+
+
+def generate(cfg, which="base", n=None, row0=0):
+    seed = {"base": SEED_BASE, "query": SEED_QUERY, "train": SEED_TRAIN}[which]
+    if n is None:
+        n = cfg["n"] if which == "base" else cfg["nq"]
+    return tools.gen_mixture(n, cfg["dim"], cfg["comps"], cfg["sigma"], SEED_CENTERS, seed, row0=row0)
+
+
+def knn_graph(base, k, nthreads=0, gpu_knn=None):
+    """Exact k-NN of every base row (self excluded).  gpu_knn, when given, is a
+    callable(base, k) -> ids computing the same lists on the GPU."""
+    if gpu_knn is not None:
+        return gpu_knn(base, k)
+    ids, _ = tools.exact_knn(base, base, k, exclude_self=True, nthreads=nthreads)
+    return ids
+
+
+def cfg_key(cfg):
+    return hashlib.sha256(json.dumps(cfg, sort_keys=True).encode()).hexdigest()[:12]
+
+
+def prepare(name_or_cfg, cache_dir=None, nthreads=0, gpu_knn=None, truth_k=100, verbose=True):
+    """Return dict(base, queries, adj, deg, ep, gt_ids, gt_dists, cfg)."""
+    cfg = CONFIGS[name_or_cfg] if isinstance(name_or_cfg, str) else dict(name_or_cfg)
+    d = None
+    if cache_dir:
+        d = os.path.join(cache_dir, f"panns_{cfg_key(cfg)}")
+        os.makedirs(d, exist_ok=True)
+        paths = {k: os.path.join(d, f) for k, f in
+                 (("base", "base.fvecs"), ("query", "query.fvecs"), ("graph", "graph.nsg"),
+                  ("truth", "truth.bin"))}
+        if all(os.path.exists(p) for p in paths.values()):
+            if verbose:
+                log(f"cache hit {d}")
+            base = tools.fvecs_read(paths["base"])
+            queries = tools.fvecs_read(paths["query"])
+            adj, deg, ep, _ = tools.nsg_read(paths["graph"])
+            gt_ids, gt_d = tools.truth_read(paths["truth"])
+            return dict(base=base, queries=queries, adj=adj, deg=deg, ep=ep, gt_ids=gt_ids,
+                        gt_dists=gt_d, cfg=cfg, dir=d)
+    t0 = time.time()
+    base = generate(cfg, "base")
+    queries = generate(cfg, "query")
+    if verbose:
+        log(f"generated {base.shape} + {queries.shape} in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    knn = knn_graph(base, cfg["knn"], nthreads, gpu_knn)
+    if verbose:
+        log(f"exact {cfg['knn']}-NN in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    adj, deg, ep = tools.build_graph(base, knn, cfg["R"], L_build=cfg.get("L_build", 40),
+                                     C=cfg.get("C", 500), nthreads=nthreads)
+    del knn
+    if verbose:
+        log(f"graph R<={cfg['R']} avg deg {deg.mean():.2f} max {deg.max()} ep {ep} in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    if gpu_knn is not None and hasattr(gpu_knn, "truth"):
+        gt_ids, gt_d = gpu_knn.truth(base, queries, truth_k)
+    else:
+        gt_ids, gt_d = tools.exact_knn(base, queries, truth_k, nthreads=nthreads)
+    if verbose:
+        log(f"ground truth {truth_k}-NN in {time.time() - t0:.1f}s")
+    out = dict(base=base, queries=queries, adj=adj, deg=deg, ep=ep, gt_ids=gt_ids, gt_dists=gt_d,
+               cfg=cfg, dir=d)
+    if d:
+        tools.fvecs_write(paths["base"], base)
+        tools.fvecs_write(paths["query"], queries)
+        tools.nsg_write(paths["graph"], adj, deg, ep)
+        tools.truth_write(paths["truth"], gt_ids, gt_d)
+    return out

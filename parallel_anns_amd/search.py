@@ -1,0 +1,199 @@
+"""Python face of the MI355X search C-ABI (include/panns_amd.h).
+
+Index mirrors the search-path methods of the reference's PANNS::Searching
+(core/Searching.202011241615.top_m_pure_large_graph.h:142-352 and
+core/Searching.202102031939.PSS_v5.large_graph.dist_thresh.profiling.h:30-268)
+over ctypes; every call runs the HIP kernels and raises SearchError on a
+non-zero status.  There is no CPU fallback.
+"""
+import numpy as np
+
+from . import _native
+from ._native import PannsCounters, PannsOutputs, PannsParams, ptr
+
+MODES = {"top_m": 0, "pss_v5": 1, "scale_m": 2}
+STOPS = {"fixed": 0, "learned": 1}
+
+
+class SearchError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"panns error {code}: {msg}")
+        self.code = code
+
+
+def _chk(rc):
+    if rc != 0:
+        raise SearchError(rc, _native.amd().panns_last_error().decode(errors="replace"))
+
+
+def make_params(K, L, M=1, mode="top_m", num_threads=1, stop="fixed", stop_threshold=0.5,
+                hash_slots=0):
+    p = PannsParams()
+    p.K, p.L, p.M = K, L, M
+    p.mode = MODES[mode] if isinstance(mode, str) else int(mode)
+    p.num_threads = num_threads
+    p.stop = STOPS[stop] if isinstance(stop, str) else int(stop)
+    p.stop_threshold = stop_threshold
+    p.hash_slots = hash_slots
+    return p
+
+
+def counters_dict(c):
+    return {f: getattr(c, f) for f, _ in PannsCounters._fields_}
+
+
+class Plan:
+    """A resident batch: inputs in HBM, re-runnable, results fetched on demand."""
+
+    def __init__(self, index, handle, nq, max_k, max_l):
+        self.index, self.h, self.nq, self.max_k, self.max_l = index, handle, nq, max_k, max_l
+
+    def run(self, sync=True):
+        ms = _native.c_float(0.0)
+        _chk(_native.amd().panns_plan_run(self.h, 1 if sync else 0, ms))
+        return ms.value
+
+    def describe(self):
+        g, lds, hs, ml = (_native.c_u32() for _ in range(4))
+        _chk(_native.amd().panns_plan_describe(self.h, g, lds, hs, ml))
+        return {"workgroups": g.value, "lds_bytes": lds.value, "hash_slots": hs.value, "max_L": ml.value}
+
+    def fetch(self, want_queue=False):
+        nq, K, L = self.nq, self.max_k, self.max_l
+        res = {"ids": np.empty((nq, K), np.uint32), "dists": np.empty((nq, K), np.float32),
+               "ndist": np.empty(nq, np.uint32), "nexp": np.empty(nq, np.uint32),
+               "niter": np.empty(nq, np.uint32)}
+        if want_queue:
+            res["queue_ids"] = np.empty((nq, L), np.uint32)
+            res["queue_dists"] = np.empty((nq, L), np.float32)
+            res["queue_checked"] = np.empty((nq, L), np.uint8)
+        o = PannsOutputs()
+        for f, _ in PannsOutputs._fields_:
+            setattr(o, f, ptr(res[f]) if f in res else None)
+        c = PannsCounters()
+        _chk(_native.amd().panns_plan_fetch(self.h, o, c))
+        res["counters"] = counters_dict(c)
+        return res
+
+    def close(self):
+        if self.h:
+            _native.amd().panns_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Index:
+    def __init__(self, device=0):
+        lib = _native.amd()
+        h = _native.vp()
+        _chk(lib.panns_create(device, h))
+        self.h = h
+        self.lib = lib
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.panns_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # loaders (PSS5LG.cpp:14-49, :138-251)
+    def load_data_load(self, path):
+        _chk(self.lib.panns_load_data_load(self.h, str(path).encode()))
+
+    def load_queries_load(self, path):
+        _chk(self.lib.panns_load_queries_load(self.h, str(path).encode()))
+
+    def load_common_nsg_graph(self, path):
+        _chk(self.lib.panns_load_common_nsg_graph(self.h, str(path).encode()))
+
+    def set_base(self, data):
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        _chk(self.lib.panns_set_base(self.h, ptr(data), data.shape[0], data.shape[1]))
+
+    def set_queries(self, data):
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        _chk(self.lib.panns_set_queries(self.h, ptr(data), data.shape[0], data.shape[1]))
+
+    def set_graph(self, adj, deg, ep, width=None):
+        adj = np.ascontiguousarray(adj, dtype=np.uint32)
+        deg = np.ascontiguousarray(deg, dtype=np.uint32)
+        if width is None:
+            width = int(deg.max())
+        _chk(self.lib.panns_set_graph(self.h, width, ep, adj.shape[0], adj.shape[1], ptr(adj), ptr(deg)))
+
+    def info(self):
+        v = [_native.c_u32() for _ in range(5)]
+        ne = _native.c_u64()
+        _chk(self.lib.panns_get_info(self.h, *v, ne))
+        return {"num_v": v[0].value, "dim": v[1].value, "num_queries": v[2].value,
+                "width": v[3].value, "ep": v[4].value, "num_e": ne.value}
+
+    def prepare_init_ids(self, L):
+        out = np.empty(L, dtype=np.uint32)
+        _chk(self.lib.panns_prepare_init_ids(self.h, L, ptr(out)))
+        return out
+
+    def search(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, want_queue=False,
+               num_threads=1, stop="fixed", stop_threshold=0.5, hash_slots=0):
+        """Batch form of para_search_with_top_m_pure_large_graph /
+        para_search_PSS_v5_large_graph_dist_thresh_profiling (num_threads_=1)."""
+        qids = np.ascontiguousarray(np.atleast_1d(query_ids), dtype=np.uint32)
+        nq = qids.shape[0]
+        if init_ids is None:
+            init_ids = self.prepare_init_ids(L)
+        init_ids = np.ascontiguousarray(init_ids, dtype=np.uint32)
+        p = make_params(K, L, M, mode, num_threads, stop, stop_threshold, hash_slots)
+        res = {"ids": np.empty((nq, K), np.uint32), "dists": np.empty((nq, K), np.float32),
+               "ndist": np.empty(nq, np.uint32), "nexp": np.empty(nq, np.uint32),
+               "niter": np.empty(nq, np.uint32)}
+        if want_queue:
+            res["queue_ids"] = np.empty((nq, L), np.uint32)
+            res["queue_dists"] = np.empty((nq, L), np.float32)
+            res["queue_checked"] = np.empty((nq, L), np.uint8)
+        o = PannsOutputs()
+        for f, _ in PannsOutputs._fields_:
+            setattr(o, f, ptr(res[f]) if f in res else None)
+        c = PannsCounters()
+        _chk(self.lib.panns_search(self.h, p, ptr(qids), nq, ptr(init_ids), init_ids.shape[0], o, c))
+        res["counters"] = counters_dict(c)
+        return res
+
+    def plan(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, L_per_query=None,
+             K_per_query=None, stop="fixed", stop_threshold=0.5, hash_slots=0):
+        qids = np.ascontiguousarray(np.atleast_1d(query_ids), dtype=np.uint32)
+        Lq = None if L_per_query is None else np.ascontiguousarray(L_per_query, dtype=np.uint32)
+        Kq = None if K_per_query is None else np.ascontiguousarray(K_per_query, dtype=np.uint32)
+        maxL = int(Lq.max()) if Lq is not None else L
+        maxK = int(Kq.max()) if Kq is not None else K
+        if init_ids is None:
+            init_ids = self.prepare_init_ids(maxL)
+        init_ids = np.ascontiguousarray(init_ids, dtype=np.uint32)
+        p = make_params(K, L, M, mode, 1, stop, stop_threshold, hash_slots)
+        h = _native.vp()
+        _chk(self.lib.panns_plan_create(self.h, p, ptr(qids), qids.shape[0], ptr(Lq), ptr(Kq),
+                                        ptr(init_ids), init_ids.shape[0], h))
+        return Plan(self, h, qids.shape[0], maxK, maxL)
+
+    def subsearch_with_top_m(self, M_max, query_id, local_L, queue_ids, queue_dists, queue_checked,
+                             visited_words):
+        """TOPMLG.cpp:1023-1071 on a caller-held full queue; arrays are updated in place."""
+        for a, dt in ((queue_ids, np.uint32), (queue_dists, np.float32), (queue_checked, np.uint8),
+                      (visited_words, np.uint64)):
+            if a.dtype != dt or not a.flags["C_CONTIGUOUS"]:
+                raise TypeError("subsearch arrays must be C-contiguous with the documented dtypes")
+        size = _native.c_u32(local_L)
+        cnt = _native.c_u64(0)
+        _chk(self.lib.panns_subsearch_with_top_m(self.h, M_max, query_id, local_L, ptr(queue_ids),
+                                                 ptr(queue_dists), ptr(queue_checked), size,
+                                                 ptr(visited_words), cnt))
+        return cnt.value

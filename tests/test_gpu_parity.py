@@ -1,0 +1,220 @@
+"""GPU parity: the HIP search (through the C-ABI) against the CPU oracle.
+
+Bar: result ids, queue ids and distances bit-exact; distance-computation
+counts equal (the visited sets are identical).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from parallel_anns_amd import tools
+from parallel_anns_amd.search import Index, SearchError
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_run(ds, qids, K, L, M, init_ids, mode="top_m", X=1):
+    g = po.CsrGraph(ds["adj"], ds["deg"], ds["ep"])
+    out = [po.search_one(g, ds["base"], ds["queries"][q], K, L, M, init_ids, mode=mode, X=X) for q in qids]
+    return out
+
+
+def _print_case(**kw):
+    print("case", " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+@pytest.mark.parametrize("L,M", [(100, 1), (100, 2), (100, 4), (100, 8), (200, 1), (200, 8), (64, 3)])
+def test_top_m_bit_exact(small_set, gpu_index, L, M):
+    ds = small_set
+    K = min(100, L)
+    qids = np.arange(64, dtype=np.uint32)
+    _print_case(test="top_m", L=L, M=M, K=K)
+    init = gpu_index.prepare_init_ids(L)
+    g = po.CsrGraph(ds["adj"], ds["deg"], ds["ep"])
+    assert (init == po.prepare_init_ids(g, L)).all()
+    res = gpu_index.search(qids, K, L, M, init_ids=init, want_queue=True)
+    ref = oracle_run(ds, qids, K, L, M, init)
+    for i, q in enumerate(qids):
+        r = ref[i]
+        assert (res["ids"][i] == r["ids"]).all(), f"ids differ for query {q}"
+        assert (res["queue_ids"][i] == r["queue"]["id"]).all(), f"queue ids differ for query {q}"
+        assert (res["queue_dists"][i].view(np.uint32) == r["queue"]["dist"].view(np.uint32)).all()
+        assert (res["queue_checked"][i] == r["queue"]["checked"]).all()
+        assert res["ndist"][i] == r["ndist"], f"distance count differs for query {q}"
+        assert res["nexp"][i] == r["nexp"]
+        assert res["niter"][i] == r["merges"]
+    assert res["counters"]["distance_computations"] == sum(r["ndist"] for r in ref)
+    assert res["counters"]["merges"] == sum(r["merges"] for r in ref)
+
+
+@pytest.mark.parametrize("X", [1, 4, 8])
+def test_pss_v5_t1_bit_exact(small_set, gpu_index, X):
+    ds = small_set
+    K, L = 100, 100
+    qids = np.arange(32, dtype=np.uint32)
+    _print_case(test="pss_v5", X=X)
+    init = gpu_index.prepare_init_ids(L)
+    res = gpu_index.search(qids, K, L, X, mode="pss_v5", init_ids=init)
+    ref = oracle_run(ds, qids, K, L, 1, init, mode="pss_v5", X=X)
+    for i in range(len(qids)):
+        assert (res["ids"][i] == ref[i]["ids"]).all()
+        assert res["ndist"][i] == ref[i]["ndist"]
+    assert res["counters"]["merges"] == sum(r["merges"] for r in ref)
+
+
+def test_all_queries_recall_and_exact(small_set, gpu_index):
+    ds = small_set
+    K, L, M = 100, 100, 4
+    qids = np.arange(ds["queries"].shape[0], dtype=np.uint32)
+    _print_case(test="all_queries", L=L, M=M)
+    init = gpu_index.prepare_init_ids(L)
+    res = gpu_index.search(qids, K, L, M, init_ids=init)
+    g = po.CsrGraph(ds["adj"], ds["deg"], ds["ep"])
+    ids, dists, nd, total = po.search_batch(g, ds["base"], ds["queries"], K, L, M, init)
+    assert (res["ids"] == ids).all()
+    assert (res["dists"].view(np.uint32) == dists.view(np.uint32)).all()
+    assert (res["ndist"] == nd).all()
+    rec = tools.recall(ds["gt_ids"], res["ids"], L)
+    assert rec > 0.7
+
+
+def test_tiny_hash_spills_keep_results(small_set, gpu_index):
+    """A 64-slot LDS hash forces most visited ids into the global bitmap."""
+    ds = small_set
+    K, L, M = 100, 100, 2
+    qids = np.arange(48, dtype=np.uint32)
+    _print_case(test="spill", hash_slots=64)
+    init = gpu_index.prepare_init_ids(L)
+    res = gpu_index.search(qids, K, L, M, init_ids=init, hash_slots=64)
+    assert res["counters"]["visited_spills"] > 0
+    ref = gpu_index.search(qids, K, L, M, init_ids=init)
+    assert (res["ids"] == ref["ids"]).all()
+    assert (res["ndist"] == ref["ndist"]).all()
+    # a second run on the same scratch must start from clean bitmaps
+    again = gpu_index.search(qids, K, L, M, init_ids=init, hash_slots=64)
+    assert (again["ids"] == ref["ids"]).all()
+
+
+def test_uneven_plan_matches_uniform(small_set, gpu_index):
+    ds = small_set
+    rng = np.random.default_rng(5)
+    nq = 96
+    qids = np.arange(nq, dtype=np.uint32)
+    Ls = rng.choice([64, 100, 150, 220], size=nq).astype(np.uint32)
+    Ks = np.minimum(rng.choice([10, 50, 100], size=nq), Ls).astype(np.uint32)
+    _print_case(test="uneven", nq=nq)
+    maxL = int(Ls.max())
+    init = gpu_index.prepare_init_ids(maxL)
+    plan = gpu_index.plan(qids, 100, maxL, 2, init_ids=init, L_per_query=Ls, K_per_query=Ks)
+    plan.run()
+    out = plan.fetch()
+    plan.close()
+    for L in np.unique(Ls):
+        sel = np.nonzero(Ls == L)[0]
+        r = gpu_index.search(qids[sel], 100 if L >= 100 else int(L), int(L), 2, init_ids=init[:L])
+        for j, q in enumerate(sel):
+            k = int(Ks[q])
+            assert (out["ids"][q, :k] == r["ids"][j, :k]).all()
+            assert (out["ids"][q, k:] == 0xFFFFFFFF).all()
+
+
+def test_subsearch_with_top_m(small_set, gpu_index):
+    ds = small_set
+    L = 120
+    g = po.CsrGraph(ds["adj"], ds["deg"], ds["ep"])
+    init = po.prepare_init_ids(g, L)
+    for q, mmax in ((0, 1), (1, 4), (2, 16), (3, 3)):
+        _print_case(test="subsearch", q=q, M_max=mmax)
+        set_L, visited, _ = po.initialize_set_L(ds["base"], ds["queries"][q], L, init, ds["base"].shape[0])
+        ref_L, ref_vis = set_L.copy(), visited.copy()
+        nd_ref = po.subsearch_with_top_m(g, ds["base"], ds["queries"][q], mmax, L, ref_L, ref_vis)
+        qi = set_L["id"].copy()
+        qd = set_L["dist"].copy()
+        qc = set_L["checked"].copy()
+        vis = visited.copy()
+        nd = gpu_index.subsearch_with_top_m(mmax, q, L, qi, qd, qc, vis)
+        assert nd == nd_ref
+        assert (qi == ref_L["id"]).all()
+        assert (qd.view(np.uint32) == ref_L["dist"].view(np.uint32)).all()
+        assert (qc == ref_L["checked"]).all()
+        assert (vis == ref_vis).all()
+
+
+def test_edge_cases_tiny_graph():
+    """L == N (the queue holds every vertex: exact), K = 1, M > L, duplicate init ids."""
+    rng = np.random.default_rng(11)
+    n, d = 200, 32
+    base = tools.gen_mixture(n, d, 4, 0.3, 0, 7)
+    queries = tools.gen_mixture(8, d, 4, 0.3, 0, 8)
+    knn, _ = tools.exact_knn(base, base, 16, exclude_self=True)
+    adj, deg, ep = tools.build_graph(base, knn, 12)
+    idx = Index(0)
+    idx.set_base(base)
+    idx.set_queries(queries)
+    idx.set_graph(adj, deg, ep)
+    g = po.CsrGraph(adj, deg, ep)
+    qids = np.arange(8, dtype=np.uint32)
+    bf_ids, _ = po.brute_force_knn(base, queries, n)
+    cases = [(n, n, 1), (n, 10, 64), (50, 1, 1), (50, 50, 7), (64, 10, 1000), (33, 5, 2)]
+    for L, K, M in cases:
+        _print_case(test="edge", L=L, K=K, M=M)
+        init = idx.prepare_init_ids(L)
+        res = idx.search(qids, K, L, M, init_ids=init, want_queue=True)
+        for i, q in enumerate(qids):
+            r = po.search_one(g, base, queries[q], K, L, M, init)
+            assert (res["ids"][i] == r["ids"]).all()
+            assert (res["queue_ids"][i] == r["queue"]["id"]).all()
+            assert res["ndist"][i] == r["ndist"]
+            if L == n:
+                assert (res["queue_ids"][i] == bf_ids[q]).all()
+    # duplicate init ids (a caller-built list): the reference keeps both copies
+    init = np.concatenate([idx.prepare_init_ids(20), idx.prepare_init_ids(20)[:10]]).astype(np.uint32)
+    _print_case(test="edge_dup_init")
+    res = idx.search(qids, 10, 30, 2, init_ids=init, want_queue=True)
+    for i, q in enumerate(qids):
+        r = po.search_one(g, base, queries[q], 10, 30, 2, init)
+        assert (res["queue_ids"][i] == r["queue"]["id"]).all()
+        assert res["ndist"][i] == r["ndist"]
+    idx.close()
+
+
+@pytest.mark.parametrize("dim", [16, 32, 64, 128])
+def test_other_dimensions(dim):
+    n = 3000
+    base = tools.gen_mixture(n, dim, 8, 0.3, 0, 21)
+    queries = tools.gen_mixture(16, dim, 8, 0.3, 0, 22)
+    knn, _ = tools.exact_knn(base, base, 24, exclude_self=True)
+    adj, deg, ep = tools.build_graph(base, knn, 16)
+    idx = Index(0)
+    idx.set_base(base)
+    idx.set_queries(queries)
+    idx.set_graph(adj, deg, ep)
+    g = po.CsrGraph(adj, deg, ep)
+    _print_case(test="dims", dim=dim)
+    init = idx.prepare_init_ids(80)
+    res = idx.search(np.arange(16, dtype=np.uint32), 40, 80, 2, init_ids=init)
+    for q in range(16):
+        r = po.search_one(g, base, queries[q], 40, 80, 2, init)
+        assert (res["ids"][q] == r["ids"]).all()
+        assert res["ndist"][q] == r["ndist"]
+    idx.close()
+
+
+def test_error_paths(small_set, gpu_index):
+    q = np.arange(2, dtype=np.uint32)
+    init = gpu_index.prepare_init_ids(100)
+    bad = [dict(K=101, L=100, M=1), dict(K=10, L=100, M=0), dict(K=10, L=30000, M=1),
+           dict(K=10, L=100, M=1, mode="pss_v5", num_threads=4)]
+    for kw in bad:
+        with pytest.raises(SearchError):
+            gpu_index.search(q, init_ids=init, **kw)
+    with pytest.raises(SearchError):
+        gpu_index.search(np.array([10**6], dtype=np.uint32), 10, 100, 1, init_ids=init)
+    with pytest.raises(SearchError):
+        gpu_index.search(q, 10, 100, 1, init_ids=init[:50])
+    idx = Index(0)
+    with pytest.raises(SearchError):
+        idx.search(q, 10, 100, 1, init_ids=init)  # nothing loaded
+    with pytest.raises(SearchError):
+        idx.set_base(np.zeros((10, 12), np.float32))  # D not a multiple of 8
+    idx.close()
