@@ -155,6 +155,13 @@ int panns_plan_destroy(panns_plan *plan);
 int panns_plan_describe(panns_plan *plan, uint32_t *workgroups, uint32_t *lds_bytes,
                         uint32_t *hash_slots, uint32_t *max_L);
 
+/* Exact k-NN by brute force on the GPU with the reference's distance and
+ * (distance, id) order: the semantics of tools/save_true_nearest_neighbors.cpp:16-35
+ * (get_true_NN).  source 0: the base rows are the queries and each row's own
+ * id is skipped (the k-NN lists a graph is built from); source 1: the loaded
+ * queries (ground truth).  k <= 128.  out_* hold (number of queries) * k. */
+int panns_exact_knn(panns_index *h, int source, uint32_t k, uint32_t *out_ids, float *out_dists);
+
 #ifdef __cplusplus
 }
 #endif

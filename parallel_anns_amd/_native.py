@@ -83,6 +83,7 @@ AMD_API = {
     "panns_plan_fetch": (c_int, [vp, P(PannsOutputs), P(PannsCounters)]),
     "panns_plan_destroy": (c_int, [vp]),
     "panns_plan_describe": (c_int, [vp, P(c_u32), P(c_u32), P(c_u32), P(c_u32)]),
+    "panns_exact_knn": (c_int, [vp, c_int, c_u32, vp, vp]),
 }
 
 _libs = {}

@@ -12,6 +12,7 @@
 #include "panns_amd.h"
 #include "panns_tools.h"
 #include "search_kernel.h"
+#include "knn_kernel.h"
 
 namespace {
 
@@ -723,6 +724,65 @@ int panns_subsearch_with_top_m(panns_index *h, uint32_t M_max, uint32_t query_id
     if (rc) return rc;
     HIP_TRY(hipMemcpy(visited_words, bm.p, words32 * 4, hipMemcpyDeviceToHost));
     if (count_distance) *count_distance += nd;
+    return PANNS_OK;
+}
+
+int panns_exact_knn(panns_index *h, int source, uint32_t k, uint32_t *out_ids, float *out_dists)
+{
+    if (!h || !out_ids) return fail(PANNS_ERR_ARG, "null argument");
+    if (!h->n) return fail(PANNS_ERR_STATE, "base set not loaded");
+    if (source != 0 && source != 1) return fail(PANNS_ERR_ARG, "source must be 0 (base rows) or 1 (queries)");
+    if (source == 1 && !h->nq) return fail(PANNS_ERR_STATE, "queries not loaded");
+    if (k == 0 || k > (uint32_t)panns::kKnnKCap) return fail(PANNS_ERR_ARG, "k must be in [1, 128]");
+    if (k > h->n - (source == 0 ? 1u : 0u)) return fail(PANNS_ERR_ARG, "k exceeds the number of candidates");
+    HIP_TRY(hipSetDevice(h->device));
+    const float *qsrc = source == 0 ? h->base.p : h->queries.p;
+    const uint32_t nq = source == 0 ? h->n : h->nq;
+    const void *kp = nullptr;
+    void (*launch)(const float *, uint32_t, const float *, uint32_t, uint32_t, int, uint32_t, uint32_t *,
+                   float *, uint32_t, uint32_t, hipStream_t) = nullptr;
+    switch (h->dim) {
+#define PANNS_KNN_DIM(DD)                                                                             \
+    case DD:                                                                                          \
+        kp = reinterpret_cast<const void *>(&panns::knn_kernel<DD>);                                  \
+        launch = [](const float *b, uint32_t n, const float *q, uint32_t nq_, uint32_t kk, int ex,    \
+                    uint32_t qo, uint32_t *oi, float *od, uint32_t grid, uint32_t lds, hipStream_t s) { \
+            hipLaunchKernelGGL(panns::knn_kernel<DD>, dim3(grid), dim3(256), lds, s, b, n, q, nq_, kk, \
+                               ex, qo, oi, od);                                                       \
+        };                                                                                            \
+        break;
+        PANNS_KNN_DIM(16)
+        PANNS_KNN_DIM(32)
+        PANNS_KNN_DIM(64)
+        PANNS_KNN_DIM(96)
+        PANNS_KNN_DIM(128)
+#undef PANNS_KNN_DIM
+    default:
+        return fail(PANNS_ERR_DIM, "dimension has no k-NN kernel (16, 32, 64, 96, 128)");
+    }
+    const uint32_t lds = 8u * (panns::kKnnQT * panns::kKnnKCap + panns::kKnnQT * 64u + 4u * 64u) +
+                         4u * panns::kKnnRT * (h->dim + 4u) + 4u * 2u * panns::kKnnQT + 8u * panns::kKnnQT;
+    HIP_TRY(hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    // launches of at most 64k queries keep each kernel short
+    const uint32_t chunk = 65536;
+    DevBuf<uint32_t> di;
+    DevBuf<float> dd;
+    HIP_TRY(di.ensure((size_t)std::min(nq, chunk) * k));
+    HIP_TRY(dd.ensure((size_t)std::min(nq, chunk) * k));
+    for (uint32_t q0 = 0; q0 < nq; q0 += chunk) {
+        const uint32_t cn = std::min(chunk, nq - q0);
+        const uint32_t grid = (cn + panns::kKnnQT - 1) / panns::kKnnQT;
+        // q0 offsets the chunk's query index so that exclude_self compares
+        // base-row ids with global query ids.
+        launch(h->base.p, h->n, qsrc + (size_t)q0 * h->dim, cn, k, source == 0 ? 1 : 0, q0, di.p, dd.p, grid,
+               lds, h->stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out_ids + (size_t)q0 * k, di.p, (size_t)cn * k * 4, hipMemcpyDeviceToHost, h->stream));
+        if (out_dists)
+            HIP_TRY(hipMemcpyAsync(out_dists + (size_t)q0 * k, dd.p, (size_t)cn * k * 4, hipMemcpyDeviceToHost,
+                                   h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
     return PANNS_OK;
 }
 

@@ -9,6 +9,7 @@ the reference's own formats (fvecs, .nsg, truth).
 import hashlib
 import json
 import os
+import sys
 import time
 
 import numpy as np
@@ -29,7 +30,7 @@ CONFIGS = {
 
 
 def log(msg):
-    print(f"[datasets {time.strftime('%H:%M:%S')}] {msg}", flush=True)
+    print(f"[datasets {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def generate(cfg, which="base", n=None, row0=0):
@@ -46,6 +47,33 @@ def knn_graph(base, k, nthreads=0, gpu_knn=None):
         return gpu_knn(base, k)
     ids, _ = tools.exact_knn(base, base, k, exclude_self=True, nthreads=nthreads)
     return ids
+
+
+class GpuKnn:
+    """Exact k-NN lists and ground truth on the GPU (search.Index.exact_knn)."""
+
+    def __init__(self, device=0):
+        self.device = device
+
+    def __call__(self, base, k):
+        from .search import Index
+        idx = Index(self.device)
+        try:
+            idx.set_base(base)
+            ids, _ = idx.exact_knn(k, source="base")
+        finally:
+            idx.close()
+        return ids
+
+    def truth(self, base, queries, k):
+        from .search import Index
+        idx = Index(self.device)
+        try:
+            idx.set_base(base)
+            idx.set_queries(queries)
+            return idx.exact_knn(k, source="queries")
+        finally:
+            idx.close()
 
 
 def cfg_key(cfg):

@@ -184,6 +184,17 @@ class Index:
                                         ptr(init_ids), init_ids.shape[0], h))
         return Plan(self, h, qids.shape[0], maxK, maxL)
 
+    def exact_knn(self, k, source="queries"):
+        """Exact k-NN on the GPU (reference distance, ties to the lower id).
+        source="queries": of the loaded queries (ground truth); "base": of
+        every base row with itself excluded (graph k-NN lists)."""
+        info = self.info()
+        nq = info["num_queries"] if source == "queries" else info["num_v"]
+        ids = np.empty((nq, k), np.uint32)
+        dists = np.empty((nq, k), np.float32)
+        _chk(self.lib.panns_exact_knn(self.h, 1 if source == "queries" else 0, k, ptr(ids), ptr(dists)))
+        return ids, dists
+
     def subsearch_with_top_m(self, M_max, query_id, local_L, queue_ids, queue_dists, queue_checked,
                              visited_words):
         """TOPMLG.cpp:1023-1071 on a caller-held full queue; arrays are updated in place."""
