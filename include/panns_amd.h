@@ -162,6 +162,17 @@ int panns_plan_describe(panns_plan *plan, uint32_t *workgroups, uint32_t *lds_by
  * queries (ground truth).  k <= 128.  out_* hold (number of queries) * k. */
 int panns_exact_knn(panns_index *h, int source, uint32_t k, uint32_t *out_ids, float *out_dists);
 
+/* Measurement helpers (no reference counterpart).
+ * panns_plan_phase_cycles: shader-clock cycles the last run spent per search
+ * phase, summed over workgroups: [init, pick, claim, distance, merge, stop,
+ * output, idle].  Only libraries built with PANNS_PHASE_TIMERS record them;
+ * others return PANNS_ERR_UNSUPPORTED.
+ * panns_device_copy: copies `bytes` (a multiple of 16) between two device
+ * buffers with a plain streaming kernel, so hardware byte counters can be
+ * scaled against a kernel of known traffic; *ms gets its device time. */
+int panns_plan_phase_cycles(panns_plan *plan, uint64_t *out8);
+int panns_device_copy(panns_index *h, uint64_t bytes, float *ms);
+
 #ifdef __cplusplus
 }
 #endif

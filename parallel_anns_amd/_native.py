@@ -8,7 +8,8 @@ import os
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 TOOLS_SO = os.path.join(_PKG, "libpanns_tools.so")
-AMD_SO = os.path.join(_PKG, "libpanns_amd.so")
+# PANNS_AMD_SO selects an instrumented build of the same library.
+AMD_SO = os.environ.get("PANNS_AMD_SO", os.path.join(_PKG, "libpanns_amd.so"))
 
 c_u32 = ctypes.c_uint32
 c_u64 = ctypes.c_uint64
@@ -84,6 +85,8 @@ AMD_API = {
     "panns_plan_destroy": (c_int, [vp]),
     "panns_plan_describe": (c_int, [vp, P(c_u32), P(c_u32), P(c_u32), P(c_u32)]),
     "panns_exact_knn": (c_int, [vp, c_int, c_u32, vp, vp]),
+    "panns_plan_phase_cycles": (c_int, [vp, vp]),
+    "panns_device_copy": (c_int, [vp, c_u64, P(c_float)]),
 }
 
 _libs = {}

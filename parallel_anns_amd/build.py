@@ -52,12 +52,14 @@ def build_tools():
     os.replace(tmp, TOOLS_SO)
 
 
-def build_hip():
+def build_hip(out=AMD_SO, defines=()):
+    """out/defines build instrumented variants (e.g. PANNS_PHASE_TIMERS) that
+    PANNS_AMD_SO can point the bindings at; they link the package's tools."""
     src = os.path.join(CSRC, "hip", "panns_amd.hip")
-    tmp = AMD_SO + ".tmp"
-    _run([hipcc(), *HIP_FLAGS, "-I", INC, "-I", os.path.join(CSRC, "hip"), src,
-          "-L", PKG, "-lpanns_tools", "-Wl,-rpath,$ORIGIN", "-o", tmp])
-    os.replace(tmp, AMD_SO)
+    tmp = out + ".tmp"
+    _run([hipcc(), *HIP_FLAGS, *[f"-D{d}" for d in defines], "-I", INC, "-I", os.path.join(CSRC, "hip"),
+          src, "-L", PKG, "-lpanns_tools", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{PKG}", "-o", tmp])
+    os.replace(tmp, out)
 
 
 def build_all():

@@ -67,7 +67,7 @@ uint32_t log2u(uint32_t x)
 
 constexpr uint32_t kMaxL = 4096;
 constexpr uint32_t kMaxM = 1024;
-constexpr uint32_t kLogCap = 8192;
+constexpr uint32_t kLogCap = 16384;
 
 }  // namespace
 
@@ -109,6 +109,7 @@ struct panns_plan {
     bool want_queue = false;
     uint32_t *shared_bitmap = nullptr;
     uint32_t grid = 0, lds = 0, hash_slots = 0;
+    DevBuf<uint64_t> phase_cycles;
     panns::KArgs args{};
     bool ran = false;
 };
@@ -171,10 +172,12 @@ int check_params(panns_index *h, const panns_params *p, uint32_t L, uint32_t K)
     return PANNS_OK;
 }
 
-// Layout of one workgroup's LDS; must match search_kernel's carving.
-uint32_t lds_bytes(uint32_t Lcap, uint32_t nb_cap, uint32_t hash_slots, uint32_t Mcap)
+// Layout of one workgroup's LDS; must match search_kernel's carving:
+// Q[Lcap] cand[nb_cap] bb[64] (8 B each), query row (4*dim B),
+// hash[hash_slots] newids[nb_cap] picks[Mcap] counts[4] h1[64] (4 B each).
+uint32_t lds_bytes(uint32_t Lcap, uint32_t nb_cap, uint32_t hash_slots, uint32_t Mcap, uint32_t dim)
 {
-    return 8u * (Lcap + nb_cap + 64u) + 4u * (hash_slots + nb_cap + Mcap + 4u) + 4u * 64u;
+    return 8u * (Lcap + nb_cap + 64u) + 4u * dim + 4u * (hash_slots + nb_cap + Mcap + 4u) + 4u * 64u;
 }
 
 int ensure_scratch(panns_index *h, uint32_t grid)
@@ -226,15 +229,15 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
             if (init_ids[i] >= h->n) return fail(PANNS_ERR_RANGE, "init id out of range");
 
     const uint32_t stride = 1u << h->stride_log2;
-    const uint32_t nb_cap = std::max<uint32_t>(256u, stride);
+    const uint32_t nb_cap = std::max<uint32_t>(panns::kGroupSlots, stride);
     const uint32_t Mcap = std::max<uint32_t>(4u, (p.M + 3u) & ~3u);
     uint32_t hs = p.hash_slots;
-    if (!hs) hs = std::min<uint32_t>(8192u, std::max<uint32_t>(2048u, next_pow2(16u * pl->maxL)));
+    if (!hs) hs = pl->maxL <= 512 ? 2048u : 4096u;
     if (pl->shared_bitmap) hs = 64;
     pl->hash_slots = hs;
     // Lcap rounded to 8 keys keeps every LDS region 16-byte aligned.
     const uint32_t Lcap = (pl->maxL + 7u) & ~7u;
-    pl->lds = lds_bytes(Lcap, nb_cap, hs, Mcap);
+    pl->lds = lds_bytes(Lcap, nb_cap, hs, Mcap, h->dim);
     if (pl->lds > 65536) {
         HIP_TRY(hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->lds));
     }
@@ -263,6 +266,10 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     HIP_TRY(pl->out_ids.ensure((size_t)pl->nq * pl->maxK));
     HIP_TRY(pl->out_dists.ensure((size_t)pl->nq * pl->maxK));
     HIP_TRY(pl->out_stats.ensure((size_t)pl->nq * 5));
+#ifdef PANNS_PHASE_TIMERS
+    HIP_TRY(pl->phase_cycles.ensure((size_t)pl->grid * panns::kPhases));
+    HIP_TRY(hipMemsetAsync(pl->phase_cycles.p, 0, (size_t)pl->grid * panns::kPhases * 8, h->stream));
+#endif
     if (pl->want_queue) {
         HIP_TRY(pl->out_qids.ensure((size_t)pl->nq * pl->maxL));
         HIP_TRY(pl->out_qdists.ensure((size_t)pl->nq * pl->maxL));
@@ -324,6 +331,7 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     a.trace_feat = pl->trace_cap ? pl->trace_feat.p : nullptr;
     a.trace_topk = pl->trace_cap ? pl->trace_topk.p : nullptr;
     a.trace_cap = pl->trace_cap;
+    a.phase_cycles = pl->phase_cycles.p;
     return PANNS_OK;
 }
 
@@ -724,6 +732,47 @@ int panns_subsearch_with_top_m(panns_index *h, uint32_t M_max, uint32_t query_id
     if (rc) return rc;
     HIP_TRY(hipMemcpy(visited_words, bm.p, words32 * 4, hipMemcpyDeviceToHost));
     if (count_distance) *count_distance += nd;
+    return PANNS_OK;
+}
+
+int panns_plan_phase_cycles(panns_plan *plan, uint64_t *out8)
+{
+    if (!plan || !out8) return fail(PANNS_ERR_ARG, "null argument");
+#ifdef PANNS_PHASE_TIMERS
+    HIP_TRY(hipSetDevice(plan->h->device));
+    HIP_TRY(hipStreamSynchronize(plan->h->stream));
+    std::vector<uint64_t> c((size_t)plan->grid * panns::kPhases);
+    HIP_TRY(hipMemcpy(c.data(), plan->phase_cycles.p, c.size() * 8, hipMemcpyDeviceToHost));
+    for (int i = 0; i < panns::kPhases; ++i) {
+        out8[i] = 0;
+        for (uint32_t g = 0; g < plan->grid; ++g) out8[i] += c[(size_t)g * panns::kPhases + i];
+    }
+    return PANNS_OK;
+#else
+    return fail(PANNS_ERR_UNSUPPORTED, "library built without PANNS_PHASE_TIMERS");
+#endif
+}
+
+int panns_device_copy(panns_index *h, uint64_t bytes, float *ms)
+{
+    if (!h || bytes == 0 || bytes % 16) return fail(PANNS_ERR_ARG, "bytes must be a positive multiple of 16");
+    HIP_TRY(hipSetDevice(h->device));
+    DevBuf<uint4> src, dst;
+    const uint64_t n16 = bytes / 16;
+    HIP_TRY(src.ensure(n16));
+    HIP_TRY(dst.ensure(n16));
+    HIP_TRY(hipMemsetAsync(src.p, 1, bytes, h->stream));
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n16 + 255) / 256, 8192);
+    hipLaunchKernelGGL(panns::copy_kernel, dim3(grid), dim3(256), 0, h->stream, src.p, dst.p, n16);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(h->ev0, h->stream));
+    hipLaunchKernelGGL(panns::copy_kernel, dim3(grid), dim3(256), 0, h->stream, src.p, dst.p, n16);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(h->ev1, h->stream));
+    HIP_TRY(hipEventSynchronize(h->ev1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, h->ev0, h->ev1));
+    if (ms) *ms = t;
     return PANNS_OK;
 }
 

@@ -14,8 +14,9 @@
 //           exactly once, so visited = init ids plus every vertex whose
 //           distance was computed, as boost::dynamic_bitset gives it in
 //           PSS5LG.cpp:1017-1020.
+//   qs      the query row
 //   newids  neighbour ids newly claimed in the current pick group
-//   cand    (key) of new neighbours that beat the current L-th key
+//   cand    keys of new neighbours that beat the current L-th key
 //   bb      one sorted 64-key batch being merged
 //   picks   the ids picked this iteration
 //
@@ -27,6 +28,11 @@
 // add_into_queue calls reach, because the visited set grows by the same union
 // and the L-th distance never increases.  Distances use the reference's
 // compiled summation order (PSS5LG.cpp:525-560), so every key is bit-equal.
+//
+// Latency structure: a group of picks has all its adjacency rows loaded in
+// one batch, its visited claims resolved in LDS and then with one batch of
+// global atomics, and its distances computed 64 rows per pass with both
+// 32-row halves in flight together.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,6 +45,8 @@ constexpr uint64_t kChk = 0x80000000ull;
 constexpr uint64_t kKeyMax = ~0ull;
 constexpr uint32_t kIdMask = 0x7FFFFFFFu;
 constexpr int kProbe = 16;
+constexpr int kClaimChunks = 8;      // 64-slot chunks claimed per batch
+constexpr uint32_t kGroupSlots = 512;  // neighbour slots per pick group
 
 enum : uint32_t { kModeTopM = 0, kModePssV5 = 1, kModeScaleM = 2 };
 
@@ -93,7 +101,39 @@ struct KArgs {
     float *trace_feat;      // [n_work][trace_cap][8] or null
     uint32_t *trace_topk;   // [n_work][trace_cap] order-free hash of the top-K id set
     uint32_t trace_cap;
+    // per-workgroup cycle counts of the search phases (builds with
+    // PANNS_PHASE_TIMERS only): [grid][kPhases]
+    uint64_t *phase_cycles;
 };
+
+// Phases timed by PANNS_PHASE_TIMERS builds.
+enum : int { kPhInit = 0, kPhPick, kPhClaim, kPhDist, kPhMerge, kPhStop, kPhOut, kPhIdle, kPhases };
+
+#ifdef PANNS_PHASE_TIMERS
+#define PANNS_PT_DECL                      \
+    uint64_t pt_acc_[kPhases] = {};        \
+    uint64_t pt_t_ = clock64();
+#define PANNS_PT(ph)                       \
+    do {                                   \
+        const uint64_t n_ = clock64();     \
+        pt_acc_[ph] += n_ - pt_t_;         \
+        pt_t_ = n_;                        \
+    } while (0)
+#define PANNS_PT_FLUSH(a)                                                              \
+    do {                                                                               \
+        if (threadIdx.x == 0 && (a).phase_cycles)                                      \
+            for (int i_ = 0; i_ < kPhases; ++i_)                                       \
+                (a).phase_cycles[(size_t)blockIdx.x * kPhases + i_] = pt_acc_[i_];     \
+    } while (0)
+#else
+#define PANNS_PT_DECL
+#define PANNS_PT(ph) \
+    do {             \
+    } while (0)
+#define PANNS_PT_FLUSH(a) \
+    do {                  \
+    } while (0)
+#endif
 
 __device__ __forceinline__ uint64_t mk_key(float d, uint32_t id)
 {
@@ -136,44 +176,54 @@ __device__ __forceinline__ constexpr int blk_of(int i)
     return (nb & 1) ? (i == 0 ? nb - 1 : i - 1) : i;
 }
 
+// Squared L2 of two base rows (A and B) against the query, each computed by
+// a lane pair: the even lane owns accumulators 0-3, the odd lane 4-7 of every
+// 8-wide block; each accumulator is an fma chain in the reference's block
+// order and the horizontal sum is ((((((a0+a1)+a2)+a3)+a4)+a5)+a6)+a7.  Both
+// rows' loads are issued before either is reduced.  The exact values are
+// returned in the odd lane.  qs is the query row in LDS.
 template <int D>
-__device__ __forceinline__ void load_query(const float *q, int half, float4 (&qr)[D / 8])
+__device__ __forceinline__ void pair_dist2(const float *__restrict__ base, uint32_t idA, bool vA,
+                                           uint32_t idB, bool vB, int half, const float4 *qs,
+                                           float &dA, float &dB)
 {
-    const float4 *q4 = reinterpret_cast<const float4 *>(q);
+    constexpr int NB = D / 8;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 *ra = reinterpret_cast<const float4 *>(base + (size_t)idA * D);
+    const float4 *rb = reinterpret_cast<const float4 *>(base + (size_t)idB * D);
+    // Loads are unconditional: callers pass a valid row (id 0) for inactive
+    // lanes and discard those results, because exec-masked loads make the
+    // compiler wait for each one where the branches join.
+    (void)vA;
+    (void)vB;
+    (void)z;
+    float4 va[NB], vb[NB];
 #pragma unroll
-    for (int i = 0; i < D / 8; ++i) qr[i] = q4[2 * blk_of<D>(i) + half];
-}
-
-// Squared L2 of one base row against the query, computed by a lane pair:
-// the even lane owns accumulators 0-3, the odd lane 4-7 of every 8-wide
-// block; each accumulator is an fma chain in the reference's block order and
-// the horizontal sum is ((((((a0+a1)+a2)+a3)+a4)+a5)+a6)+a7.  The exact
-// value is returned in the odd lane.
-template <int D>
-__device__ __forceinline__ float pair_dist(const float *__restrict__ base, uint32_t id, bool valid,
-                                           int half, const float4 (&qr)[D / 8])
-{
+    for (int i = 0; i < NB; ++i) va[i] = ra[2 * blk_of<D>(i) + half];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) vb[i] = rb[2 * blk_of<D>(i) + half];
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-    if (valid) {
-        const float4 *row = reinterpret_cast<const float4 *>(base + (size_t)id * D);
-        float4 v[D / 8];
+    float b0 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f;
 #pragma unroll
-        for (int i = 0; i < D / 8; ++i) v[i] = row[2 * blk_of<D>(i) + half];
-#pragma unroll
-        for (int i = 0; i < D / 8; ++i) {
-            const float tx = v[i].x - qr[i].x;
-            const float ty = v[i].y - qr[i].y;
-            const float tz = v[i].z - qr[i].z;
-            const float tw = v[i].w - qr[i].w;
-            a0 = __builtin_fmaf(tx, tx, a0);
-            a1 = __builtin_fmaf(ty, ty, a1);
-            a2 = __builtin_fmaf(tz, tz, a2);
-            a3 = __builtin_fmaf(tw, tw, a3);
-        }
+    for (int i = 0; i < NB; ++i) {
+        const float4 q = qs[2 * blk_of<D>(i) + half];
+        const float tx = va[i].x - q.x, ty = va[i].y - q.y, tz = va[i].z - q.z, tw = va[i].w - q.w;
+        a0 = __builtin_fmaf(tx, tx, a0);
+        a1 = __builtin_fmaf(ty, ty, a1);
+        a2 = __builtin_fmaf(tz, tz, a2);
+        a3 = __builtin_fmaf(tw, tw, a3);
+        const float ux = vb[i].x - q.x, uy = vb[i].y - q.y, uz = vb[i].z - q.z, uw = vb[i].w - q.w;
+        b0 = __builtin_fmaf(ux, ux, b0);
+        b1 = __builtin_fmaf(uy, uy, b1);
+        b2 = __builtin_fmaf(uz, uz, b2);
+        b3 = __builtin_fmaf(uw, uw, b3);
     }
-    const float s = ((a0 + a1) + a2) + a3;
-    const float o = __shfl_xor(s, 1);
-    return (((o + a0) + a1) + a2) + a3;
+    const float sa = ((a0 + a1) + a2) + a3;
+    const float sb = ((b0 + b1) + b2) + b3;
+    const float oa = __shfl_xor(sa, 1);
+    const float ob = __shfl_xor(sb, 1);
+    dA = (((oa + a0) + a1) + a2) + a3;
+    dB = (((ob + b0) + b1) + b2) + b3;
 }
 
 __device__ __forceinline__ uint32_t lb_queue(const uint64_t *Q, uint32_t S, uint64_t x)
@@ -195,9 +245,11 @@ __device__ __forceinline__ uint32_t lb_queue(const uint64_t *Q, uint32_t S, uint
 // Merge a sorted batch (one key per lane, kKeyMax past nb) into Q[0, S),
 // capacity L: Q becomes the L smallest of both.  Old entries move right in
 // place, processed from the right end so nothing is overwritten before it is
-// read.  Returns the queue position of the smallest inserted key, or kSent.
+// read.  Returns the queue position of the smallest inserted key, or kSent;
+// adds the number of inserted keys to *ninserted when given.
 __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32_t L, uint64_t x,
-                                                uint32_t nb, uint64_t *bb, int lane)
+                                                uint32_t nb, uint64_t *bb, int lane,
+                                                uint32_t *ninserted = nullptr)
 {
     uint32_t pos = kSent;
     bool valid = false;
@@ -208,6 +260,7 @@ __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32
     const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
     const uint32_t p0 = __shfl(pos, 0);
     if (nv == 0) return kSent;
+    if (ninserted) *ninserted += nv;
     if (valid) bb[lane] = x;
     __syncthreads();
     uint32_t hi = S;
@@ -243,43 +296,100 @@ __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32
     return p0;
 }
 
+// Visited set of one query: LDS hash first, per-workgroup global bitmap for
+// what the hash does not hold.  fill and spills are wave-uniform counters kept
+// in registers (counted with ballots, so no lane contends on one address).
 struct Visited {
     uint32_t *hash;     // LDS
-    uint32_t *counts;   // LDS: [0] ids in hash, [1] ids spilled
     uint32_t mask, shift, limit;
     uint32_t *bitmap;   // global
-    uint32_t *log;      // global, may be null
+    uint32_t *log;      // global: words touched in the bitmap, may be null
     uint32_t log_cap;
+    uint32_t fill;      // ids stored in the hash
+    uint32_t spills;    // ids stored in the bitmap
 
-    // Returns true iff id was not visited before; marks it visited.
-    __device__ __forceinline__ bool claim(uint32_t id) const
+    // 0: visited before; 1: new and now in the hash; 2: not in the hash, which
+    // takes no more ids on this probe run -- the bitmap decides.
+    __device__ __forceinline__ int probe(uint32_t id, bool allow) const
     {
-        if (limit) {
-            uint32_t slot = (id * 2654435761u) >> shift;
+        uint32_t slot = (id * 2654435761u) >> shift;
 #pragma unroll 1
-            for (int p = 0; p < kProbe; ++p) {
-                const uint32_t cur = hash[slot];
-                if (cur == id) return false;
-                if (cur == kSent) {
-                    if (atomicAdd(&counts[0], 0u) >= limit) break;
-                    const uint32_t old = atomicCAS(&hash[slot], kSent, id);
-                    if (old == kSent) {
-                        atomicAdd(&counts[0], 1u);
-                        return true;
-                    }
-                    if (old == id) return false;
-                }
-                slot = (slot + 1) & mask;
+        for (int p = 0; p < kProbe; ++p) {
+            const uint32_t cur = hash[slot];
+            if (cur == id) return 0;
+            if (cur == kSent) {
+                if (!allow) return 2;
+                const uint32_t old = atomicCAS(&hash[slot], kSent, id);
+                if (old == kSent) return 1;
+                if (old == id) return 0;
             }
+            slot = (slot + 1) & mask;
         }
-        const uint32_t bit = 1u << (id & 31);
-        const uint32_t old = atomicOr(&bitmap[id >> 5], bit);
-        if (old & bit) return false;
-        const uint32_t i = atomicAdd(&counts[1], 1u);
-        if (log && i < log_cap) __hip_atomic_store(&log[i], id >> 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return true;
+        return 2;
     }
 };
+
+// Claims up to kClaimChunks ids per lane (kSent = empty slot): hash probes
+// chunk by chunk, then one batch of bitmap atomics for the ids the hash could
+// not take.  Appends the newly visited ids to newids[nnew...] in chunk order
+// and returns the new count.  An id offered twice is claimed once: within a
+// chunk by the hash CAS or the atomic, across chunks because the hash only
+// grows and the bitmap atomics are exact.
+// `okbits` marks the chunks where this lane's value is a slot at all; the
+// values themselves are only read here, so the loads that produced them stay
+// in flight until the first probe.
+__device__ __forceinline__ uint32_t claim_chunks(Visited &vis, uint32_t (&ids)[kClaimChunks],
+                                                 uint32_t okbits, uint32_t nch, uint32_t *newids,
+                                                 uint32_t nnew, uint64_t lt, uint32_t &nadj)
+{
+    uint32_t nbits = 0, gbits = 0;
+#pragma unroll
+    for (int u = 0; u < kClaimChunks; ++u) {
+        if ((uint32_t)u < nch) {
+            if (!((okbits >> u) & 1u)) ids[u] = kSent;
+            int r = 0;
+            if (ids[u] != kSent) {
+                ++nadj;
+                r = vis.probe(ids[u], vis.fill < vis.limit);
+            }
+            vis.fill += (uint32_t)__popcll(__ballot(r == 1));
+            if (r == 1) nbits |= 1u << u;
+            if (r == 2) gbits |= 1u << u;
+        }
+    }
+    // Every lane issues the atomic of a chunk that any lane needs, so the
+    // batch is not split by exec-mask joins; lanes with nothing to claim OR 0
+    // into a word of their own (their id's word, or word `lane` for an empty
+    // slot), which changes nothing.
+    uint32_t old[kClaimChunks];
+#pragma unroll
+    for (int u = 0; u < kClaimChunks; ++u) {
+        old[u] = 0;
+        const bool g = (gbits >> u) & 1u;
+        if (__ballot(g)) {
+            const uint32_t wi = ids[u] != kSent ? ids[u] >> 5 : (uint32_t)(threadIdx.x & 63);
+            old[u] = atomicOr(&vis.bitmap[wi], g ? 1u << (ids[u] & 31) : 0u);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kClaimChunks; ++u) {
+        if ((uint32_t)u < nch) {
+            const bool g = ((gbits >> u) & 1u) && !(old[u] & (1u << (ids[u] & 31)));
+            const uint64_t gm = __ballot(g);
+            if (g) {
+                nbits |= 1u << u;
+                const uint32_t i = vis.spills + (uint32_t)__popcll(gm & lt);
+                if (vis.log && i < vis.log_cap) vis.log[i] = ids[u] >> 5;
+            }
+            vis.spills += (uint32_t)__popcll(gm);
+            const bool nw = (nbits >> u) & 1u;
+            const uint64_t nm = __ballot(nw);
+            if (nw) newids[nnew + (uint32_t)__popcll(nm & lt)] = ids[u];
+            nnew += (uint32_t)__popcll(nm);
+        }
+    }
+    return nnew;
+}
 
 // One-wave fp32 MLP: lane j computes hidden unit j.  Returns the sigmoid
 // output in every lane.
@@ -313,69 +423,103 @@ __device__ __forceinline__ float mlp_eval(const StopModel &m, const float (&f)[k
     return 1.f / (1.f + __expf(-o));
 }
 
+// Distances for ids[0, cnt) (in LDS), 64 per pass; keys below `bound` are
+// appended to cand[ncand...].  Returns the new ncand.
 template <int D>
-__global__ __launch_bounds__(64) void search_kernel(KArgs a)
+__device__ __forceinline__ uint32_t distance_pass(const float *__restrict__ base, const uint32_t *ids,
+                                                  uint32_t cnt, const float4 *qs, uint64_t bound,
+                                                  uint64_t *cand, uint32_t ncand, int lane, uint64_t lt)
+{
+    const int half = lane & 1;
+    for (uint32_t p = 0; p < cnt; p += 64) {
+        const uint32_t ia = p + (uint32_t)(lane >> 1), ib = ia + 32;
+        const bool va = ia < cnt, vb = ib < cnt;
+        const uint32_t ra = ids[va ? ia : 0u], rb = ids[vb ? ib : 0u];
+        const uint32_t ida = va ? ra : 0u, idb = vb ? rb : 0u;
+        float da, db;
+        pair_dist2<D>(base, ida, va, idb, vb, half, qs, da, db);
+        const uint64_t ka = mk_key(da, ida), kb = mk_key(db, idb);
+        const bool sa = va && half && ka < bound;
+        const bool sb = vb && half && kb < bound;
+        const uint64_t ma = __ballot(sa);
+        if (sa) cand[ncand + (uint32_t)__popcll(ma & lt)] = ka;
+        ncand += (uint32_t)__popcll(ma);
+        const uint64_t mb = __ballot(sb);
+        if (sb) cand[ncand + (uint32_t)__popcll(mb & lt)] = kb;
+        ncand += (uint32_t)__popcll(mb);
+    }
+    return ncand;
+}
+
+template <int D>
+__global__ __launch_bounds__(64, 3) void search_kernel(KArgs a)
 {
     extern __shared__ uint64_t smem[];
     const int lane = threadIdx.x;
-    const int half = lane & 1;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
     uint64_t *Q = smem;
     uint64_t *cand = Q + a.Lcap;
     uint64_t *bb = cand + a.nb_cap;
-    uint32_t *hash = reinterpret_cast<uint32_t *>(bb + 64);
+    float4 *qs = reinterpret_cast<float4 *>(bb + 64);
+    uint32_t *hash = reinterpret_cast<uint32_t *>(qs + D / 4);
     uint32_t *newids = hash + a.hash_slots;
     uint32_t *picks = newids + a.nb_cap;
-    uint32_t *counts = picks + a.Mcap;     // [0] hash fill, [1] spills, [2] work item
+    uint32_t *counts = picks + a.Mcap;     // [2] work item
     float *h1s = reinterpret_cast<float *>(counts + 4);
 
     const uint32_t stride = 1u << a.stride_log2;
     Visited vis;
     vis.hash = hash;
-    vis.counts = counts;
     vis.mask = a.hash_slots - 1;
     vis.shift = a.hash_shift;
     vis.limit = a.hash_limit;
     vis.bitmap = a.shared_bitmap ? a.shared_bitmap : a.bitmaps + (size_t)blockIdx.x * a.bitmap_words;
     vis.log = a.shared_bitmap ? nullptr : a.spill_log + (size_t)blockIdx.x * a.log_cap;
     vis.log_cap = a.log_cap;
+    PANNS_PT_DECL
 
     for (;;) {
         if (lane == 0) counts[2] = atomicAdd(a.work_counter, 1u);
         __syncthreads();
         const uint32_t w = counts[2];
+        PANNS_PT(kPhIdle);
         if (w >= a.n_work) break;
         const uint32_t qid = a.qids[w];
         const uint32_t L = a.qL ? a.qL[w] : a.L;
         const uint32_t K = a.qK ? a.qK[w] : a.K;
 
-        float4 qr[D / 8];
-        load_query<D>(a.queries + (size_t)qid * D, half, qr);
-
+        if (lane < D / 4) qs[lane] = reinterpret_cast<const float4 *>(a.queries + (size_t)qid * D)[lane];
         for (uint32_t i = lane; i < a.hash_slots; i += 64) hash[i] = kSent;
-        if (lane == 0) {
-            counts[0] = 0;
-            counts[1] = 0;
-        }
+        vis.fill = 0;
+        vis.spills = 0;
         __syncthreads();
 
         uint32_t S = 0, ndist = 0, nexp = 0, niter = 0, nadj = 0;
         uint32_t ins_last = 0;
         float ep_dist = 0.f;
         if (!a.init_queue) {
-            // initialize_set_L_para (PSS5LG.cpp:1055-1096)
+            // initialize_set_L_para (PSS5LG.cpp:1055-1096): mark every init id
+            // visited, compute its distance, sort the L keys.
+            for (uint32_t c0 = 0; c0 < L; c0 += 64 * kClaimChunks) {
+                uint32_t ids[kClaimChunks];
+                uint32_t okbits = 0;
+                const uint32_t nch = min((uint32_t)kClaimChunks, (L - c0 + 63) / 64);
+#pragma unroll
+                for (int u = 0; u < kClaimChunks; ++u) {
+                    const uint32_t i = c0 + (uint32_t)u * 64 + lane;
+                    const bool ok = (uint32_t)u < nch && i < L;
+                    ids[u] = a.init_ids[ok ? i : 0u];
+                    okbits |= ok ? 1u << u : 0u;
+                }
+                uint32_t dummy = 0;
+                claim_chunks(vis, ids, okbits, nch, newids, 0, lt, dummy);
+            }
             for (uint32_t c0 = 0; c0 < L; c0 += 64) {
                 const uint32_t cnt = min(64u, L - c0);
-                const uint32_t id = (uint32_t)lane < cnt ? a.init_ids[c0 + lane] : 0u;
-                if ((uint32_t)lane < cnt) vis.claim(id);
-                for (uint32_t p = 0; p < cnt; p += 32) {
-                    const uint32_t idx = p + (uint32_t)(lane >> 1);
-                    const bool valid = idx < cnt;
-                    const uint32_t vid = __shfl(id, (int)(idx & 63));
-                    const float d = pair_dist<D>(a.base, vid, valid, half, qr);
-                    if (valid && half) cand[idx] = mk_key(d, vid);
-                }
+                const uint32_t n0 = distance_pass<D>(a.base, a.init_ids + c0, cnt, qs, kKeyMax, cand, 0,
+                                                     lane, lt);
+                (void)n0;
                 __syncthreads();
                 uint64_t x = (uint32_t)lane < cnt ? cand[lane] : kKeyMax;
                 x = sort64(x, lane);
@@ -389,6 +533,7 @@ __global__ __launch_bounds__(64) void search_kernel(KArgs a)
             __syncthreads();
         }
         ep_dist = __uint_as_float((uint32_t)(Q[0] >> 32));
+        PANNS_PT(kPhInit);
 
         uint32_t k = 0;
         uint32_t Mcur = (a.mode == kModeScaleM) ? 1u : (a.mode == kModePssV5 ? 1u : a.M);
@@ -415,6 +560,7 @@ __global__ __launch_bounds__(64) void search_kernel(KArgs a)
                 pos += 64;
             }
             __syncthreads();
+            PANNS_PT(kPhPick);
             if (m == 0) break;
             ++niter;
             nexp += m;
@@ -424,49 +570,37 @@ __global__ __launch_bounds__(64) void search_kernel(KArgs a)
                 const uint32_t gs = min(a.group_picks, m - g0);
                 const uint32_t slots = gs << a.stride_log2;
                 uint32_t nnew = 0;
-                for (uint32_t s0 = 0; s0 < slots; s0 += 64) {
-                    const uint32_t s = s0 + (uint32_t)lane;
-                    uint32_t nb = kSent;
-                    if (s < slots) {
-                        const uint32_t pi = s >> a.stride_log2;
-                        const uint32_t off = s & (stride - 1u);
-                        nb = a.adj[((size_t)picks[g0 + pi] << a.stride_log2) + off];
+                for (uint32_t c0 = 0; c0 < slots; c0 += 64 * kClaimChunks) {
+                    uint32_t ids[kClaimChunks];
+                    uint32_t okbits = 0;
+                    const uint32_t nch = min((uint32_t)kClaimChunks, (slots - c0 + 63) / 64);
+#pragma unroll
+                    for (int u = 0; u < kClaimChunks; ++u) {
+                        // unconditional load (slot 0 of the group for unused
+                        // lanes) so all chunks are in flight together
+                        const uint32_t s = c0 + (uint32_t)u * 64 + lane;
+                        const bool ok = (uint32_t)u < nch && s < slots;
+                        const uint32_t sc = ok ? s : 0u;
+                        ids[u] = a.adj[((size_t)picks[g0 + (sc >> a.stride_log2)] << a.stride_log2) +
+                                       (sc & (stride - 1u))];
+                        okbits |= ok ? 1u << u : 0u;
                     }
-                    bool isnew = false;
-                    if (nb != kSent) {
-                        ++nadj;
-                        isnew = vis.claim(nb);
-                    }
-                    const uint64_t msk = __ballot(isnew);
-                    if (isnew) newids[nnew + (uint32_t)__popcll(msk & lt)] = nb;
-                    nnew += (uint32_t)__popcll(msk);
+                    nnew = claim_chunks(vis, ids, okbits, nch, newids, nnew, lt, nadj);
                 }
                 __syncthreads();
+                PANNS_PT(kPhClaim);
                 ndist += nnew;
-                uint32_t ncand = 0;
-                const uint64_t bound = km(Q[S - 1]);
-                for (uint32_t p = 0; p < nnew; p += 32) {
-                    const uint32_t idx = p + (uint32_t)(lane >> 1);
-                    const bool valid = idx < nnew;
-                    const uint32_t vid = valid ? newids[idx] : 0u;
-                    const float d = pair_dist<D>(a.base, vid, valid, half, qr);
-                    const uint64_t key = mk_key(d, vid);
-                    const bool surv = valid && half && key < bound;
-                    const uint64_t msk = __ballot(surv);
-                    if (surv) cand[ncand + (uint32_t)__popcll(msk & lt)] = key;
-                    ncand += (uint32_t)__popcll(msk);
-                }
+                const uint32_t ncand = distance_pass<D>(a.base, newids, nnew, qs, km(Q[S - 1]), cand, 0, lane, lt);
                 __syncthreads();
+                PANNS_PT(kPhDist);
                 for (uint32_t b0 = 0; b0 < ncand; b0 += 64) {
                     const uint32_t cnt = min(64u, ncand - b0);
                     uint64_t x = (uint32_t)lane < cnt ? cand[b0 + lane] : kKeyMax;
                     x = sort64(x, lane);
-                    const uint32_t p0 = merge_batch(Q, S, L, x, cnt, bb, lane);
-                    if (p0 != kSent) {
-                        nk = min(nk, p0);
-                        ++ins;
-                    }
+                    const uint32_t p0 = merge_batch(Q, S, L, x, cnt, bb, lane, &ins);
+                    if (p0 != kSent) nk = min(nk, p0);
                 }
+                PANNS_PT(kPhMerge);
             }
             k = (nk <= last_k) ? nk : last_k + 1u;
             ins_last = ins;
@@ -475,16 +609,17 @@ __global__ __launch_bounds__(64) void search_kernel(KArgs a)
             if (a.stop_learned || a.trace_feat) {
                 // features: step, top-1 dist, K-th dist, ratio, insertions in
                 // the last step, first-init (entry) dist, query norm, unchecked count
-                const uint32_t j0 = (uint32_t)lane;
                 uint32_t unc = 0;
-                for (uint32_t j = j0; j < S; j += 64) unc += (Q[j] & kChk) ? 0u : 1u;
+                for (uint32_t j = lane; j < S; j += 64) unc += (Q[j] & kChk) ? 0u : 1u;
 #pragma unroll
                 for (int s = 32; s > 0; s >>= 1) unc += __shfl_xor(unc, s);
                 float qn = 0.f;
+                if (lane < D / 4) {
+                    const float4 q = qs[lane];
+                    qn = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
+                }
 #pragma unroll
-                for (int i = 0; i < D / 8; ++i)
-                    qn += qr[i].x * qr[i].x + qr[i].y * qr[i].y + qr[i].z * qr[i].z + qr[i].w * qr[i].w;
-                qn += __shfl_xor(qn, 1);
+                for (int s = 32; s > 0; s >>= 1) qn += __shfl_xor(qn, s);
                 float f[kFeat];
                 const float d1 = __uint_as_float((uint32_t)(Q[0] >> 32));
                 const float dk = __uint_as_float((uint32_t)(Q[min(K, S) - 1] >> 32));
@@ -500,7 +635,7 @@ __global__ __launch_bounds__(64) void search_kernel(KArgs a)
                     float *tf = a.trace_feat + ((size_t)w * a.trace_cap + (niter - 1)) * kFeat;
                     if (lane < kFeat) tf[lane] = f[lane];
                     uint32_t hsh = 0;
-                    for (uint32_t j = j0; j < K && j < S; j += 64) {
+                    for (uint32_t j = lane; j < K && j < S; j += 64) {
                         uint32_t x = ((uint32_t)Q[j] & kIdMask) * 0x9E3779B1u;
                         x ^= x >> 15;
                         hsh += x * 0x85EBCA77u;
@@ -513,6 +648,7 @@ __global__ __launch_bounds__(64) void search_kernel(KArgs a)
                     const float pstop = mlp_eval(a.model, f, h1s, lane);
                     if (pstop >= a.stop_threshold) break;
                 }
+                PANNS_PT(kPhStop);
             }
         }
 
@@ -533,7 +669,7 @@ __global__ __launch_bounds__(64) void search_kernel(KArgs a)
         }
 #pragma unroll
         for (int s = 32; s > 0; s >>= 1) nadj += __shfl_xor(nadj, s);
-        const uint32_t nsp = counts[1];
+        const uint32_t nsp = vis.spills;
         if (lane == 0) {
             uint32_t *st = a.out_stats + (size_t)w * 5;
             st[0] = ndist;
@@ -542,19 +678,32 @@ __global__ __launch_bounds__(64) void search_kernel(KArgs a)
             st[3] = nsp;
             st[4] = nadj;
         }
-        // clear the spilled bitmap words so the next query starts empty
+        // clear the bitmap words this query set, so the next query starts
+        // empty; every atomic above has returned, so plain stores suffice
         if (!a.shared_bitmap && nsp) {
             if (nsp <= a.log_cap) {
                 for (uint32_t i = lane; i < nsp; i += 64) {
                     const uint32_t wd = __hip_atomic_load(&vis.log[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    atomicAnd(&vis.bitmap[wd], 0u);
+                    vis.bitmap[wd] = 0u;
                 }
             } else {
-                for (uint64_t i = lane; i < a.bitmap_words; i += 64) atomicAnd(&vis.bitmap[i], 0u);
+                for (uint64_t i = lane; i < a.bitmap_words; i += 64) vis.bitmap[i] = 0u;
             }
+            __threadfence();
         }
         __syncthreads();
+        PANNS_PT(kPhOut);
     }
+    PANNS_PT_FLUSH(a);
+}
+
+// Streaming copy used to calibrate hardware byte counters against a kernel
+// of known traffic (n16 16-byte elements read and written once).
+__global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n16)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 __global__ void reset_counter_kernel(uint32_t *c)

@@ -75,6 +75,14 @@ class Plan:
         res["counters"] = counters_dict(c)
         return res
 
+    PHASES = ("init", "pick", "claim", "distance", "merge", "stop", "output", "idle")
+
+    def phase_cycles(self):
+        """Cycles per search phase of the last run (timer builds only)."""
+        out = np.zeros(8, np.uint64)
+        _chk(_native.amd().panns_plan_phase_cycles(self.h, ptr(out)))
+        return dict(zip(self.PHASES, (int(x) for x in out)))
+
     def close(self):
         if self.h:
             _native.amd().panns_plan_destroy(self.h)
@@ -183,6 +191,12 @@ class Index:
         _chk(self.lib.panns_plan_create(self.h, p, ptr(qids), qids.shape[0], ptr(Lq), ptr(Kq),
                                         ptr(init_ids), init_ids.shape[0], h))
         return Plan(self, h, qids.shape[0], maxK, maxL)
+
+    def device_copy(self, nbytes):
+        """Device time (ms) of a plain streaming copy of nbytes."""
+        ms = _native.c_float()
+        _chk(self.lib.panns_device_copy(self.h, nbytes, ms))
+        return ms.value
 
     def exact_knn(self, k, source="queries"):
         """Exact k-NN on the GPU (reference distance, ties to the lower id).
