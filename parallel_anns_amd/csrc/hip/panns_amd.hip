@@ -136,6 +136,7 @@ bool kernel_for_dim(uint32_t dim, LaunchFn *fn, KernelPtr *kp)
         return true;
         PANNS_DIM(16)
         PANNS_DIM(32)
+        PANNS_DIM(40)
         PANNS_DIM(64)
         PANNS_DIM(96)
         PANNS_DIM(128)
@@ -205,7 +206,7 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     LaunchFn fn;
     KernelPtr kp;
     if (!kernel_for_dim(h->dim, &fn, &kp))
-        return fail(PANNS_ERR_DIM, "dimension " + std::to_string(h->dim) + " has no kernel (16, 32, 64, 96, 128)");
+        return fail(PANNS_ERR_DIM, "dimension " + std::to_string(h->dim) + " has no kernel (16, 32, 40, 64, 96, 128)");
     pl->maxL = p.L;
     pl->maxK = p.K;
     if (Lq || Kq) {
@@ -802,12 +803,13 @@ int panns_exact_knn(panns_index *h, int source, uint32_t k, uint32_t *out_ids, f
         break;
         PANNS_KNN_DIM(16)
         PANNS_KNN_DIM(32)
+        PANNS_KNN_DIM(40)
         PANNS_KNN_DIM(64)
         PANNS_KNN_DIM(96)
         PANNS_KNN_DIM(128)
 #undef PANNS_KNN_DIM
     default:
-        return fail(PANNS_ERR_DIM, "dimension has no k-NN kernel (16, 32, 64, 96, 128)");
+        return fail(PANNS_ERR_DIM, "dimension has no k-NN kernel (16, 32, 40, 64, 96, 128)");
     }
     const uint32_t lds = 8u * (panns::kKnnQT * panns::kKnnKCap + panns::kKnnQT * 64u + 4u * 64u) +
                          4u * panns::kKnnRT * (h->dim + 4u) + 4u * 2u * panns::kKnnQT + 8u * panns::kKnnQT;

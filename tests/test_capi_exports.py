@@ -1,0 +1,51 @@
+"""The C-ABI libraries load on a host without a GPU and export every entry
+point their headers declare (no compute calls are made here)."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header, prefix):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(" + prefix + r"\w+)\s*\(", text)))
+
+
+def test_amd_library_exports_header():
+    from parallel_anns_amd import _native
+    lib = ctypes.CDLL(_native.AMD_SO)
+    names = declared("panns_amd.h", "panns_")
+    assert "panns_search" in names and "panns_subsearch_with_top_m" in names
+    for n in names:
+        assert hasattr(lib, n), n
+    # and the Python bindings cover all of them
+    assert set(names) <= set(_native.AMD_API)
+
+
+def test_tools_library_exports_header():
+    from parallel_anns_amd import _native
+    lib = ctypes.CDLL(_native.TOOLS_SO)
+    names = declared("panns_tools.h", "pt_")
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) <= set(_native.TOOLS_API)
+
+
+def test_oracle_exports_header():
+    import pyoracle
+    lib = pyoracle.lib()
+    text = open(os.path.join(ROOT, "oracle", "panns_oracle.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    for n in set(re.findall(r"\b(po_\w+)\s*\(", text)):
+        assert hasattr(lib, n), n
+
+
+def test_product_does_not_import_the_oracle():
+    pkg = os.path.join(ROOT, "parallel_anns_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".h")):
+                src = open(os.path.join(dirpath, f)).read()
+                assert "pyoracle" not in src and "panns_oracle" not in src, f

@@ -1,0 +1,147 @@
+"""Golden fixtures (tests/golden/fixtures.json) and the host tools around the
+search path: generator reproducibility, file formats, exact k-NN, graphs."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from parallel_anns_amd import datasets, tools
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = json.load(open(os.path.join(HERE, "golden", "fixtures.json")))
+
+
+def f32(bits):
+    return np.asarray(bits, np.uint32).view(np.float32)
+
+
+def test_distance_fixtures_oracle_and_package():
+    for rec in FIX["distance"]:
+        v, q = f32(rec["v"]), f32(rec["q"])
+        assert int(np.float32(po.distance(v, q)).view(np.uint32)) == rec["d_bits"]
+        assert int(np.float32(tools.l2(v, q)).view(np.uint32)) == rec["d_bits"]
+
+
+def test_generator_is_bit_reproducible():
+    for rec in FIX["generator"]:
+        x = tools.gen_mixture(rec["n"], rec["dim"], rec["comps"], rec["sigma"], rec["seed_centers"],
+                              rec["seed_rows"], row0=rec["row0"])
+        assert hashlib.sha256(x.tobytes()).hexdigest() == rec["sha256"]
+        assert x[0, :4].view(np.uint32).tolist() == rec["first"]
+
+
+def test_generator_properties():
+    x = tools.gen_mixture(2000, 96, 64, 0.3, 0, 1)
+    assert np.allclose(np.linalg.norm(x, axis=1), 1.0, atol=1e-5)
+    # rows are independent of how they are batched
+    y = tools.gen_mixture(100, 96, 64, 0.3, 0, 1, row0=1900)
+    assert (y == x[1900:]).all()
+    assert (tools.gen_mixture(10, 96, 64, 0.3, 0, 1, nthreads=1) == x[:10]).all()
+    # same centres, different row seed -> different rows, same cluster structure
+    q = tools.gen_mixture(500, 96, 64, 0.3, 0, 2)
+    assert not (q[:10] == x[:10]).all()
+    nn = (x @ q.T).max(axis=0)
+    assert np.median(nn) > 0.8  # queries lie in the base clusters
+
+
+def test_tiny_search_fixture_oracle():
+    t = FIX["tiny_search"]
+    base = f32(t["base"]).reshape(t["n"], t["dim"])
+    queries = f32(t["queries"]).reshape(-1, t["dim"])
+    adj = np.asarray(t["adj"], np.uint32)
+    deg = np.full(t["n"], adj.shape[1], np.uint32)
+    g = po.CsrGraph(adj, deg, t["ep"])
+    init = po.prepare_init_ids(g, t["L"])
+    assert init.tolist() == t["init_ids"]
+    for c in t["cases"]:
+        r = po.search_one(g, base, queries[c["query"]], t["K"], t["L"], c["M"], init)
+        assert r["queue"]["id"].tolist() == c["queue_ids"]
+        assert r["queue"]["dist"].view(np.uint32).tolist() == c["queue_dist_bits"]
+        assert r["ndist"] == c["ndist"]
+
+
+def test_fvecs_nsg_truth_roundtrip(tmp_path):
+    x = tools.gen_mixture(50, 16, 3, 0.3, 0, 9)
+    p = tmp_path / "x.fvecs"
+    tools.fvecs_write(p, x)
+    assert (tools.fvecs_read(p) == x).all()
+    assert os.path.getsize(p) == 50 * (4 + 16 * 4)
+    adj = np.full((5, 4), 0xFFFFFFFF, np.uint32)
+    deg = np.array([2, 0, 4, 1, 3], np.uint32)
+    for v in range(5):
+        adj[v, :deg[v]] = [(v + j + 1) % 5 for j in range(deg[v])]
+    g = tmp_path / "g.nsg"
+    tools.nsg_write(g, adj, deg, 2, width=4)
+    raw = np.fromfile(g, np.uint32)
+    assert raw[:2].tolist() == [4, 2] and raw[2] == 2 and raw[3:5].tolist() == [1, 2]
+    a2, d2, ep, w = tools.nsg_read(g)
+    assert ep == 2 and w == 4 and (d2 == deg).all()
+    for v in range(5):
+        assert (a2[v, :deg[v]] == adj[v, :deg[v]]).all()
+    ids = np.arange(12, dtype=np.uint32).reshape(2, 6)
+    ds = np.linspace(0, 1, 12, dtype=np.float32).reshape(2, 6)
+    t = tmp_path / "t.bin"
+    tools.truth_write(t, ids, ds)
+    i2, d3 = tools.truth_read(t)
+    assert (i2 == ids).all() and (d3 == ds).all()
+
+
+def test_format_errors(tmp_path):
+    bad = tmp_path / "bad.nsg"
+    np.array([4, 0, 3, 1, 2], np.uint32).tofile(bad)  # degree 3 but only 2 ids
+    with pytest.raises(tools.ToolsError):
+        tools.nsg_read(bad)
+    badf = tmp_path / "bad.fvecs"
+    np.array([4, 1, 2, 3], np.int32).tofile(badf)  # header says 4 floats, 3 present
+    with pytest.raises(tools.ToolsError):
+        tools.fvecs_read(badf)
+    with pytest.raises(tools.ToolsError):
+        tools.fvecs_read(tmp_path / "missing.fvecs")
+
+
+def test_exact_knn_matches_oracle_and_ties():
+    base = tools.gen_mixture(1500, 32, 6, 0.3, 0, 11)
+    queries = tools.gen_mixture(40, 32, 6, 0.3, 0, 12)
+    ids, ds = tools.exact_knn(base, queries, 50)
+    oi, od = po.brute_force_knn(base, queries, 50)
+    assert (ids == oi).all() and (ds.view(np.uint32) == od.view(np.uint32)).all()
+    dup = np.concatenate([base[:30], base[:30]])
+    i2, _ = tools.exact_knn(dup, base[:5], 4)
+    for q in range(5):
+        assert i2[q, :2].tolist() == [q, q + 30]  # equal distances: lower id first
+    self_ids, _ = tools.exact_knn(base, base, 10, exclude_self=True)
+    assert not (self_ids == np.arange(1500)[:, None]).any()
+
+
+def test_graph_properties(tiny_set):
+    adj, deg, ep = tiny_set["adj"], tiny_set["deg"], tiny_set["ep"]
+    n, R = adj.shape[0], tiny_set["cfg"]["R"]
+    assert deg.max() <= R
+    assert tools.reachable(adj, deg, ep) == n
+    for v in range(n):
+        row = adj[v, :deg[v]]
+        assert v not in row
+        assert len(set(row.tolist())) == deg[v]
+    assert ep == tools.navigating_node(tiny_set["base"])
+
+
+def test_recall_closed_form():
+    gt = np.array([[1, 2, 3, 4], [5, 6, 7, 8]], np.uint32)
+    res = np.array([[4, 3, 9, 9], [5, 0, 0, 6]], np.uint32)
+    assert tools.recall(gt, res, L=4, R=4) == pytest.approx((2 + 2) / 8)
+    assert tools.recall(gt, res, L=1, R=4) == pytest.approx((1 + 1) / 8)  # only res[:, 0] counts
+    assert po.lib().po_recall(gt.ctypes.data, 4, res.ctypes.data, 4, 2, 4, 4) == pytest.approx(0.5)
+    assert tools.recall_per_query(gt, res, 4, 4).tolist() == [0.5, 0.5]
+
+
+def test_cpu_search_recall_sanity(tiny_set):
+    ds = tiny_set
+    g = po.CsrGraph(ds["adj"], ds["deg"], ds["ep"])
+    init = po.prepare_init_ids(g, 64)
+    ids, _, nd, _ = po.search_batch(g, ds["base"], ds["queries"], 50, 64, 2, init)
+    assert tools.recall(ds["gt_ids"], ids, 64, R=50) > 0.9
+    # the config table is what the benchmark uses
+    assert datasets.CONFIGS["c1"]["n"] == 1_000_000 and datasets.CONFIGS["c1"]["R"] == 50

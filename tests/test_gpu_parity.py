@@ -178,7 +178,60 @@ def test_edge_cases_tiny_graph():
     idx.close()
 
 
-@pytest.mark.parametrize("dim", [16, 32, 64, 128])
+def _fixtures():
+    import json
+    import os
+    return json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fixtures.json")))
+
+
+def test_golden_distance_bits_on_gpu():
+    """The fixture rows v of one dimension form the base set and their partners
+    q the queries; with L = N the final queue holds every row with the kernel's
+    distance, and row i's distance to query i must carry the fixture's bits."""
+    fx = _fixtures()
+    for dim in sorted({r["dim"] for r in fx["distance"]}):
+        if dim not in (16, 32, 40, 64, 96, 128):
+            continue
+        recs = [r for r in fx["distance"] if r["dim"] == dim]
+        base = np.stack([np.asarray(r["v"], np.uint32).view(np.float32) for r in recs])
+        queries = np.stack([np.asarray(r["q"], np.uint32).view(np.float32) for r in recs])
+        n = base.shape[0]
+        adj = np.array([[j for j in range(n) if j != i] for i in range(n)], np.uint32)
+        deg = np.full(n, n - 1, np.uint32)
+        idx = Index(0)
+        idx.set_base(base)
+        idx.set_queries(queries)
+        idx.set_graph(adj, deg, 0)
+        _print_case(test="golden_distance", dim=dim)
+        res = idx.search(np.arange(n, dtype=np.uint32), 1, n, 1, want_queue=True)
+        for i in range(n):
+            pos = int(np.nonzero(res["queue_ids"][i] == i)[0][0])
+            assert int(res["queue_dists"][i][pos].view(np.uint32)) == recs[i]["d_bits"], (dim, i)
+        idx.close()
+
+
+def test_golden_tiny_search_on_gpu():
+    t = _fixtures()["tiny_search"]
+    base = np.asarray(t["base"], np.uint32).view(np.float32).reshape(t["n"], t["dim"])
+    queries = np.asarray(t["queries"], np.uint32).view(np.float32).reshape(-1, t["dim"])
+    adj = np.asarray(t["adj"], np.uint32)
+    deg = np.full(t["n"], adj.shape[1], np.uint32)
+    idx = Index(0)
+    idx.set_base(base)
+    idx.set_queries(queries)
+    idx.set_graph(adj, deg, t["ep"])
+    init = idx.prepare_init_ids(t["L"])
+    assert init.tolist() == t["init_ids"]
+    for c in t["cases"]:
+        _print_case(test="golden_tiny", M=c["M"], q=c["query"])
+        r = idx.search(np.array([c["query"]], np.uint32), t["K"], t["L"], c["M"], init_ids=init, want_queue=True)
+        assert r["queue_ids"][0].tolist() == c["queue_ids"]
+        assert r["queue_dists"][0].view(np.uint32).tolist() == c["queue_dist_bits"]
+        assert int(r["ndist"][0]) == c["ndist"]
+    idx.close()
+
+
+@pytest.mark.parametrize("dim", [16, 32, 40, 64, 128])
 def test_other_dimensions(dim):
     n = 3000
     base = tools.gen_mixture(n, dim, 8, 0.3, 0, 21)
