@@ -1,17 +1,27 @@
-"""Benchmark: QPS at recall@100 >= 0.95 on 1M x 96 fp32 (one MI355X).
+"""Benchmark: QPS at recall@100 = 0.95 on 1M x 96 fp32, learned stop vs
+fixed L, one MI355X.
 
-Workload: 1,000,000 x 96 float32 base from the seeded normalised 64-component
-Gaussian mixture (sigma 0.3), 10,000 queries from the same mixture, NSG-style
-graph from exact 100-NN lists with out-degree <= 50, K = 100.  A step is one
-pass of the search over all 10,000 queries with inputs resident in HBM (one
-persistent-kernel launch).  The operating point (M, L) is the fastest setting
-whose recall@100 reaches the target, found by bisection on L per M, as the
-reference's find-L driver and result selection do
-(app/profile_find_L_para_single_query_top_m_search_pure_large_graph.cpp:210-260,
-scripts/output_find_runtime_above_presicion.py:40-60).
+Workload: 1,000,000 x 96 float32 base from the seeded normalised
+64-component Gaussian mixture (sigma 0.3), 10,000 test queries from the same
+mixture (another seed), graph from exact 100-NN lists pruned by RNG occlusion
+to out-degree <= 50 and stitched from the navigating node, K = 100.
 
-Prints one JSON line.  --gpus above 1 exits with a message: the build targets
-one GPU (queries shard across GPUs with no exchange; see the design notes).
+A step is one pass of the search over all 10,000 test queries with inputs
+resident in HBM (one persistent-kernel launch).
+
+  fixed L   the operating point is the fastest (M, L) whose recall reaches the
+            target, L found by bisection per M as the reference's find-L
+            driver does (app/profile_find_L_para_single_query_top_m_search_
+            pure_large_graph.cpp:210-260; scripts/output_find_runtime_above_
+            presicion.py:40-60 picks the fastest setting above the target).
+  learned   the stop MLP is trained in the run on the GPU (BF16-MFMA kernels)
+            from traces of the plain search at L = 400 over seed-disjoint
+            training queries; its threshold is calibrated on held-out training
+            queries; the test queries are only used for the timed runs.
+
+`value` is the learned-stop QPS; the fixed-L QPS, their ratio and the bound
+any per-query stop rule could reach on the measured L grid are reported
+beside it.  --gpus above 1 exits with a message: this build targets one GPU.
 """
 import argparse
 import json
@@ -25,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "QPS @ recall@100=0.95 (1M×96 fp32): learned-stop vs fixed-L, 1 MI355X"
 
 
 def log(msg):
@@ -32,21 +43,28 @@ def log(msg):
 
 
 def algorithmic_bytes(res, dim, K, nq):
-    """B_q = 4*D*n_dist + 4*(n_exp + sum of expanded degrees) + 4*D + 8*K summed
-    over queries: vector gathers, CSR adjacency reads (degree word + ids), the
-    query row and the K results."""
+    """Sum over queries of 4*D*n_dist + 4*(n_exp + expanded degrees) + 4*D + 8*K:
+    vector gathers, CSR adjacency reads (degree word + ids), query row, results."""
     c = res["counters"]
     return (4.0 * dim * c["distance_computations"] + 4.0 * (c["expansions"] + c["adjacency_slots"])
             + nq * (4.0 * dim + 8.0 * K))
 
 
-def find_L(index, qids, K, M, gt, target, lo, hi, hash_slots):
-    """Smallest L in [lo, hi] with recall@100 >= target (bisection)."""
+def time_plan(plan, steps, warmup):
+    for _ in range(warmup):
+        plan.run(sync=True)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        kms.append(plan.run(sync=True))
+    return time.perf_counter() - t0, kms
+
+
+def find_L(index, qids, K, M, gt, target, lo, hi, **kw):
     from parallel_anns_amd import tools
 
     def rec(L):
-        r = index.search(qids, K, L, M, hash_slots=hash_slots)
-        return tools.recall(gt, r["ids"], L)
+        return tools.recall(gt, index.search(qids, K, L, M, **kw)["ids"], L)
 
     if rec(hi) < target:
         return None
@@ -57,17 +75,6 @@ def find_L(index, qids, K, M, gt, target, lo, hi, hash_slots):
         else:
             lo = mid + 1
     return lo
-
-
-def time_plan(plan, steps, warmup):
-    for _ in range(warmup):
-        plan.run(sync=True)
-    kms = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        kms.append(plan.run(sync=True))
-    wall = time.perf_counter() - t0
-    return wall, kms
 
 
 def cpu_baseline(ds, K, L, M, init_ids, budget_s=12.0):
@@ -87,10 +94,10 @@ def cpu_baseline(ds, K, L, M, init_ids, budget_s=12.0):
     t0 = time.perf_counter()
     ids, _, _, _ = po.search_batch(g, ds["base"], q[:n_s], K, L, M, init_ids, nthreads=cores)
     el = time.perf_counter() - t0
-    return {"value": n_s / el, "unit": "QPS", "cores": cores, "kind": "port",
-            "sample": f"first {n_s} of {q.shape[0]} queries, top-M M={M} L={L} K={K}, "
-                      f"OpenMP inter-query with {cores} threads (oracle restatement, ids identical "
-                      f"to the GPU search)"}, ids
+    return {"value": round(n_s / el, 1), "unit": "QPS", "cores": cores, "kind": "port",
+            "sample": f"first {n_s} of {q.shape[0]} test queries, fixed-L top-M M={M} L={L} K={K}, "
+                      f"OpenMP inter-query over {cores} threads (oracle restatement; ids identical "
+                      f"to the GPU fixed-L search)"}, ids
 
 
 def main():
@@ -101,11 +108,16 @@ def main():
     ap.add_argument("--config", default="c1")
     ap.add_argument("--cache-dir", default=os.environ.get("PANNS_CACHE", "/tmp/panns_cache"))
     ap.add_argument("--target-recall", type=float, default=0.95)
-    ap.add_argument("--op", default=None, help="M,L: skip the operating-point search")
-    ap.add_argument("--ms", default="1,2,4,8", help="M values tried by the operating-point search")
-    ap.add_argument("--hash-slots", type=int, default=0)
+    ap.add_argument("--ms", default="2,4,8", help="M values tried for the fixed-L operating point")
+    ap.add_argument("--op", default=None, help="M,L: fixed-L operating point (skips its search)")
+    ap.add_argument("--learned-op", default=None, help="M,Lmax,threshold: skips training/calibration "
+                                                        "and reuses the model saved in the cache")
+    ap.add_argument("--train-queries", type=int, default=100_000)
+    ap.add_argument("--val-queries", type=int, default=5_000)
+    ap.add_argument("--train-L", type=int, default=400)
+    ap.add_argument("--learned-L", default="300,400")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--curve", action="store_true", help="also report QPS/recall for L in 100..400")
+    ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -113,7 +125,7 @@ def main():
                                    "exchange step, so N>1 is not offered", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    from parallel_anns_amd import datasets, tools
+    from parallel_anns_amd import datasets, learned, tools
     from parallel_anns_amd.search import Index
 
     ds = datasets.prepare(args.config, cache_dir=args.cache_dir, gpu_knn=datasets.GpuKnn(0))
@@ -127,92 +139,149 @@ def main():
     index.set_graph(ds["adj"], ds["deg"], ds["ep"])
     qids = np.arange(nq, dtype=np.uint32)
     gt = ds["gt_ids"]
+    target = args.target_recall
 
-    # operating point
+    # ---- fixed-L operating point
     if args.op:
         M, L = (int(x) for x in args.op.split(","))
         cands = [(M, L)]
     else:
         cands = []
         for M in (int(x) for x in args.ms.split(",")):
-            L = find_L(index, qids, K, M, gt, args.target_recall, K, 1024, args.hash_slots)
+            L = find_L(index, qids, K, M, gt, target, K, 1024)
             log(f"find-L M={M}: L={L}")
             if L is not None:
                 cands.append((M, L))
     best = None
     for M, L in cands:
-        plan = index.plan(qids, K, L, M, hash_slots=args.hash_slots)
-        wall, kms = time_plan(plan, 3, 1)
+        plan = index.plan(qids, K, L, M)
+        wall, _ = time_plan(plan, 3, 1)
         qps = 3 * nq / wall
-        log(f"candidate M={M} L={L}: {qps:,.0f} QPS, kernel {np.mean(kms):.3f} ms")
+        log(f"fixed candidate M={M} L={L}: {qps:,.0f} QPS")
         if best is None or qps > best[0]:
             best = (qps, M, L)
         plan.close()
-    _, M, L = best
-
-    plan = index.plan(qids, K, L, M, hash_slots=args.hash_slots)
-    desc = plan.describe()
+    _, fM, fL = best
+    plan = index.plan(qids, K, fL, fM)
     wall, kms = time_plan(plan, args.steps, args.warmup)
-    res = plan.fetch()
-    recall = tools.recall(gt, res["ids"], L)
-    qps = args.steps * nq / wall
-    ms_step = 1000.0 * wall / args.steps
-    k_ms = float(np.mean(kms))
-    B = algorithmic_bytes(res, dim, K, nq)
-    achieved = B / (k_ms * 1e-3) / 1e9
-    log(f"timed M={M} L={L}: {qps:,.0f} QPS recall {recall:.4f} kernel {k_ms:.3f} ms "
-        f"{achieved:,.0f} GB/s algorithmic; geometry {desc}")
+    fres = plan.fetch()
+    plan.close()
+    fixed = {"M": fM, "L": fL, "qps": round(args.steps * nq / wall, 1),
+             "recall": round(tools.recall(gt, fres["ids"], fL), 4), "kernel_ms": round(float(np.mean(kms)), 4),
+             "ndist": round(float(fres["ndist"].mean()), 1)}
+    log(f"fixed-L: {fixed}")
 
-    curve = None
-    if args.curve:
-        curve = []
-        for Lc in (100, 150, 200, 300, 400):
-            p2 = index.plan(qids, K, Lc, M, hash_slots=args.hash_slots)
-            w2, k2 = time_plan(p2, 3, 1)
-            r2 = p2.fetch()
-            curve.append({"L": Lc, "M": M, "qps": round(3 * nq / w2, 1),
-                          "recall": round(tools.recall(gt, r2["ids"], Lc), 4),
-                          "kernel_ms": round(float(np.mean(k2)), 3)})
-            p2.close()
+    # ---- per-query bound on the fixed-L grid
+    grid = sorted({max(K, int(fL * f)) for f in (0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.15, 1.3, 1.6, 2.0)})
+    R, N = [], []
+    for Lg in grid:
+        r = index.search(qids, K, Lg, fM)
+        R.append(tools.recall_per_query(gt, r["ids"], Lg))
+        N.append(r["ndist"].astype(np.float64))
+    bound = learned.oracle_bound(np.stack(R, 1), np.stack(N, 1), target)
+    oracle = None if bound is None else {
+        "ndist": round(bound[0], 1), "recall": round(bound[1], 4),
+        "distance_ratio_vs_fixed": round(fixed["ndist"] / bound[0], 3),
+        "grid_L": grid}
+    log(f"per-query bound: {oracle}")
 
+    # ---- learned stop
+    lrn = None
+    main_res, main_ms, main_desc, main_recall = fres, fixed["kernel_ms"], None, fixed["recall"]
+    main_qps = fixed["qps"]
+    if not args.no_learned:
+        model_path = os.path.join(ds["dir"] or args.cache_dir, "stop_model.npy")
+        if args.learned_op:
+            lM, lL, thr = args.learned_op.split(",")
+            lM, lL, thr = int(lM), int(lL), float(thr)
+            model = np.load(model_path)
+            train_info = {"reused": model_path}
+        else:
+            t0 = time.time()
+            ntr, nval = args.train_queries, args.val_queries
+            tq = datasets.generate(cfg, "train", n=ntr + nval)
+            index.set_queries(tq[:ntr])
+            lM = fM
+            feat, hsh, niter, _ = index.search_trace(np.arange(ntr, dtype=np.uint32), K, args.train_L, lM,
+                                                     trace_cap=192)
+            X, Y = learned.samples_from_trace(feat, hsh, niter)
+            del feat, hsh
+            model, losses = learned.train(index, X, Y, epochs=4, batch=4096, lr=2e-3, seed=0)
+            np.save(model_path, model)
+            log(f"trained on {X.shape[0]:,} steps of {ntr:,} queries, positives {Y.mean():.3f}, "
+                f"loss {losses.round(4).tolist()} in {time.time() - t0:.1f}s")
+            train_info = {"train_queries": ntr, "samples": int(X.shape[0]), "positive_frac": round(float(Y.mean()), 4),
+                          "epoch_loss": [round(float(x), 5) for x in losses], "train_L": args.train_L, "M": lM,
+                          "label": "top-K set already final", "kernels": "BF16 MFMA fwd/bwd, FP32 accumulate, Adam"}
+            del X, Y
+            # calibrate on held-out training queries (their own exact truth)
+            index.set_queries(tq[ntr:])
+            vgt, _ = index.exact_knn(K, source="queries")
+            vq = np.arange(nval, dtype=np.uint32)
+            index.set_stop_model(model)
+            best_l = None
+            for lL in (int(x) for x in args.learned_L.split(",")):
+                thr = learned.calibrate(index, vq, vgt, K, lL, lM, target + 0.002)
+                if thr is None:
+                    continue
+                rv, nd, _ = learned.recall_work(index, vq, vgt, K, lL, lM, thr)
+                log(f"calibrated L={lL} threshold {thr:.4g}: validation recall {rv:.4f}, ndist {nd:.0f}")
+                if best_l is None or nd < best_l[0]:
+                    best_l = (nd, lL, thr)
+            _, lL, thr = best_l
+            index.set_queries(ds["queries"])
+        index.set_stop_model(model)
+        plan = index.plan(qids, K, lL, lM, stop="learned", stop_threshold=thr)
+        desc = plan.describe()
+        wall, kms = time_plan(plan, args.steps, args.warmup)
+        lres = plan.fetch()
+        plan.close()
+        lrn = {"M": lM, "L_max": lL, "threshold": thr, "qps": round(args.steps * nq / wall, 1),
+               "recall": round(tools.recall(gt, lres["ids"], lL), 4), "kernel_ms": round(float(np.mean(kms)), 4),
+               "ndist": round(float(lres["ndist"].mean()), 1), "training": train_info}
+        log(f"learned: {lrn}")
+        main_res, main_ms, main_desc, main_recall, main_qps = lres, lrn["kernel_ms"], desc, lrn["recall"], lrn["qps"]
+
+    B = algorithmic_bytes(main_res, dim, K, nq)
+    achieved = B / (main_ms * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu:
-        init = index.prepare_init_ids(L)
-        cpu, cpu_ids = cpu_baseline(ds, K, L, M, init)
-        same = bool((cpu_ids == res["ids"][:cpu_ids.shape[0]]).all())
-        cpu["ids_identical_to_gpu"] = same
-        log(f"cpu baseline {cpu['value']:,.0f} QPS on {cpu['cores']} cores; ids identical: {same}")
+        init = index.prepare_init_ids(fL)
+        cpu, cpu_ids = cpu_baseline(ds, K, fL, fM, init)
+        cpu["ids_identical_to_gpu"] = bool((cpu_ids == fres["ids"][:cpu_ids.shape[0]]).all())
+        log(f"cpu baseline {cpu['value']:,.0f} QPS on {cpu['cores']} cores; ids identical: {cpu['ids_identical_to_gpu']}")
 
     out = {
-        "metric": "QPS @ recall@100=0.95 (1M x 96 fp32): plain fixed-L search, 1 MI355X",
-        "value": round(qps, 1),
+        "metric": METRIC,
+        "value": main_qps,
         "unit": "QPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(1e3 * nq / main_qps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded normalised Gaussian mixture, generated in the run)",
+        "data": "synthetic (seeded normalised Gaussian mixture generated in the run; training queries "
+                "seed-disjoint from the test queries)",
         "config": {"workload": f"{args.config}: {cfg['n']:,} x {dim} fp32, {cfg['comps']}-component mixture "
-                               f"sigma {cfg['sigma']}, {nq:,} queries, graph R<={cfg['R']} from exact "
-                               f"{cfg['knn']}-NN, K={K}",
-                   "M": M, "L": L, "stop": "fixed", "recall@100": round(recall, 4),
-                   "target_recall": args.target_recall, "parallelism": "single GPU, persistent kernel",
-                   "workgroups": desc["workgroups"], "lds_bytes": desc["lds_bytes"],
-                   "hash_slots": desc["hash_slots"]},
+                               f"sigma {cfg['sigma']}, {nq:,} test queries, graph R<={cfg['R']} from exact "
+                               f"{cfg['knn']}-NN, K={K}, recall@100 target {target}",
+                   "stop": "learned" if lrn else "fixed", "recall@100": main_recall,
+                   "parallelism": "single GPU, one persistent kernel per step",
+                   "fixed_L": fixed, "learned": lrn,
+                   "learned_over_fixed": round(lrn["qps"] / fixed["qps"], 3) if lrn else None,
+                   "per_query_stop_bound": oracle,
+                   "workgroups": (main_desc or {}).get("workgroups"),
+                   "lds_bytes": (main_desc or {}).get("lds_bytes")},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "search_kernel<96>", "kernel_ms": round(k_ms, 4),
+                     "kernel": "search_kernel<96>", "kernel_ms": round(main_ms, 4),
                      "algorithmic_bytes_per_launch": int(B)},
         "cpu_baseline": cpu,
     }
-    if curve:
-        out["curve"] = curve
     print(json.dumps(out), flush=True)
-    plan.close()
     index.close()
 
 

@@ -162,6 +162,47 @@ int panns_plan_describe(panns_plan *plan, uint32_t *workgroups, uint32_t *lds_by
  * queries (ground truth).  k <= 128.  out_* hold (number of queries) * k. */
 int panns_exact_knn(panns_index *h, int source, uint32_t k, uint32_t *out_ids, float *out_dists);
 
+/* Learned early termination (no reference counterpart; the reference's
+ * studies of when the top-K stops changing are
+ * core/Searching.202012041559.top_m_pure_large_graph.last_update_step.* and
+ * ...num_unchecked.*).
+ *
+ * panns_search_trace: the fixed-L search of panns_search, also recording
+ * after every iteration s < trace_cap the 8 stop features
+ *   [iteration, top-1 dist, K-th dist, K-th/top-1, keys inserted in the
+ *    iteration, best initial dist, query norm, unchecked entries]
+ * into feat[(q*trace_cap + s)*8 ...] and an order-free hash of the top-K id
+ * set into topk_hash[q*trace_cap + s]; niter[q] receives the iteration count.
+ * A step is labelled "final" when its hash equals the last step's.
+ *
+ * panns_set_stop_model: the 8-64-64-1 MLP (ReLU, ReLU, sigmoid) evaluated by
+ * one wavefront after every iteration when panns_params.stop = LEARNED; the
+ * search stops once its output reaches stop_threshold.  params holds
+ * PANNS_STOP_MODEL_FLOATS fp32 values: W1[64][8], b1[64], W2[64][64], b2[64],
+ * w3[64], b3, 3 padding, mu[8], sd[8] (features enter as (f - mu) / sd). */
+#define PANNS_STOP_MODEL_FLOATS 4820
+int panns_search_trace(panns_index *h, const panns_params *p, const uint32_t *query_ids,
+                       uint32_t nq, const uint32_t *init_ids, uint32_t n_init, uint32_t trace_cap,
+                       float *feat, uint32_t *topk_hash, uint32_t *niter, uint32_t *out_ids);
+int panns_set_stop_model(panns_index *h, const float *params, uint32_t n_floats);
+
+/* Training of the stop model on the GPU: BF16 MFMA forward and backward
+ * kernels with FP32 accumulation, Adam on FP32 master weights, binary
+ * cross-entropy.  X holds n raw feature rows (8 floats), Y the 0/1 labels;
+ * features are normalised with params' mu/sd (left unchanged).  params
+ * (PANNS_STOP_MODEL_FLOATS) is read as the initial model and receives the
+ * trained one.  Samples are visited in a seeded random order each epoch;
+ * epoch_loss[e] (nullable) gets the mean loss of epoch e. */
+int panns_mlp_train(panns_index *h, const float *X, const float *Y, uint64_t n, float *params,
+                    uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float *epoch_loss);
+/* Gradient of the mean loss over one batch of n <= 65536 samples (for
+ * checks): grad receives 4801 floats in the parameter layout, *loss the mean. */
+int panns_mlp_grad(panns_index *h, const float *params, const float *X, const float *Y, uint32_t n,
+                   float *grad, float *loss);
+/* One 16x16x16 BF16 MFMA tile, C = A * B (row-major fp32 in and out), to
+ * check the fragment layout the training kernels assume. */
+int panns_mfma_probe(panns_index *h, const float *A, const float *B, float *C);
+
 /* Measurement helpers (no reference counterpart).
  * panns_plan_phase_cycles: shader-clock cycles the last run spent per search
  * phase, summed over workgroups: [init, pick, claim, distance, merge, stop,

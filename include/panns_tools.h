@@ -95,10 +95,14 @@ int pt_navigating_node(const float *base, uint32_t n, uint32_t dim, uint32_t *ep
  *      vertex among its k-NN that still has degree < R (else the first
  *      reached vertex with spare degree), so degree never exceeds R
  *      (index_nsg.cpp:638-718 tree_grow, whose findroot can exceed R).
+ * L_build = 0 skips step 1's search (the pool is the k-NN list alone) and
+ * options without PT_GRAPH_REVERSE_EDGES skip step 3, which together give the
+ * plain "k-NN, RNG prune, DFS stitch" graph.
  * Output adj[v*R + i] for i < deg[v] (sorted by (dist, id) from v), ep. */
+#define PT_GRAPH_REVERSE_EDGES 1u
 int pt_build_graph(const float *base, uint32_t n, uint32_t dim, const uint32_t *knn,
                    uint32_t knn_k, uint32_t R, uint32_t L_build, uint32_t C, uint32_t ep,
-                   uint32_t *adj, uint32_t *deg, int nthreads);
+                   uint32_t options, uint32_t *adj, uint32_t *deg, int nthreads);
 
 /* Vertices reachable from ep (BFS); returns the count in *reached. */
 int pt_reachable(const uint32_t *adj, const uint32_t *deg, uint32_t n, uint32_t stride,

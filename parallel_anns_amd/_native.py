@@ -60,7 +60,7 @@ TOOLS_API = {
     "pt_recall": (c_double, [vp, c_u32, vp, c_u32, c_u32, c_u32, c_u32]),
     "pt_recall_per_query": (c_int, [vp, c_u32, vp, c_u32, c_u32, c_u32, c_u32, vp]),
     "pt_navigating_node": (c_int, [vp, c_u32, c_u32, P(c_u32), c_int]),
-    "pt_build_graph": (c_int, [vp, c_u32, c_u32, vp, c_u32, c_u32, c_u32, c_u32, c_u32, vp, vp, c_int]),
+    "pt_build_graph": (c_int, [vp, c_u32, c_u32, vp, c_u32, c_u32, c_u32, c_u32, c_u32, c_u32, vp, vp, c_int]),
     "pt_reachable": (c_int, [vp, vp, c_u32, c_u32, c_u32, P(c_u32)]),
 }
 
@@ -85,6 +85,11 @@ AMD_API = {
     "panns_plan_destroy": (c_int, [vp]),
     "panns_plan_describe": (c_int, [vp, P(c_u32), P(c_u32), P(c_u32), P(c_u32)]),
     "panns_exact_knn": (c_int, [vp, c_int, c_u32, vp, vp]),
+    "panns_search_trace": (c_int, [vp, P(PannsParams), vp, c_u32, vp, c_u32, c_u32, vp, vp, vp, vp]),
+    "panns_set_stop_model": (c_int, [vp, vp, c_u32]),
+    "panns_mlp_train": (c_int, [vp, vp, vp, c_u64, vp, c_u32, c_u32, c_float, c_u64, vp]),
+    "panns_mlp_grad": (c_int, [vp, vp, vp, vp, c_u32, vp, P(c_float)]),
+    "panns_mfma_probe": (c_int, [vp, vp, vp, vp]),
     "panns_plan_phase_cycles": (c_int, [vp, vp]),
     "panns_device_copy": (c_int, [vp, c_u64, P(c_float)]),
 }

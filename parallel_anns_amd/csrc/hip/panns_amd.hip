@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -13,6 +14,9 @@
 #include "panns_tools.h"
 #include "search_kernel.h"
 #include "knn_kernel.h"
+#include "mlp_kernels.h"
+
+#include <random>
 
 namespace {
 
@@ -267,6 +271,10 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     HIP_TRY(pl->out_ids.ensure((size_t)pl->nq * pl->maxK));
     HIP_TRY(pl->out_dists.ensure((size_t)pl->nq * pl->maxK));
     HIP_TRY(pl->out_stats.ensure((size_t)pl->nq * 5));
+    if (pl->trace_cap) {
+        HIP_TRY(pl->trace_feat.ensure((size_t)pl->nq * pl->trace_cap * panns::kFeat));
+        HIP_TRY(pl->trace_topk.ensure((size_t)pl->nq * pl->trace_cap));
+    }
 #ifdef PANNS_PHASE_TIMERS
     HIP_TRY(pl->phase_cycles.ensure((size_t)pl->grid * panns::kPhases));
     HIP_TRY(hipMemsetAsync(pl->phase_cycles.p, 0, (size_t)pl->grid * panns::kPhases * 8, h->stream));
@@ -326,8 +334,6 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
         a.model.b2 = m + 576 + 4096;
         a.model.w3 = m + 576 + 4096 + 64;
         a.model.b3 = m + 576 + 4096 + 128;
-        a.model.mu = m + 576 + 4096 + 132;
-        a.model.sd = m + 576 + 4096 + 140;
     }
     a.trace_feat = pl->trace_cap ? pl->trace_feat.p : nullptr;
     a.trace_topk = pl->trace_cap ? pl->trace_topk.p : nullptr;
@@ -733,6 +739,210 @@ int panns_subsearch_with_top_m(panns_index *h, uint32_t M_max, uint32_t query_id
     if (rc) return rc;
     HIP_TRY(hipMemcpy(visited_words, bm.p, words32 * 4, hipMemcpyDeviceToHost));
     if (count_distance) *count_distance += nd;
+    return PANNS_OK;
+}
+
+int panns_search_trace(panns_index *h, const panns_params *p, const uint32_t *query_ids, uint32_t nq,
+                       const uint32_t *init_ids, uint32_t n_init, uint32_t trace_cap, float *feat,
+                       uint32_t *topk_hash, uint32_t *niter, uint32_t *out_ids)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!p || !query_ids || !init_ids || !feat || !topk_hash || !niter || nq == 0 || trace_cap == 0)
+        return fail(PANNS_ERR_ARG, "bad trace arguments");
+    if (p->stop != PANNS_STOP_FIXED) return fail(PANNS_ERR_ARG, "traces are taken from the fixed-L search");
+    if (!h->nq) return fail(PANNS_ERR_STATE, "queries not loaded");
+    HIP_TRY(hipSetDevice(h->device));
+    std::unique_ptr<panns_plan> pl(new panns_plan());
+    pl->h = h;
+    pl->p = *p;
+    pl->nq = nq;
+    pl->trace_cap = trace_cap;
+    rc = build_plan(pl.get(), query_ids, nullptr, nullptr, init_ids, n_init);
+    if (rc) return rc;
+    float ms = 0.f;
+    rc = run_plan(pl.get(), 1, &ms);
+    if (rc) return rc;
+    std::vector<uint32_t> ids((size_t)nq * pl->maxK);
+    panns_outputs o{};
+    o.ids = out_ids ? out_ids : ids.data();
+    o.niter = niter;
+    rc = fetch_plan(pl.get(), &o, nullptr, ms);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(feat, pl->trace_feat.p, (size_t)nq * trace_cap * panns::kFeat * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(topk_hash, pl->trace_topk.p, (size_t)nq * trace_cap * 4, hipMemcpyDeviceToHost));
+    return PANNS_OK;
+}
+
+int panns_set_stop_model(panns_index *h, const float *params, uint32_t n_floats)
+{
+    if (!h || !params) return fail(PANNS_ERR_ARG, "null argument");
+    if (n_floats != PANNS_STOP_MODEL_FLOATS)
+        return fail(PANNS_ERR_ARG, "stop model must hold " + std::to_string(PANNS_STOP_MODEL_FLOATS) + " floats");
+    for (uint32_t i = 0; i < n_floats; ++i)
+        if (!std::isfinite(params[i]))
+            return fail(PANNS_ERR_ARG, "stop model holds a non-finite value");
+    for (uint32_t i = 0; i < panns::kFeat; ++i)
+        if (!(params[4812 + i] > 0.f)) return fail(PANNS_ERR_ARG, "feature scales must be positive");
+    HIP_TRY(hipSetDevice(h->device));
+    // Fold the feature normalisation into the first layer:
+    // W1 ((f - mu) / sd) + b1 = (W1 / sd) f + (b1 - (W1 / sd) mu).
+    std::vector<float> folded(params, params + n_floats);
+    const float *mu = params + 4804, *sd = params + 4812;
+    for (int j = 0; j < panns::kHid; ++j) {
+        double b = params[512 + j];
+        for (int i = 0; i < panns::kFeat; ++i) {
+            const float w = params[j * panns::kFeat + i] / sd[i];
+            folded[j * panns::kFeat + i] = w;
+            b -= (double)w * (double)mu[i];
+        }
+        folded[512 + j] = (float)b;
+    }
+    for (int i = 0; i < panns::kFeat; ++i) {
+        folded[4804 + i] = 0.f;
+        folded[4812 + i] = 1.f;
+    }
+    HIP_TRY(h->model.ensure(n_floats));
+    HIP_TRY(hipMemcpy(h->model.p, folded.data(), n_floats * 4ull, hipMemcpyHostToDevice));
+    h->has_model = true;
+    return PANNS_OK;
+}
+
+namespace {
+
+// (x - mu) / sd for n rows of 8 features.
+std::vector<float> normalise_features(const float *X, uint64_t n, const float *params)
+{
+    std::vector<float> out((size_t)n * panns::kMlpIn);
+    const float *mu = params + 4804, *sd = params + 4812;
+    for (uint64_t r = 0; r < n; ++r)
+        for (int f = 0; f < panns::kMlpIn; ++f)
+            out[r * panns::kMlpIn + f] = (X[r * panns::kMlpIn + f] - mu[f]) / sd[f];
+    return out;
+}
+
+int check_model_scales(const float *params)
+{
+    for (int i = 0; i < panns::kMlpIn; ++i)
+        if (!(params[4812 + i] > 0.f) || !std::isfinite(params[4804 + i]))
+            return fail(PANNS_ERR_ARG, "feature mu must be finite and sd positive");
+    return PANNS_OK;
+}
+
+}  // namespace
+
+int panns_mlp_grad(panns_index *h, const float *params, const float *X, const float *Y, uint32_t n, float *grad,
+                   float *loss)
+{
+    if (!h || !params || !X || !Y || !grad || n == 0 || n > 65536) return fail(PANNS_ERR_ARG, "bad gradient arguments");
+    int rc = check_model_scales(params);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(h->device));
+    const std::vector<float> xn = normalise_features(X, n, params);
+    const uint32_t nwg = (n + panns::kMlpTile - 1) / panns::kMlpTile;
+    DevBuf<float> dP, dX, dY, part, dl, dg;
+    HIP_TRY(dP.ensure(panns::kMlpParamsPad));
+    HIP_TRY(dX.ensure(xn.size()));
+    HIP_TRY(dY.ensure(n));
+    HIP_TRY(part.ensure((size_t)nwg * panns::kMlpParamsPad));
+    HIP_TRY(dl.ensure(nwg));
+    HIP_TRY(dg.ensure(panns::kMlpParamsPad));
+    HIP_TRY(hipMemcpy(dP.p, params, panns::kMlpParams * 4ull, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dX.p, xn.data(), xn.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dY.p, Y, n * 4ull, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(panns::mlp_fwd_bwd_kernel, dim3(nwg), dim3(256), 0, h->stream, dP.p, dX.p, dY.p, nullptr,
+                       (uint64_t)0, n, part.p, dl.p);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(panns::mlp_adam_kernel, dim3((panns::kMlpParams + 255) / 256), dim3(256), 0, h->stream, dP.p,
+                       part.p, nwg, nullptr, nullptr, 0.f, 0.f, 0.f, 0.f, 0.f, dg.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(grad, dg.p, panns::kMlpParams * 4ull, hipMemcpyDeviceToHost));
+    std::vector<float> ls(nwg);
+    HIP_TRY(hipMemcpy(ls.data(), dl.p, nwg * 4ull, hipMemcpyDeviceToHost));
+    double tot = 0.0;
+    for (float v : ls) tot += v;
+    if (loss) *loss = (float)(tot / n);
+    return PANNS_OK;
+}
+
+int panns_mlp_train(panns_index *h, const float *X, const float *Y, uint64_t n, float *params, uint32_t epochs,
+                    uint32_t batch, float lr, uint64_t seed, float *epoch_loss)
+{
+    if (!h || !X || !Y || !params || n == 0 || epochs == 0 || batch == 0 || batch > 65536 || !(lr > 0.f))
+        return fail(PANNS_ERR_ARG, "bad training arguments");
+    int rc = check_model_scales(params);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(h->device));
+    const std::vector<float> xn = normalise_features(X, n, params);
+    const uint32_t nwg = (batch + panns::kMlpTile - 1) / panns::kMlpTile;
+    const uint64_t nbatch = (n + batch - 1) / batch;
+    DevBuf<float> dP, dX, dY, part, dl, m1, m2;
+    DevBuf<uint32_t> dperm;
+    HIP_TRY(dP.ensure(panns::kMlpParamsPad));
+    HIP_TRY(dX.ensure(xn.size()));
+    HIP_TRY(dY.ensure(n));
+    HIP_TRY(dperm.ensure(n));
+    HIP_TRY(part.ensure((size_t)nwg * panns::kMlpParamsPad));
+    HIP_TRY(dl.ensure((size_t)nbatch * nwg));
+    HIP_TRY(m1.ensure(panns::kMlpParamsPad));
+    HIP_TRY(m2.ensure(panns::kMlpParamsPad));
+    HIP_TRY(hipMemcpy(dP.p, params, panns::kMlpParams * 4ull, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dX.p, xn.data(), xn.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dY.p, Y, n * 4ull, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(m1.p, 0, panns::kMlpParamsPad * 4ull));
+    HIP_TRY(hipMemset(m2.p, 0, panns::kMlpParamsPad * 4ull));
+    std::vector<uint32_t> perm(n);
+    for (uint64_t i = 0; i < n; ++i) perm[i] = (uint32_t)i;
+    std::mt19937_64 rng(seed);
+    float t = 0.f;
+    for (uint32_t e = 0; e < epochs; ++e) {
+        for (uint64_t i = n - 1; i > 0; --i) {
+            const uint64_t j = rng() % (i + 1);
+            std::swap(perm[i], perm[j]);
+        }
+        HIP_TRY(hipMemcpyAsync(dperm.p, perm.data(), n * 4ull, hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipMemsetAsync(dl.p, 0, (size_t)nbatch * nwg * 4, h->stream));
+        for (uint64_t b = 0; b < nbatch; ++b) {
+            const uint64_t first = b * batch;
+            const uint32_t cnt = (uint32_t)std::min<uint64_t>(batch, n - first);
+            const uint32_t g = (cnt + panns::kMlpTile - 1) / panns::kMlpTile;
+            hipLaunchKernelGGL(panns::mlp_fwd_bwd_kernel, dim3(g), dim3(256), 0, h->stream, dP.p, dX.p, dY.p,
+                               dperm.p, first, cnt, part.p, dl.p + b * nwg);
+            HIP_TRY(hipGetLastError());
+            t += 1.f;
+            hipLaunchKernelGGL(panns::mlp_adam_kernel, dim3((panns::kMlpParams + 255) / 256), dim3(256), 0,
+                               h->stream, dP.p, part.p, g, m1.p, m2.p, lr, 0.9f, 0.999f, 1e-8f, t, nullptr);
+            HIP_TRY(hipGetLastError());
+        }
+        if (epoch_loss) {
+            std::vector<float> ls((size_t)nbatch * nwg);
+            HIP_TRY(hipMemcpyAsync(ls.data(), dl.p, ls.size() * 4, hipMemcpyDeviceToHost, h->stream));
+            HIP_TRY(hipStreamSynchronize(h->stream));
+            double tot = 0.0;
+            for (float v : ls) tot += v;
+            epoch_loss[e] = (float)(tot / (double)n);
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(params, dP.p, panns::kMlpParams * 4ull, hipMemcpyDeviceToHost));
+    return PANNS_OK;
+}
+
+int panns_mfma_probe(panns_index *h, const float *A, const float *B, float *C)
+{
+    if (!h || !A || !B || !C) return fail(PANNS_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(h->device));
+    DevBuf<float> dA, dB, dC;
+    HIP_TRY(dA.ensure(256));
+    HIP_TRY(dB.ensure(256));
+    HIP_TRY(dC.ensure(256));
+    HIP_TRY(hipMemcpy(dA.p, A, 1024, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dB.p, B, 1024, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(panns::mfma_probe_kernel, dim3(1), dim3(64), 0, h->stream, dA.p, dB.p, dC.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(C, dC.p, 1024, hipMemcpyDeviceToHost));
     return PANNS_OK;
 }
 

@@ -60,8 +60,6 @@ struct StopModel {
     const float *b2;  // [64]
     const float *w3;  // [64]
     const float *b3;  // [1]
-    const float *mu;  // [8] feature centring
-    const float *sd;  // [8] feature scaling
 };
 
 struct KArgs {
@@ -391,35 +389,58 @@ __device__ __forceinline__ uint32_t claim_chunks(Visited &vis, uint32_t (&ids)[k
     return nnew;
 }
 
-// One-wave fp32 MLP: lane j computes hidden unit j.  Returns the sigmoid
-// output in every lane.
-__device__ __forceinline__ float mlp_eval(const StopModel &m, const float (&f)[kFeat], float *h1s,
-                                          int lane)
+// The per-lane part of the stop model, kept in registers for the kernel's
+// life: lane j's row of W1 (feature normalisation folded in by the host), its
+// biases and its output weight.
+struct LaneModel {
+    float w1[kFeat];
+    float b1, b2, w3, b3;
+};
+
+__device__ __forceinline__ LaneModel load_lane_model(const StopModel &m, int lane)
 {
-    float h = m.b1[lane];
+    LaneModel r;
 #pragma unroll
-    for (int i = 0; i < kFeat; ++i) h = __builtin_fmaf(m.w1[lane * kFeat + i], (f[i] - m.mu[i]) / m.sd[i], h);
+    for (int i = 0; i < kFeat; ++i) r.w1[i] = m.w1[lane * kFeat + i];
+    r.b1 = m.b1[lane];
+    r.b2 = m.b2[lane];
+    r.w3 = m.w3[lane];
+    r.b3 = m.b3[0];
+    return r;
+}
+
+// One-wave fp32 MLP: lane j computes hidden unit j of both layers; only the
+// 64x64 layer's row is read from memory (L1-resident, shared by all waves).
+// Returns the sigmoid output in every lane.
+__device__ __forceinline__ float mlp_eval(const LaneModel &lm, const float *__restrict__ w2,
+                                          const float (&f)[kFeat], float *h1s, int lane)
+{
+    const float4 *w2r = reinterpret_cast<const float4 *>(w2 + lane * kHid);
+    float4 wv[kHid / 4];
+#pragma unroll
+    for (int i = 0; i < kHid / 4; ++i) wv[i] = w2r[i];
+    float h = lm.b1;
+#pragma unroll
+    for (int i = 0; i < kFeat; ++i) h = __builtin_fmaf(lm.w1[i], f[i], h);
     h = h > 0.f ? h : 0.f;
     h1s[lane] = h;
     __syncthreads();
-    float g = m.b2[lane];
-    const float4 *w2r = reinterpret_cast<const float4 *>(m.w2 + lane * kHid);
+    float g = lm.b2;
     const float4 *h1v = reinterpret_cast<const float4 *>(h1s);
-#pragma unroll 4
+#pragma unroll
     for (int i = 0; i < kHid / 4; ++i) {
-        const float4 w = w2r[i];
         const float4 x = h1v[i];
-        g = __builtin_fmaf(w.x, x.x, g);
-        g = __builtin_fmaf(w.y, x.y, g);
-        g = __builtin_fmaf(w.z, x.z, g);
-        g = __builtin_fmaf(w.w, x.w, g);
+        g = __builtin_fmaf(wv[i].x, x.x, g);
+        g = __builtin_fmaf(wv[i].y, x.y, g);
+        g = __builtin_fmaf(wv[i].z, x.z, g);
+        g = __builtin_fmaf(wv[i].w, x.w, g);
     }
     g = g > 0.f ? g : 0.f;
-    float o = m.w3[lane] * g;
+    float o = lm.w3 * g;
 #pragma unroll
     for (int s = 32; s > 0; s >>= 1) o += __shfl_xor(o, s);
     __syncthreads();
-    o += m.b3[0];
+    o += lm.b3;
     return 1.f / (1.f + __expf(-o));
 }
 
@@ -477,6 +498,8 @@ __global__ __launch_bounds__(64, 3) void search_kernel(KArgs a)
     vis.bitmap = a.shared_bitmap ? a.shared_bitmap : a.bitmaps + (size_t)blockIdx.x * a.bitmap_words;
     vis.log = a.shared_bitmap ? nullptr : a.spill_log + (size_t)blockIdx.x * a.log_cap;
     vis.log_cap = a.log_cap;
+    LaneModel lm{};
+    if (a.stop_learned) lm = load_lane_model(a.model, lane);
     PANNS_PT_DECL
 
     for (;;) {
@@ -645,7 +668,7 @@ __global__ __launch_bounds__(64, 3) void search_kernel(KArgs a)
                     if (lane == 0) a.trace_topk[(size_t)w * a.trace_cap + (niter - 1)] = hsh;
                 }
                 if (a.stop_learned) {
-                    const float pstop = mlp_eval(a.model, f, h1s, lane);
+                    const float pstop = mlp_eval(lm, a.model.w2, f, h1s, lane);
                     if (pstop >= a.stop_threshold) break;
                 }
                 PANNS_PT(kPhStop);

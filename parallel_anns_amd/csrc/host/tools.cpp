@@ -579,8 +579,10 @@ void knn_graph_search(const float *base, uint32_t dim, const uint32_t *knn, uint
 
 extern "C" int pt_build_graph(const float *base, uint32_t n, uint32_t dim, const uint32_t *knn,
                               uint32_t knn_k, uint32_t R, uint32_t L_build, uint32_t C,
-                              uint32_t ep, uint32_t *adj, uint32_t *deg, int nthreads)
+                              uint32_t ep, uint32_t options, uint32_t *adj, uint32_t *deg,
+                              int nthreads)
 {
+    const bool reverse_edges = (options & PT_GRAPH_REVERSE_EDGES) != 0;
     if (n == 0 || R == 0 || dim % 8 || ep >= n) return fail(PT_ERR_ARG, "pt_build_graph: bad argument");
     const int nt = nthreads_or_default(nthreads);
     std::vector<std::vector<Cand>> out(n);
@@ -636,7 +638,7 @@ extern "C" int pt_build_graph(const float *base, uint32_t n, uint32_t dim, const
             const uint32_t v = (uint32_t)vi;
             pool = out[v];
             const float *x = base + (size_t)v * dim;
-            for (uint32_t i = indeg[v]; i < indeg[v + 1]; ++i) {
+            for (uint32_t i = reverse_edges ? indeg[v] : indeg[v + 1]; i < indeg[v + 1]; ++i) {
                 const uint32_t s = rev[i];
                 if (s == v) continue;
                 pool.push_back({pt_l2(base + (size_t)s * dim, x, dim), s});

@@ -109,8 +109,12 @@ def prepare(name_or_cfg, cache_dir=None, nthreads=0, gpu_knn=None, truth_k=100, 
     if verbose:
         log(f"exact {cfg['knn']}-NN in {time.time() - t0:.1f}s")
     t0 = time.time()
-    adj, deg, ep = tools.build_graph(base, knn, cfg["R"], L_build=cfg.get("L_build", 40),
-                                     C=cfg.get("C", 500), nthreads=nthreads)
+    # default: the configured construction, exact k-NN -> RNG prune -> DFS
+    # stitch; "L_build" > 0 / "reverse" add NSG's search-path pool and
+    # reverse edges
+    adj, deg, ep = tools.build_graph(base, knn, cfg["R"], L_build=cfg.get("L_build", 0),
+                                     C=cfg.get("C", 500), reverse_edges=cfg.get("reverse", False),
+                                     nthreads=nthreads)
     del knn
     if verbose:
         log(f"graph R<={cfg['R']} avg deg {deg.mean():.2f} max {deg.max()} ep {ep} in {time.time() - t0:.1f}s")

@@ -1,0 +1,109 @@
+"""Learned early termination for the top-M search.
+
+Pipeline (all device work runs through include/panns_amd.h):
+  1. traces: the plain fixed-L search (search.Index.search_trace) records after
+     every iteration the 8 stop features and a hash of the top-K id set;
+  2. labels: a step is positive when its top-K set already equals the final
+     one (the set the search ends with);
+  3. training: the 8-64-64-1 MLP on the GPU with the BF16-MFMA forward and
+     backward kernels (search.Index.mlp_train), features normalised by the
+     training set's mean and deviation;
+  4. calibration: the stop threshold is the smallest-work threshold whose
+     recall on held-out validation queries reaches the target;
+  5. search: panns_params.stop = LEARNED runs the MLP after every iteration in
+     the search kernel and stops the query once its output reaches the
+     threshold.
+
+The reference has no learned component; its hand-tuned stopping studies are
+core/Searching.202011301508.top_m_pure_large_graph.num_unchecked.* and
+core/Searching.202012041559.top_m_pure_large_graph.last_update_step.*.
+"""
+import numpy as np
+
+from . import tools
+
+FEATURES = ("iteration", "top1_dist", "kth_dist", "kth_over_top1", "inserted_last_step",
+            "best_initial_dist", "query_norm", "unchecked")
+MODEL_FLOATS = 4820
+
+
+def init_model(seed=0):
+    """He-initialised weights, zero biases, identity normalisation."""
+    rng = np.random.default_rng(seed)
+    P = np.zeros(MODEL_FLOATS, np.float32)
+    P[0:512] = rng.standard_normal(512) * np.sqrt(2 / 8)
+    P[576:4672] = rng.standard_normal(4096) * np.sqrt(2 / 64)
+    P[4736:4800] = rng.standard_normal(64) * np.sqrt(1 / 64)
+    P[4812:4820] = 1.0
+    return P
+
+
+def samples_from_trace(feat, topk_hash, niter):
+    """(X [n, 8], Y [n]) with Y = 1 where the step's top-K set is the final one."""
+    cap = feat.shape[1]
+    X, Y = [], []
+    for q in range(feat.shape[0]):
+        n = int(min(niter[q], cap))
+        if n == 0:
+            continue
+        final = topk_hash[q, n - 1]
+        X.append(feat[q, :n])
+        Y.append((topk_hash[q, :n] == final).astype(np.float32))
+    return np.concatenate(X), np.concatenate(Y)
+
+
+def set_normalisation(P, X):
+    P = P.copy()
+    P[4804:4812] = X.mean(0)
+    P[4812:4820] = X.std(0) + 1e-6
+    return P
+
+
+def train(index, X, Y, epochs=4, batch=4096, lr=2e-3, seed=0):
+    P = set_normalisation(init_model(seed), X)
+    P, losses = index.mlp_train(X, Y, P, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return P, losses
+
+
+def recall_work(index, qids, gt, K, L, M, threshold):
+    """(recall@K, mean distance computations, kernel ms) of one learned-stop run."""
+    plan = index.plan(qids, K, L, M, stop="learned", stop_threshold=threshold)
+    ms = plan.run()
+    res = plan.fetch()
+    plan.close()
+    return tools.recall(gt, res["ids"], L, R=min(K, gt.shape[1])), float(res["ndist"].mean()), ms
+
+
+def calibrate(index, qids, gt, K, L, M, target, lo=1e-4, hi=1.0, steps=14):
+    """Largest-saving threshold whose recall on (qids, gt) reaches target: the
+    recall rises with the threshold, so bisect on it (log scale)."""
+    r_hi = recall_work(index, qids, gt, K, L, M, hi)[0]
+    if r_hi < target:
+        return None
+    a, b = np.log(lo), np.log(hi)
+    if recall_work(index, qids, gt, K, L, M, lo)[0] >= target:
+        return lo
+    for _ in range(steps):
+        m = 0.5 * (a + b)
+        if recall_work(index, qids, gt, K, L, M, float(np.exp(m)))[0] >= target:
+            b = m
+        else:
+            a = m
+    return float(np.exp(b))
+
+
+def oracle_bound(per_query_recall, per_query_ndist, target):
+    """Mean distance computations when every query may pick its own L from the
+    measured grid (columns), the best any per-query stop rule can do on that
+    grid: the cheapest per-query recall floor whose mean reaches target."""
+    R = np.asarray(per_query_recall)
+    N = np.asarray(per_query_ndist, np.float64)
+    best = None
+    for r0 in np.linspace(0.0, 1.0, 101):
+        ok = R >= r0 - 1e-9
+        pick = np.where(ok.any(1), ok.argmax(1), R.shape[1] - 1)
+        rec = R[np.arange(R.shape[0]), pick].mean()
+        nd = N[np.arange(R.shape[0]), pick].mean()
+        if rec >= target and (best is None or nd < best[0]):
+            best = (float(nd), float(rec), float(r0))
+    return best

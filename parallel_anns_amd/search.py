@@ -192,6 +192,55 @@ class Index:
                                         ptr(init_ids), init_ids.shape[0], h))
         return Plan(self, h, qids.shape[0], maxK, maxL)
 
+    def search_trace(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, trace_cap=256):
+        """Fixed-L search that also records the stop features and the top-K
+        hash after every iteration; returns (feat [nq, cap, 8], topk_hash
+        [nq, cap], niter [nq], ids [nq, K])."""
+        qids = np.ascontiguousarray(np.atleast_1d(query_ids), dtype=np.uint32)
+        nq = qids.shape[0]
+        if init_ids is None:
+            init_ids = self.prepare_init_ids(L)
+        init_ids = np.ascontiguousarray(init_ids, dtype=np.uint32)
+        p = make_params(K, L, M, mode)
+        feat = np.zeros((nq, trace_cap, 8), np.float32)
+        hsh = np.zeros((nq, trace_cap), np.uint32)
+        niter = np.zeros(nq, np.uint32)
+        ids = np.zeros((nq, K), np.uint32)
+        _chk(self.lib.panns_search_trace(self.h, p, ptr(qids), nq, ptr(init_ids), init_ids.shape[0],
+                                         trace_cap, ptr(feat), ptr(hsh), ptr(niter), ptr(ids)))
+        return feat, hsh, niter, ids
+
+    def set_stop_model(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        _chk(self.lib.panns_set_stop_model(self.h, ptr(params), params.shape[0]))
+
+    def mlp_train(self, X, Y, params, epochs=4, batch=4096, lr=2e-3, seed=0):
+        """Trains the stop model on the GPU (BF16 MFMA kernels); returns
+        (trained params, per-epoch mean loss)."""
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        P = np.array(params, np.float32, copy=True)
+        losses = np.zeros(epochs, np.float32)
+        _chk(self.lib.panns_mlp_train(self.h, ptr(X), ptr(Y), X.shape[0], ptr(P), epochs, batch, lr, seed,
+                                      ptr(losses)))
+        return P, losses
+
+    def mlp_grad(self, params, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        P = np.ascontiguousarray(params, np.float32)
+        g = np.zeros(4801, np.float32)
+        loss = _native.c_float()
+        _chk(self.lib.panns_mlp_grad(self.h, ptr(P), ptr(X), ptr(Y), X.shape[0], ptr(g), loss))
+        return g, loss.value
+
+    def mfma_probe(self, A, B):
+        A = np.ascontiguousarray(A, np.float32)
+        B = np.ascontiguousarray(B, np.float32)
+        C = np.zeros((16, 16), np.float32)
+        _chk(self.lib.panns_mfma_probe(self.h, ptr(A), ptr(B), ptr(C)))
+        return C
+
     def device_copy(self, nbytes):
         """Device time (ms) of a plain streaming copy of nbytes."""
         ms = _native.c_float()

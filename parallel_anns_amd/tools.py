@@ -133,9 +133,11 @@ def navigating_node(base, nthreads=0):
     return ep.value
 
 
-def build_graph(base, knn_ids, R, L_build=40, C=500, ep=None, nthreads=0):
+def build_graph(base, knn_ids, R, L_build=40, C=500, ep=None, reverse_edges=True, nthreads=0):
     """NSG-style graph (kNN + search-path pool, RNG prune, reverse edges,
-    DFS stitching with out-degree <= R).  Returns (adj [n, R], deg, ep)."""
+    DFS stitching with out-degree <= R).  L_build=0 and reverse_edges=False
+    give the plain k-NN -> RNG prune -> DFS stitch graph.
+    Returns (adj [n, R], deg, ep)."""
     base, knn_ids = _f32(base), _u32(knn_ids)
     n, dim = base.shape
     if ep is None:
@@ -143,7 +145,8 @@ def build_graph(base, knn_ids, R, L_build=40, C=500, ep=None, nthreads=0):
     adj = np.full((n, R), U32_NONE, dtype=np.uint32)
     deg = np.zeros(n, dtype=np.uint32)
     _chk(_native.tools().pt_build_graph(ptr(base), n, dim, ptr(knn_ids), knn_ids.shape[1], R, L_build,
-                                        C, ep, ptr(adj), ptr(deg), nthreads), "build_graph")
+                                        C, ep, 1 if reverse_edges else 0, ptr(adj), ptr(deg), nthreads),
+         "build_graph")
     return adj, deg, ep
 
 

@@ -1,0 +1,90 @@
+"""Learned stop on the GPU: MFMA fragment layout, BF16-MFMA gradients against
+the FP64 oracle, training, and the in-loop stop's semantics."""
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+from parallel_anns_amd import learned, tools
+from parallel_anns_amd.search import Index
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def idx():
+    i = Index(0)
+    i.set_base(np.zeros((8, 96), np.float32))
+    yield i
+    i.close()
+
+
+def test_mfma_fragment_layout(idx):
+    rng = np.random.default_rng(0)
+    A = mo.bf16(rng.standard_normal((16, 16))).astype(np.float32)
+    B = mo.bf16(rng.standard_normal((16, 16))).astype(np.float32)
+    C = idx.mfma_probe(A, B)
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    assert np.allclose(C, ref, rtol=1e-5, atol=1e-5)
+    # a transposed operand layout would fail this
+    assert not np.allclose(C, A.T.astype(np.float64) @ B, rtol=1e-3, atol=1e-3)
+
+
+@pytest.mark.parametrize("n", [64, 200, 1000])
+def test_mlp_gradient_matches_oracle(idx, n):
+    rng = np.random.default_rng(n)
+    P = mo.init_params(1)
+    P[512:576] = rng.standard_normal(64) * 0.1
+    P[4672:4736] = rng.standard_normal(64) * 0.1
+    P[4800] = 0.2
+    X = rng.standard_normal((n, 8)).astype(np.float32) * 3 + 1
+    P[4804:4812] = X.mean(0)
+    P[4812:4820] = X.std(0)
+    Y = (rng.random(n) < 0.4).astype(np.float32)
+    g, loss = idx.mlp_grad(P, X, Y)
+    Xn = mo.normalise(X, P)
+    l_ref, _, g_ref = mo.forward_backward(P[:mo.N_PARAMS], Xn, Y, bf16_operands=True)
+    assert abs(loss - l_ref) < 1e-4 * max(1.0, abs(l_ref))
+    scale = np.abs(g_ref).max()
+    assert np.abs(g - g_ref).max() <= 2e-4 * scale
+    # and the plain fp64 gradient agrees to bf16 precision
+    _, _, g64 = mo.forward_backward(P[:mo.N_PARAMS], Xn, Y)
+    assert np.abs(g - g64).max() <= 3e-2 * scale
+
+
+def test_training_reduces_loss(idx):
+    rng = np.random.default_rng(5)
+    n = 20000
+    X = rng.standard_normal((n, 8)).astype(np.float32)
+    Y = ((X[:, 0] + 0.5 * X[:, 3] > 0.2) ^ (X[:, 5] > 1.5)).astype(np.float32)
+    P, losses = learned.train(idx, X, Y, epochs=6, batch=2048, lr=3e-3)
+    assert losses[-1] < 0.6 * losses[0]
+    l_final, p, _ = mo.forward_backward(P[:mo.N_PARAMS], mo.normalise(X, P), Y)
+    assert np.mean((p > 0.5) == (Y > 0.5)) > 0.9
+
+
+def test_learned_stop_semantics(small_set, gpu_index):
+    ds = small_set
+    K, L, M = 100, 160, 4
+    q = np.arange(ds["queries"].shape[0], dtype=np.uint32)
+    feat, hsh, niter, ids_t = gpu_index.search_trace(q, K, L, M, trace_cap=256)
+    fixed = gpu_index.search(q, K, L, M)
+    assert (ids_t == fixed["ids"]).all()
+    assert (niter == fixed["niter"]).all()
+    X, Y = learned.samples_from_trace(feat, hsh, niter)
+    assert X.shape[0] == int(np.minimum(niter, 256).sum())
+    # iteration feature counts 1, 2, ...; the last step of each query is positive
+    assert feat[0, 0, 0] == 1.0 and feat[0, 1, 0] == 2.0
+    assert Y.mean() > 0
+    P, _ = learned.train(gpu_index, X, Y, epochs=2, batch=1024)
+    gpu_index.set_stop_model(P)
+    # a threshold the sigmoid never reaches leaves the fixed-L search unchanged
+    never = gpu_index.search(q, K, L, M, stop="learned", stop_threshold=2.0)
+    assert (never["ids"] == fixed["ids"]).all() and (never["ndist"] == fixed["ndist"]).all()
+    # threshold 0 stops every query after its first iteration
+    first = gpu_index.search(q, K, L, M, stop="learned", stop_threshold=0.0)
+    assert (first["niter"] == 1).all()
+    mid = gpu_index.search(q, K, L, M, stop="learned", stop_threshold=0.5)
+    assert (mid["ndist"] <= fixed["ndist"]).all()
+    # stopping early never reorders what was found: the partial queue is sorted
+    assert (np.diff(mid["dists"], axis=1) >= 0).all()
+    assert tools.recall(ds["gt_ids"], mid["ids"], L) <= tools.recall(ds["gt_ids"], fixed["ids"], L) + 1e-9

@@ -189,3 +189,30 @@ def test_prepare_init_ids_known_answers():
     assert list(po.prepare_init_ids(g, 3)) == [5, 1, 7]
     # then ep+1, ep+2, ... wrapping at N, skipping chosen ids
     assert list(po.prepare_init_ids(g, 9)) == [5, 1, 7, 4, 6, 8, 0, 2, 3]
+
+
+# ----------------------------------------------------------------------------
+# learned-stop MLP oracle: its backward pass against finite differences
+def test_mlp_oracle_gradient_finite_differences():
+    import mlp_oracle as mo
+    rng = np.random.default_rng(2)
+    P = mo.init_params(3)[:mo.N_PARAMS].astype(np.float64)
+    P[512:576] = rng.standard_normal(64) * 0.1
+    X = rng.standard_normal((40, 8))
+    Y = (rng.random(40) < 0.5).astype(np.float64)
+    _, _, g = mo.forward_backward(P, X, Y)
+    for i in rng.choice(mo.N_PARAMS, 60, replace=False).tolist() + [4800, 0, 600, 4740]:
+        e = np.zeros_like(P)
+        e[i] = 1e-6
+        fd = (mo.loss_only(P + e, X, Y) - mo.loss_only(P - e, X, Y)) / 2e-6
+        assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd), i
+
+
+def test_mlp_oracle_bf16_rounding():
+    import mlp_oracle as mo
+    # 1 + 2^-8 is halfway between 1 and 1 + 2^-7: ties go to the even 1.0;
+    # 1 + 3*2^-8 is halfway between 1 + 2^-7 (odd) and 1 + 2^-6 (even)
+    x = np.array([1.0, 1.00390625, 1.01171875, 1.0 + 2 ** -8 + 2 ** -12, -3.0e-5], np.float32)
+    r = mo.bf16(x)
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.015625 and r[3] == 1.0078125
+    assert (np.asarray(r, np.float32).view(np.uint32) & 0xFFFF == 0).all()
