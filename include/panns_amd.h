@@ -55,7 +55,8 @@ typedef struct {
     uint32_t num_threads;  /* PSS_V5: the reference's num_threads_; only 1 is offered */
     uint32_t stop;         /* panns_stop */
     float stop_threshold;  /* LEARNED: stop when sigmoid output >= threshold */
-    uint32_t hash_slots;   /* 0 = automatic; else power of two, LDS visited-hash slots per query */
+    uint32_t hash_slots;   /* 0 = automatic; else a power of two in [64, 32768]: 16-bit slots of
+                              the per-query LDS visited table (raised when the id space needs) */
     uint32_t reserved[4];
 } panns_params;
 

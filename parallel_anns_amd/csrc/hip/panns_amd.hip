@@ -172,17 +172,18 @@ int check_params(panns_index *h, const panns_params *p, uint32_t L, uint32_t K)
     if (p->M == 0) return fail(PANNS_ERR_ARG, "M must be positive");
     if (p->M > kMaxM) return fail(PANNS_ERR_UNSUPPORTED, "M above 1024 is not offered");
     if (p->stop == PANNS_STOP_LEARNED && !h->has_model) return fail(PANNS_ERR_STATE, "no stop model loaded");
-    if (p->hash_slots && ((p->hash_slots & (p->hash_slots - 1)) || p->hash_slots < 64 || p->hash_slots > 16384))
-        return fail(PANNS_ERR_ARG, "hash_slots must be a power of two in [64, 16384]");
+    if (p->hash_slots && ((p->hash_slots & (p->hash_slots - 1)) || p->hash_slots < 64 || p->hash_slots > 32768))
+        return fail(PANNS_ERR_ARG, "hash_slots must be a power of two in [64, 32768]");
     return PANNS_OK;
 }
 
 // Layout of one workgroup's LDS; must match search_kernel's carving:
 // Q[Lcap] cand[nb_cap] bb[64] (8 B each), query row (4*dim B),
-// hash[hash_slots] newids[nb_cap] picks[Mcap] counts[4] h1[64] (4 B each).
-uint32_t lds_bytes(uint32_t Lcap, uint32_t nb_cap, uint32_t hash_slots, uint32_t Mcap, uint32_t dim)
+// visited table (2 B per slot), newids[nb_cap] picks[Mcap] counts[4] h1[64]
+// (4 B each).
+uint32_t lds_bytes(uint32_t Lcap, uint32_t nb_cap, uint32_t tbl_slots, uint32_t Mcap, uint32_t dim)
 {
-    return 8u * (Lcap + nb_cap + 64u) + 4u * dim + 4u * (hash_slots + nb_cap + Mcap + 4u) + 4u * 64u;
+    return 8u * (Lcap + nb_cap + 64u) + 4u * dim + 2u * tbl_slots + 4u * (nb_cap + Mcap + 4u) + 4u * 64u;
 }
 
 int ensure_scratch(panns_index *h, uint32_t grid)
@@ -236,9 +237,18 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     const uint32_t stride = 1u << h->stride_log2;
     const uint32_t nb_cap = std::max<uint32_t>(panns::kGroupSlots, stride);
     const uint32_t Mcap = std::max<uint32_t>(4u, (p.M + 3u) & ~3u);
+    // visited table: hs 16-bit slots in buckets of 8; the tag of an id has
+    // id_bits - lbits bits, which must stay <= 15 (0xFFFF marks empty)
     uint32_t hs = p.hash_slots;
-    if (!hs) hs = pl->maxL <= 512 ? 2048u : 4096u;
-    if (pl->shared_bitmap) hs = 64;
+    if (!hs) hs = pl->maxL <= 512 ? 8192u : 16384u;
+    const uint32_t id_bits = std::max<uint32_t>(1u, log2u(h->n));
+    uint32_t lbits = log2u(hs / 8u);
+    if (id_bits > lbits + 15u) lbits = id_bits - 15u;
+    hs = 8u << lbits;
+    if (pl->shared_bitmap) {
+        hs = 64;
+        lbits = 3;
+    }
     pl->hash_slots = hs;
     // Lcap rounded to 8 keys keeps every LDS region 16-byte aligned.
     const uint32_t Lcap = (pl->maxL + 7u) & ~7u;
@@ -316,9 +326,10 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     a.spill_log = h->spill_log.p;
     a.log_cap = kLogCap;
     a.Lcap = Lcap;
-    a.hash_slots = hs;
-    a.hash_shift = 32u - log2u(hs);
-    a.hash_limit = pl->shared_bitmap ? 0u : (hs / 4u) * 3u;
+    a.tbl_words = hs / 2u;
+    a.tbl_lbits = lbits;
+    a.id_bits = id_bits;
+    a.use_table = pl->shared_bitmap ? 0u : 1u;
     a.Mcap = Mcap;
     a.nb_cap = nb_cap;
     a.group_picks = std::max<uint32_t>(1u, nb_cap / stride);

@@ -7,13 +7,12 @@
 //           (float bits << 32 | id) ordered exactly like Candidate::operator<
 //           (include/Candidate.h:19-27: distance, then id); bit 31 of the low
 //           word is is_checked_ (ids < 2^31), masked off for comparisons.
-//   hash    open-addressing visited set (linear probing, never deleted);
-//           once it holds `hash_limit` ids or a probe run is exhausted, new
-//           ids go to a per-workgroup global bitmap whose touched words are
-//           logged and cleared when the query ends.  An id is claimed "new"
-//           exactly once, so visited = init ids plus every vertex whose
-//           distance was computed, as boost::dynamic_bitset gives it in
-//           PSS5LG.cpp:1017-1020.
+//   tbl     visited table: buckets of eight 16-bit tags (see Visited); an
+//           id whose bucket is full goes to a per-workgroup global bitmap
+//           whose touched words are logged and cleared when the query ends.
+//           An id is claimed "new" exactly once, so visited = init ids plus
+//           every vertex whose distance was computed, as boost::dynamic_bitset
+//           gives it in PSS5LG.cpp:1017-1020.
 //   qs      the query row
 //   newids  neighbour ids newly claimed in the current pick group
 //   cand    keys of new neighbours that beat the current L-th key
@@ -44,9 +43,8 @@ constexpr uint32_t kSent = 0xFFFFFFFFu;
 constexpr uint64_t kChk = 0x80000000ull;
 constexpr uint64_t kKeyMax = ~0ull;
 constexpr uint32_t kIdMask = 0x7FFFFFFFu;
-constexpr int kProbe = 16;
-constexpr int kClaimChunks = 8;      // 64-slot chunks claimed per batch
-constexpr uint32_t kGroupSlots = 512;  // neighbour slots per pick group
+constexpr int kClaimChunks = 4;        // 64-slot chunks claimed per batch
+constexpr uint32_t kGroupSlots = 256;  // neighbour slots per pick group
 
 enum : uint32_t { kModeTopM = 0, kModePssV5 = 1, kModeScaleM = 2 };
 
@@ -86,7 +84,8 @@ struct KArgs {
     uint64_t bitmap_words;
     uint32_t *spill_log;  // [grid][log_cap]
     uint32_t log_cap;
-    uint32_t Lcap, hash_slots, hash_shift, hash_limit;
+    uint32_t Lcap;
+    uint32_t tbl_words, tbl_lbits, id_bits, use_table;  // LDS visited table
     uint32_t Mcap, nb_cap, group_picks;
     // subsearch on a caller-held queue and visited bitmap
     const uint64_t *init_queue;  // [n_work][L] keys, or null
@@ -294,45 +293,71 @@ __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32
     return p0;
 }
 
-// Visited set of one query: LDS hash first, per-workgroup global bitmap for
-// what the hash does not hold.  fill and spills are wave-uniform counters kept
-// in registers (counted with ballots, so no lane contends on one address).
+// Visited set of one query: an LDS tag table first, a per-workgroup global
+// bitmap for what the table cannot hold.
+//
+// The table has 2^lbits buckets of eight 16-bit slots (one 16-byte LDS read
+// per lookup).  An id is scrambled by a bijection of [0, 2^idbits)
+// (multiplication by an odd constant, then an xor-shift); the low lbits of the
+// result pick the bucket and the remaining idbits - lbits <= 15 bits are the
+// tag stored in it (0xFFFF marks an empty slot), so bucket and tag together
+// identify the id exactly.  Slots are only ever filled during a query, so
+// once a bucket is full every later id of that bucket whose tag is absent is
+// decided by the bitmap, and an id is never both in the table and in the
+// bitmap.  spills is a wave-uniform count kept in registers.
 struct Visited {
-    uint32_t *hash;     // LDS
-    uint32_t mask, shift, limit;
+    uint32_t *tbl;      // LDS: buckets x 4 words
+    uint32_t bmask, lbits, idbits, idmask, xshift, use;
     uint32_t *bitmap;   // global
     uint32_t *log;      // global: words touched in the bitmap, may be null
     uint32_t log_cap;
-    uint32_t fill;      // ids stored in the hash
     uint32_t spills;    // ids stored in the bitmap
 
-    // 0: visited before; 1: new and now in the hash; 2: not in the hash, which
-    // takes no more ids on this probe run -- the bitmap decides.
-    __device__ __forceinline__ int probe(uint32_t id, bool allow) const
+    __device__ __forceinline__ uint32_t scramble(uint32_t id) const
     {
-        uint32_t slot = (id * 2654435761u) >> shift;
+        uint32_t h = (id * 0x9E3779B1u) & idmask;
+        return h ^ (h >> xshift);
+    }
+
+    // 0: visited before; 1: new and now in the table; 2: not in the table,
+    // whose bucket is full -- the bitmap decides.
+    __device__ __forceinline__ int probe(uint32_t id) const
+    {
+        const uint32_t h = scramble(id);
+        const uint32_t tag = h >> lbits;
+        uint32_t *w = tbl + ((h & bmask) << 2);
+        // every failed CAS means another slot of the bucket was filled, so
+        // at most 8 retries precede a full bucket or a success
 #pragma unroll 1
-        for (int p = 0; p < kProbe; ++p) {
-            const uint32_t cur = hash[slot];
-            if (cur == id) return 0;
-            if (cur == kSent) {
-                if (!allow) return 2;
-                const uint32_t old = atomicCAS(&hash[slot], kSent, id);
-                if (old == kSent) return 1;
-                if (old == id) return 0;
+        for (int attempt = 0; attempt < 9; ++attempt) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(w);
+            const uint32_t words[4] = {v.x, v.y, v.z, v.w};
+            int ew = -1;
+            uint32_t eshift = 0, eold = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t lo = words[i] & 0xFFFFu, hi = words[i] >> 16;
+                if (lo == tag || hi == tag) return 0;
+                if (ew < 0 && (lo == 0xFFFFu || hi == 0xFFFFu)) {
+                    ew = i;
+                    eshift = lo == 0xFFFFu ? 0u : 16u;
+                    eold = words[i];
+                }
             }
-            slot = (slot + 1) & mask;
+            if (ew < 0) return 2;
+            const uint32_t nw = (eold & ~(0xFFFFu << eshift)) | (tag << eshift);
+            if (atomicCAS(&w[ew], eold, nw) == eold) return 1;
         }
         return 2;
     }
 };
 
-// Claims up to kClaimChunks ids per lane (kSent = empty slot): hash probes
-// chunk by chunk, then one batch of bitmap atomics for the ids the hash could
+// Claims up to kClaimChunks ids per lane (kSent = empty slot): table probes
+// chunk by chunk, then one batch of bitmap atomics for the ids the table could
 // not take.  Appends the newly visited ids to newids[nnew...] in chunk order
 // and returns the new count.  An id offered twice is claimed once: within a
-// chunk by the hash CAS or the atomic, across chunks because the hash only
-// grows and the bitmap atomics are exact.
+// chunk by the table CAS or the atomic, across chunks because table slots
+// only fill and the bitmap atomics are exact.
 // `okbits` marks the chunks where this lane's value is a slot at all; the
 // values themselves are only read here, so the loads that produced them stay
 // in flight until the first probe.
@@ -348,9 +373,8 @@ __device__ __forceinline__ uint32_t claim_chunks(Visited &vis, uint32_t (&ids)[k
             int r = 0;
             if (ids[u] != kSent) {
                 ++nadj;
-                r = vis.probe(ids[u], vis.fill < vis.limit);
+                r = vis.use ? vis.probe(ids[u]) : 2;
             }
-            vis.fill += (uint32_t)__popcll(__ballot(r == 1));
             if (r == 1) nbits |= 1u << u;
             if (r == 2) gbits |= 1u << u;
         }
@@ -483,18 +507,21 @@ __global__ __launch_bounds__(64, 3) void search_kernel(KArgs a)
     uint64_t *cand = Q + a.Lcap;
     uint64_t *bb = cand + a.nb_cap;
     float4 *qs = reinterpret_cast<float4 *>(bb + 64);
-    uint32_t *hash = reinterpret_cast<uint32_t *>(qs + D / 4);
-    uint32_t *newids = hash + a.hash_slots;
+    uint32_t *tbl = reinterpret_cast<uint32_t *>(qs + D / 4);
+    uint32_t *newids = tbl + a.tbl_words;
     uint32_t *picks = newids + a.nb_cap;
     uint32_t *counts = picks + a.Mcap;     // [2] work item
     float *h1s = reinterpret_cast<float *>(counts + 4);
 
     const uint32_t stride = 1u << a.stride_log2;
     Visited vis;
-    vis.hash = hash;
-    vis.mask = a.hash_slots - 1;
-    vis.shift = a.hash_shift;
-    vis.limit = a.hash_limit;
+    vis.tbl = tbl;
+    vis.bmask = (1u << a.tbl_lbits) - 1u;
+    vis.lbits = a.tbl_lbits;
+    vis.idbits = a.id_bits;
+    vis.idmask = a.id_bits >= 32 ? ~0u : (1u << a.id_bits) - 1u;
+    vis.xshift = a.id_bits / 2 + 1;
+    vis.use = a.use_table;
     vis.bitmap = a.shared_bitmap ? a.shared_bitmap : a.bitmaps + (size_t)blockIdx.x * a.bitmap_words;
     vis.log = a.shared_bitmap ? nullptr : a.spill_log + (size_t)blockIdx.x * a.log_cap;
     vis.log_cap = a.log_cap;
@@ -513,8 +540,7 @@ __global__ __launch_bounds__(64, 3) void search_kernel(KArgs a)
         const uint32_t K = a.qK ? a.qK[w] : a.K;
 
         if (lane < D / 4) qs[lane] = reinterpret_cast<const float4 *>(a.queries + (size_t)qid * D)[lane];
-        for (uint32_t i = lane; i < a.hash_slots; i += 64) hash[i] = kSent;
-        vis.fill = 0;
+        for (uint32_t i = lane; i < a.tbl_words; i += 64) tbl[i] = kSent;
         vis.spills = 0;
         __syncthreads();
 
