@@ -21,7 +21,11 @@ resident in HBM (one persistent-kernel launch).
 
 `value` is the learned-stop QPS; the fixed-L QPS, their ratio and the bound
 any per-query stop rule could reach on the measured L grid are reported
-beside it.  --gpus above 1 exits with a message: this build targets one GPU.
+beside it, with the learned stop's per-query recall deficit against the
+fixed search at the same L_max (quantiles and a histogram).  The recall
+targets 0.90 and 0.99 are evaluated too (3 timed launches each) under
+config.other_targets.  --gpus above 1 exits with a message: this build
+targets one GPU.
 """
 import argparse
 import json
@@ -48,6 +52,28 @@ def algorithmic_bytes(res, dim, K, nq):
     c = res["counters"]
     return (4.0 * dim * c["distance_computations"] + 4.0 * (c["expansions"] + c["adjacency_slots"])
             + nq * (4.0 * dim + 8.0 * K))
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "search_kernel_traffic.json")
+
+
+def measured_traffic(config, kernel, point):
+    """HBM bytes per launch measured for this workload, kernel variant and
+    operating point by rocprofv3 FETCH_SIZE and WRITE_SIZE passes (each
+    scaled by its copy-kernel factor), as recorded in TRAFFIC_FILE; the
+    counters cannot be read from inside the process.  (None, None) when no
+    record matches."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            recs = json.load(f)["records"]
+    except (OSError, ValueError, KeyError):
+        return None, None
+    L = point.get("L_max", point.get("L"))
+    for r in recs:
+        if (r["config"] == config and r["kernel"] == kernel and r["M"] == point["M"] and r["L"] == L
+                and abs(r.get("threshold", 0.0) - point.get("threshold", 0.0)) < 1e-6):
+            return r["traffic_bytes"], f"profiles/search_kernel_traffic.json ({r['command']})"
+    return None, None
 
 
 def time_plan(plan, steps, warmup):
@@ -100,6 +126,56 @@ def cpu_baseline(ds, K, L, M, init_ids, budget_s=12.0):
                       f"to the GPU fixed-L search)"}, ids
 
 
+def fixed_operating_point(index, qids, K, gt, target, ms, nq, op=None, hi=1024):
+    """Fastest (M, L) whose recall reaches target: L by bisection per M, each
+    candidate timed over 3 launches."""
+    if op:
+        cands = [op]
+    else:
+        cands = []
+        for M in ms:
+            L = find_L(index, qids, K, M, gt, target, K, hi)
+            log(f"target {target}: find-L M={M}: L={L}")
+            if L is not None:
+                cands.append((M, L))
+    best = None
+    for M, L in cands:
+        plan = index.plan(qids, K, L, M, want_queue=False)
+        wall, _ = time_plan(plan, 3, 1)
+        plan.close()
+        qps = 3 * nq / wall
+        log(f"target {target}: fixed candidate M={M} L={L}: {qps:,.0f} QPS")
+        if best is None or qps > best[0]:
+            best = (qps, M, L)
+    return None if best is None else best[1:]
+
+
+def timed(index, qids, K, L, M, gt, steps, warmup, **kw):
+    plan = index.plan(qids, K, L, M, want_queue=False, **kw)
+    desc = plan.describe()
+    wall, kms = time_plan(plan, steps, warmup)
+    res = plan.fetch()
+    plan.close()
+    from parallel_anns_amd import tools
+    return res, desc, {"M": M, "L": L, "qps": round(steps * qids.shape[0] / wall, 1),
+                       "recall": round(tools.recall(gt, res["ids"], L), 4),
+                       "kernel_ms": round(float(np.mean(kms)), 4),
+                       "ndist": round(float(res["ndist"].mean()), 1)}
+
+
+def deficit_distribution(d):
+    """Per-query recall deficit (fixed search at the same L minus learned
+    stop), as quantiles and a histogram."""
+    d = np.asarray(d, np.float64)
+    edges = [0.0, 1e-9, 0.01, 0.02, 0.05, 0.1, 0.2, 1.0 + 1e-9]
+    h, _ = np.histogram(d, bins=edges)
+    return {"mean": round(float(d.mean()), 5), "frac_queries_with_loss": round(float((d > 1e-9).mean()), 4),
+            "p50": round(float(np.quantile(d, 0.5)), 4), "p90": round(float(np.quantile(d, 0.9)), 4),
+            "p99": round(float(np.quantile(d, 0.99)), 4), "max": round(float(d.max()), 4),
+            "histogram": {"bins": ["0", "(0,0.01)", "[0.01,0.02)", "[0.02,0.05)", "[0.05,0.1)", "[0.1,0.2)",
+                                   "[0.2,1]"], "queries": [int(x) for x in h]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +184,8 @@ def main():
     ap.add_argument("--config", default="c1")
     ap.add_argument("--cache-dir", default=os.environ.get("PANNS_CACHE", "/tmp/panns_cache"))
     ap.add_argument("--target-recall", type=float, default=0.95)
+    ap.add_argument("--other-targets", default="0.90,0.99",
+                    help="further recall targets evaluated (3 timed launches each); empty for none")
     ap.add_argument("--ms", default="2,4,8", help="M values tried for the fixed-L operating point")
     ap.add_argument("--op", default=None, help="M,L: fixed-L operating point (skips its search)")
     ap.add_argument("--learned-op", default=None, help="M,Lmax,threshold: skips training/calibration "
@@ -115,7 +193,8 @@ def main():
     ap.add_argument("--train-queries", type=int, default=100_000)
     ap.add_argument("--val-queries", type=int, default=5_000)
     ap.add_argument("--train-L", type=int, default=400)
-    ap.add_argument("--learned-L", default="300,400")
+    ap.add_argument("--learned-L", default="1.0,1.25,1.5",
+                    help="L_max candidates for the learned stop, as multiples of the fixed-L operating L")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
@@ -140,38 +219,22 @@ def main():
     qids = np.arange(nq, dtype=np.uint32)
     gt = ds["gt_ids"]
     target = args.target_recall
+    others = [float(x) for x in args.other_targets.split(",") if x.strip()] if not args.op else []
+    ms = [int(x) for x in args.ms.split(",")]
 
-    # ---- fixed-L operating point
-    if args.op:
-        M, L = (int(x) for x in args.op.split(","))
-        cands = [(M, L)]
-    else:
-        cands = []
-        for M in (int(x) for x in args.ms.split(",")):
-            L = find_L(index, qids, K, M, gt, target, K, 1024)
-            log(f"find-L M={M}: L={L}")
-            if L is not None:
-                cands.append((M, L))
-    best = None
-    for M, L in cands:
-        plan = index.plan(qids, K, L, M)
-        wall, _ = time_plan(plan, 3, 1)
-        qps = 3 * nq / wall
-        log(f"fixed candidate M={M} L={L}: {qps:,.0f} QPS")
-        if best is None or qps > best[0]:
-            best = (qps, M, L)
-        plan.close()
-    _, fM, fL = best
-    plan = index.plan(qids, K, fL, fM)
-    wall, kms = time_plan(plan, args.steps, args.warmup)
-    fres = plan.fetch()
-    plan.close()
-    fixed = {"M": fM, "L": fL, "qps": round(args.steps * nq / wall, 1),
-             "recall": round(tools.recall(gt, fres["ids"], fL), 4), "kernel_ms": round(float(np.mean(kms)), 4),
-             "ndist": round(float(fres["ndist"].mean()), 1)}
+    # ---- fixed-L operating points
+    op = tuple(int(x) for x in args.op.split(",")) if args.op else None
+    fM, fL = fixed_operating_point(index, qids, K, gt, target, ms, nq, op)
+    fres, _, fixed = timed(index, qids, K, fL, fM, gt, args.steps, args.warmup)
     log(f"fixed-L: {fixed}")
+    fixed_other = {}
+    for t in others:
+        o = fixed_operating_point(index, qids, K, gt, t, ms, nq, hi=2048)
+        if o is not None:
+            fixed_other[t] = timed(index, qids, K, o[1], o[0], gt, 3, 1)[2]
+            log(f"fixed-L at {t}: {fixed_other[t]}")
 
-    # ---- per-query bound on the fixed-L grid
+    # ---- per-query bound on an L grid around the operating point
     grid = sorted({max(K, int(fL * f)) for f in (0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.15, 1.3, 1.6, 2.0)})
     R, N = [], []
     for Lg in grid:
@@ -182,26 +245,25 @@ def main():
     oracle = None if bound is None else {
         "ndist": round(bound[0], 1), "recall": round(bound[1], 4),
         "distance_ratio_vs_fixed": round(fixed["ndist"] / bound[0], 3),
-        "grid_L": grid}
+        "grid_L": grid, "method": "per-query L from the grid, Lagrangian relaxation"}
     log(f"per-query bound: {oracle}")
 
     # ---- learned stop
-    lrn = None
-    main_res, main_ms, main_desc, main_recall = fres, fixed["kernel_ms"], None, fixed["recall"]
-    main_qps = fixed["qps"]
+    lrn, lres, ldesc, lrn_other = None, None, None, {}
     if not args.no_learned:
         model_path = os.path.join(ds["dir"] or args.cache_dir, "stop_model.npy")
+        lM = fM
         if args.learned_op:
             lM, lL, thr = args.learned_op.split(",")
             lM, lL, thr = int(lM), int(lL), float(thr)
             model = np.load(model_path)
             train_info = {"reused": model_path}
+            picks = {target: (lL, thr)}
         else:
             t0 = time.time()
             ntr, nval = args.train_queries, args.val_queries
             tq = datasets.generate(cfg, "train", n=ntr + nval)
             index.set_queries(tq[:ntr])
-            lM = fM
             feat, hsh, niter, _ = index.search_trace(np.arange(ntr, dtype=np.uint32), K, args.train_L, lM,
                                                      trace_cap=192)
             X, Y = learned.samples_from_trace(feat, hsh, niter)
@@ -219,29 +281,46 @@ def main():
             vgt, _ = index.exact_knn(K, source="queries")
             vq = np.arange(nval, dtype=np.uint32)
             index.set_stop_model(model)
-            best_l = None
-            for lL in (int(x) for x in args.learned_L.split(",")):
-                thr = learned.calibrate(index, vq, vgt, K, lL, lM, target + 0.002)
-                if thr is None:
-                    continue
-                rv, nd, _ = learned.recall_work(index, vq, vgt, K, lL, lM, thr)
-                log(f"calibrated L={lL} threshold {thr:.4g}: validation recall {rv:.4f}, ndist {nd:.0f}")
-                if best_l is None or nd < best_l[0]:
-                    best_l = (nd, lL, thr)
-            _, lL, thr = best_l
+            picks = {}
+            for t in [target] + sorted(fixed_other):
+                Lf = fL if t == target else fixed_other[t]["L"]
+                best_l = None
+                for f in (float(x) for x in args.learned_L.split(",")):
+                    lL = min(4096, int(round(Lf * f)))
+                    thr = learned.calibrate(index, vq, vgt, K, lL, lM, t + 0.002)
+                    if thr is None:
+                        continue
+                    rv, nd, _ = learned.recall_work(index, vq, vgt, K, lL, lM, thr)
+                    log(f"target {t}: calibrated L={lL} threshold {thr:.4g}: validation recall {rv:.4f}, ndist {nd:.0f}")
+                    if best_l is None or nd < best_l[0]:
+                        best_l = (nd, lL, thr)
+                if best_l is not None:
+                    picks[t] = best_l[1:]
             index.set_queries(ds["queries"])
         index.set_stop_model(model)
-        plan = index.plan(qids, K, lL, lM, stop="learned", stop_threshold=thr)
-        desc = plan.describe()
-        wall, kms = time_plan(plan, args.steps, args.warmup)
-        lres = plan.fetch()
-        plan.close()
-        lrn = {"M": lM, "L_max": lL, "threshold": thr, "qps": round(args.steps * nq / wall, 1),
-               "recall": round(tools.recall(gt, lres["ids"], lL), 4), "kernel_ms": round(float(np.mean(kms)), 4),
-               "ndist": round(float(lres["ndist"].mean()), 1), "training": train_info}
-        log(f"learned: {lrn}")
-        main_res, main_ms, main_desc, main_recall, main_qps = lres, lrn["kernel_ms"], desc, lrn["recall"], lrn["qps"]
+        if target in picks:
+            lL, thr = picks[target]
+            lres, ldesc, lrn = timed(index, qids, K, lL, lM, gt, args.steps, args.warmup, stop="learned",
+                                     stop_threshold=thr)
+            lrn = {"M": lM, "L_max": lL, "threshold": thr, **{k: v for k, v in lrn.items() if k not in ("M", "L")},
+                   "training": train_info}
+            fmax = index.search(qids, K, lL, lM)
+            lrn["recall_deficit_vs_fixed_at_L_max"] = deficit_distribution(
+                tools.recall_per_query(gt, fmax["ids"], lL) - tools.recall_per_query(gt, lres["ids"], lL))
+            log(f"learned: {lrn}")
+        for t in sorted(fixed_other):
+            if t in picks:
+                lL, thr = picks[t]
+                r = timed(index, qids, K, lL, lM, gt, 3, 1, stop="learned", stop_threshold=thr)[2]
+                lrn_other[t] = {"fixed_L": fixed_other[t],
+                                "learned": {"L_max": lL, "threshold": thr, "qps": r["qps"], "recall": r["recall"],
+                                            "ndist": r["ndist"]},
+                                "learned_over_fixed": round(r["qps"] / fixed_other[t]["qps"], 3)}
+                log(f"target {t}: {lrn_other[t]}")
 
+    main_res, main_qps, main_recall = (lres, lrn["qps"], lrn["recall"]) if lrn else (fres, fixed["qps"],
+                                                                                   fixed["recall"])
+    main_ms = lrn["kernel_ms"] if lrn else fixed["kernel_ms"]
     B = algorithmic_bytes(main_res, dim, K, nq)
     achieved = B / (main_ms * 1e-3) / 1e9
     cpu = None
@@ -250,6 +329,8 @@ def main():
         cpu, cpu_ids = cpu_baseline(ds, K, fL, fM, init)
         cpu["ids_identical_to_gpu"] = bool((cpu_ids == fres["ids"][:cpu_ids.shape[0]]).all())
         log(f"cpu baseline {cpu['value']:,.0f} QPS on {cpu['cores']} cores; ids identical: {cpu['ids_identical_to_gpu']}")
+    kernel = f"search_kernel<{dim}, {'true' if lrn else 'false'}>"
+    traffic, traffic_src = measured_traffic(args.config, kernel, lrn or fixed)
 
     out = {
         "metric": METRIC,
@@ -273,11 +354,14 @@ def main():
                    "fixed_L": fixed, "learned": lrn,
                    "learned_over_fixed": round(lrn["qps"] / fixed["qps"], 3) if lrn else None,
                    "per_query_stop_bound": oracle,
-                   "workgroups": (main_desc or {}).get("workgroups"),
-                   "lds_bytes": (main_desc or {}).get("lds_bytes")},
+                   "other_targets": lrn_other or ({str(t): {"fixed_L": v} for t, v in fixed_other.items()}
+                                                  if fixed_other else None),
+                   "workgroups": (ldesc or {}).get("workgroups"),
+                   "lds_bytes": (ldesc or {}).get("lds_bytes")},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "search_kernel<96>", "kernel_ms": round(main_ms, 4),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "kernel": kernel, "kernel_ms": round(main_ms, 4),
                      "algorithmic_bytes_per_launch": int(B)},
         "cpu_baseline": cpu,
     }

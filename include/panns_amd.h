@@ -156,6 +156,47 @@ int panns_plan_destroy(panns_plan *plan);
 int panns_plan_describe(panns_plan *plan, uint32_t *workgroups, uint32_t *lds_bytes,
                         uint32_t *hash_slots, uint32_t *max_L);
 
+/* Uneven batches (no reference counterpart; the reference runs one query
+ * per call).  Queries are grouped by L into queue-size buckets, each bucket
+ * one persistent launch sized for its own largest L (LDS, visited table,
+ * occupancy), its queries taken longest-first from its device work counter.
+ * The buckets run one after another on the handle's stream or, with
+ * `concurrent`, side by side on streams of their own; with `use_graph` the
+ * whole batch (counter resets and launches) is captured once as a hipGraph
+ * and replayed by panns_plan_run.  Results come back in the caller's order
+ * and equal those of a single-bucket plan. */
+#define PANNS_MAX_BUCKETS 8
+typedef struct {
+    uint32_t n_buckets;                        /* 1..PANNS_MAX_BUCKETS (0 means 1) */
+    uint32_t bucket_max_L[PANNS_MAX_BUCKETS];  /* ascending upper bounds of L; all 0: split the
+                                                  queries into buckets of equal summed L */
+    uint32_t table_shift;                      /* visited-table size halved this many times (0-2)
+                                                  from the automatic size, for every bucket */
+    uint32_t use_graph;                        /* capture and replay as one hipGraph */
+    uint32_t concurrent;                       /* buckets side by side */
+    uint32_t want_queue;                       /* keep the final queues for panns_plan_fetch */
+    uint32_t reserved[4];
+} panns_plan_options;
+
+int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *query_ids,
+                         uint32_t nq, const uint32_t *L_per_query, const uint32_t *K_per_query,
+                         const uint32_t *init_ids, uint32_t n_init, const panns_plan_options *opt,
+                         panns_plan **out);
+/* Per bucket b < *n_buckets: largest L, queries, workgroups, LDS bytes,
+ * visited-table slots (arrays of PANNS_MAX_BUCKETS, each nullable). */
+int panns_plan_describe_buckets(panns_plan *plan, uint32_t *n_buckets, uint32_t *max_L, uint32_t *nq,
+                                uint32_t *workgroups, uint32_t *lds_bytes, uint32_t *hash_slots);
+/* The autotuner: times candidate bucketings (1, 2, 3, 4, 6 buckets of equal
+ * summed L; side by side or not; full or halved visited tables; always as
+ * a graph) on every `sample_stride`-th query of the batch (0: a stride that
+ * leaves about 20,000 queries) and returns the fastest in *best, with its
+ * bucket bounds filled in, and its device time on the sample in *best_ms
+ * (nullable).  Results do not depend on the choice. */
+int panns_plan_autotune(panns_index *h, const panns_params *p, const uint32_t *query_ids, uint32_t nq,
+                        const uint32_t *L_per_query, const uint32_t *K_per_query,
+                        const uint32_t *init_ids, uint32_t n_init, uint32_t sample_stride,
+                        panns_plan_options *best, float *best_ms);
+
 /* Exact k-NN by brute force on the GPU with the reference's distance and
  * (distance, id) order: the semantics of tools/save_true_nearest_neighbors.cpp:16-35
  * (get_true_NN).  source 0: the base rows are the queries and each row's own
@@ -170,8 +211,8 @@ int panns_exact_knn(panns_index *h, int source, uint32_t k, uint32_t *out_ids, f
  *
  * panns_search_trace: the fixed-L search of panns_search, also recording
  * after every iteration s < trace_cap the 8 stop features
- *   [iteration, top-1 dist, K-th dist, K-th/top-1, keys inserted in the
- *    iteration, best initial dist, query norm, unchecked entries]
+ *   [iteration, top-1 dist, K-th dist, K-th/top-1, new keys that beat the
+ *    queue's last key in the iteration, best initial dist, query norm, unchecked entries]
  * into feat[(q*trace_cap + s)*8 ...] and an order-free hash of the top-K id
  * set into topk_hash[q*trace_cap + s]; niter[q] receives the iteration count.
  * A step is labelled "final" when its hash equals the last step's.

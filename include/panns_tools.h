@@ -44,6 +44,10 @@ float pt_l2(const float *v, const float *q, uint32_t dim);
 int pt_gen_mixture(uint64_t seed_centers, uint64_t seed_rows, uint64_t row0, uint32_t n,
                    uint32_t dim, uint32_t n_comp, float sigma, int normalize, float *out,
                    int nthreads);
+/* The mixture component of rows [row0, row0+n) of pt_gen_mixture's stream
+ * seed_rows (the first draw of each row's generator). */
+int pt_mixture_labels(uint64_t seed_rows, uint64_t row0, uint32_t n, uint32_t n_comp, uint32_t *labels,
+                      int nthreads);
 
 /* fvecs */
 int pt_fvecs_info(const char *path, uint32_t *n, uint32_t *dim);

@@ -41,11 +41,20 @@ class PannsOutputs(ctypes.Structure):
                 ("queue_checked", vp), ("ndist", vp), ("nexp", vp), ("niter", vp)]
 
 
+MAX_BUCKETS = 8
+
+
+class PannsPlanOptions(ctypes.Structure):
+    _fields_ = [("n_buckets", c_u32), ("bucket_max_L", c_u32 * MAX_BUCKETS), ("table_shift", c_u32),
+                ("use_graph", c_u32), ("concurrent", c_u32), ("want_queue", c_u32), ("reserved", c_u32 * 4)]
+
+
 # name -> (restype, argtypes) for include/panns_tools.h
 TOOLS_API = {
     "pt_last_error": (c_char_p, []),
     "pt_l2": (c_float, [vp, vp, c_u32]),
     "pt_gen_mixture": (c_int, [c_u64, c_u64, c_u64, c_u32, c_u32, c_u32, c_float, c_int, vp, c_int]),
+    "pt_mixture_labels": (c_int, [c_u64, c_u64, c_u32, c_u32, vp, c_int]),
     "pt_fvecs_info": (c_int, [c_char_p, P(c_u32), P(c_u32)]),
     "pt_fvecs_read": (c_int, [c_char_p, vp, c_u32, c_u32]),
     "pt_fvecs_write": (c_int, [c_char_p, vp, c_u32, c_u32]),
@@ -84,6 +93,11 @@ AMD_API = {
     "panns_plan_fetch": (c_int, [vp, P(PannsOutputs), P(PannsCounters)]),
     "panns_plan_destroy": (c_int, [vp]),
     "panns_plan_describe": (c_int, [vp, P(c_u32), P(c_u32), P(c_u32), P(c_u32)]),
+    "panns_plan_create_ex": (c_int, [vp, P(PannsParams), vp, c_u32, vp, vp, vp, c_u32, P(PannsPlanOptions),
+                                     P(vp)]),
+    "panns_plan_describe_buckets": (c_int, [vp, P(c_u32), vp, vp, vp, vp, vp]),
+    "panns_plan_autotune": (c_int, [vp, P(PannsParams), vp, c_u32, vp, vp, vp, c_u32, c_u32,
+                                    P(PannsPlanOptions), P(c_float)]),
     "panns_exact_knn": (c_int, [vp, c_int, c_u32, vp, vp]),
     "panns_search_trace": (c_int, [vp, P(PannsParams), vp, c_u32, vp, c_u32, c_u32, vp, vp, vp, vp]),
     "panns_set_stop_model": (c_int, [vp, vp, c_u32]),

@@ -4,6 +4,8 @@
                      graph construction
   libpanns_amd.so    HIP for gfx950 (hipcc): search kernels and the C-ABI of
                      include/panns_amd.h; links libpanns_tools.so
+  bin/find_L_top_m   host C++ driver over the PANNS::Searching drop-in
+                     (include/panns/Searching.h); links libpanns_amd.so
 
 Run `python -m parallel_anns_amd.build`.  Every call recompiles.
 """
@@ -19,6 +21,8 @@ CSRC = os.path.join(PKG, "csrc")
 
 TOOLS_SO = os.path.join(PKG, "libpanns_tools.so")
 AMD_SO = os.path.join(PKG, "libpanns_amd.so")
+BIN = os.path.join(PKG, "bin")
+APPS = ("find_L_top_m",)
 
 ARCH = os.environ.get("PANNS_OFFLOAD_ARCH", "gfx950")
 
@@ -62,9 +66,21 @@ def build_hip(out=AMD_SO, defines=()):
     os.replace(tmp, out)
 
 
+def build_apps():
+    os.makedirs(BIN, exist_ok=True)
+    for app in APPS:
+        src = os.path.join(CSRC, "apps", app + ".cpp")
+        out = os.path.join(BIN, app)
+        tmp = out + ".tmp"
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INC, src, "-L", PKG, "-lpanns_amd",
+              "-Wl,-rpath,$ORIGIN/..", "-o", tmp])
+        os.replace(tmp, out)
+
+
 def build_all():
     build_tools()
     build_hip()
+    build_apps()
 
 
 if __name__ == "__main__":
@@ -73,5 +89,7 @@ if __name__ == "__main__":
         build_tools()
     if "hip" in what:
         build_hip()
+    if "apps" in what:
+        build_apps()
     if "all" in what:
         build_all()

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <string>
 #include <vector>
@@ -113,30 +114,57 @@ struct panns_plan {
     bool want_queue = false;
     uint32_t *shared_bitmap = nullptr;
     uint32_t grid = 0, lds = 0, hash_slots = 0;
+    uint32_t table_shift = 0;
+    // scratch placement: bitmap/log slots [slot0, slot0 + grid) and the work
+    // counter index, bound to the handle's buffers before every run
+    uint32_t slot0 = 0, counter_idx = 0;
     DevBuf<uint64_t> phase_cycles;
     panns::KArgs args{};
     bool ran = false;
+    // bucketed batches: parts[b] holds bucket b's queries (largest L first),
+    // part_rows[b][i] the caller's row of its work item i
+    std::vector<std::unique_ptr<panns_plan>> parts;
+    std::vector<std::vector<uint32_t>> part_rows;
+    panns_plan_options opt{};
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> side_ev;
+    hipEvent_t fork_ev = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    const void *graph_scratch = nullptr;  // bitmap buffer the graph was captured with
+
+    ~panns_plan()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : side_ev) (void)hipEventDestroy(e);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        for (hipStream_t st : side) (void)hipStreamDestroy(st);
+    }
 };
 
 namespace {
 
-template <int D>
+template <int D, bool kLearned>
 hipError_t launch_search(const panns::KArgs &a, uint32_t grid, uint32_t lds, hipStream_t s)
 {
-    hipLaunchKernelGGL(panns::search_kernel<D>, dim3(grid), dim3(64), lds, s, a);
+    hipLaunchKernelGGL((panns::search_kernel<D, kLearned>), dim3(grid), dim3(64), lds, s, a);
     return hipGetLastError();
 }
 
 using LaunchFn = hipError_t (*)(const panns::KArgs &, uint32_t, uint32_t, hipStream_t);
 using KernelPtr = const void *;
 
-bool kernel_for_dim(uint32_t dim, LaunchFn *fn, KernelPtr *kp)
+// The search kernel for a dimension; `learned` selects the variant that
+// evaluates the stop model.
+bool kernel_for_dim(uint32_t dim, bool learned, LaunchFn *fn, KernelPtr *kp)
 {
     switch (dim) {
-#define PANNS_DIM(DD)                                                  \
-    case DD:                                                           \
-        *fn = launch_search<DD>;                                       \
-        *kp = reinterpret_cast<const void *>(&panns::search_kernel<DD>); \
+#define PANNS_DIM(DD)                                                                          \
+    case DD:                                                                                   \
+        *fn = learned ? launch_search<DD, true> : launch_search<DD, false>;                     \
+        *kp = learned ? reinterpret_cast<const void *>(&panns::search_kernel<DD, true>)        \
+                      : reinterpret_cast<const void *>(&panns::search_kernel<DD, false>);      \
         return true;
         PANNS_DIM(16)
         PANNS_DIM(32)
@@ -199,7 +227,7 @@ int ensure_scratch(panns_index *h, uint32_t grid)
         h->bitmap_slots = grid;
         h->bitmap_words = words;
     }
-    if (!h->counter.p) HIP_TRY(h->counter.ensure(4));
+    if (!h->counter.p) HIP_TRY(h->counter.ensure(PANNS_MAX_BUCKETS + 8));
     return PANNS_OK;
 }
 
@@ -210,7 +238,7 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     const panns_params &p = pl->p;
     LaunchFn fn;
     KernelPtr kp;
-    if (!kernel_for_dim(h->dim, &fn, &kp))
+    if (!kernel_for_dim(h->dim, p.stop == PANNS_STOP_LEARNED, &fn, &kp))
         return fail(PANNS_ERR_DIM, "dimension " + std::to_string(h->dim) + " has no kernel (16, 32, 40, 64, 96, 128)");
     pl->maxL = p.L;
     pl->maxK = p.K;
@@ -240,7 +268,7 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     // visited table: hs 16-bit slots in buckets of 8; the tag of an id has
     // id_bits - lbits bits, which must stay <= 15 (0xFFFF marks empty)
     uint32_t hs = p.hash_slots;
-    if (!hs) hs = pl->maxL <= 512 ? 8192u : 16384u;
+    if (!hs) hs = std::max<uint32_t>(64u, (pl->maxL <= 512 ? 8192u : 16384u) >> pl->table_shift);
     const uint32_t id_bits = std::max<uint32_t>(1u, log2u(h->n));
     uint32_t lbits = log2u(hs / 8u);
     if (id_bits > lbits + 15u) lbits = id_bits - 15u;
@@ -261,7 +289,7 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     if (per_cu <= 0) return fail(PANNS_ERR_UNSUPPORTED, "kernel does not fit on a CU with this L");
     pl->grid = std::min<uint32_t>(pl->nq, (uint32_t)per_cu * (uint32_t)h->num_cu);
     if (pl->grid == 0) pl->grid = 1;
-    rc = ensure_scratch(h, pl->grid);
+    rc = ensure_scratch(h, pl->slot0 + pl->grid);
     if (rc) return rc;
 
     HIP_TRY(pl->qids.ensure(pl->nq));
@@ -314,16 +342,16 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     a.Kstride = pl->maxK;
     a.Lstride = pl->maxL;
     a.init_ids = pl->init_ids.p;
-    a.work_counter = h->counter.p;
+    a.work_counter = h->counter.p + pl->counter_idx;
     a.out_ids = pl->out_ids.p;
     a.out_dists = pl->out_dists.p;
     a.out_qids = pl->want_queue ? pl->out_qids.p : nullptr;
     a.out_qdists = pl->want_queue ? pl->out_qdists.p : nullptr;
     a.out_qchk = pl->want_queue ? pl->out_qchk.p : nullptr;
     a.out_stats = pl->out_stats.p;
-    a.bitmaps = h->bitmaps.p;
+    a.bitmaps = h->bitmaps.p + (size_t)pl->slot0 * h->bitmap_words;
     a.bitmap_words = h->bitmap_words;
-    a.spill_log = h->spill_log.p;
+    a.spill_log = h->spill_log.p + (size_t)pl->slot0 * kLogCap;
     a.log_cap = kLogCap;
     a.Lcap = Lcap;
     a.tbl_words = hs / 2u;
@@ -345,6 +373,8 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
         a.model.b2 = m + 576 + 4096;
         a.model.w3 = m + 576 + 4096 + 64;
         a.model.b3 = m + 576 + 4096 + 128;
+        a.model.mu = m + 4804;
+        a.model.sd = m + 4812;
     }
     a.trace_feat = pl->trace_cap ? pl->trace_feat.p : nullptr;
     a.trace_topk = pl->trace_cap ? pl->trace_topk.p : nullptr;
@@ -353,17 +383,96 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     return PANNS_OK;
 }
 
+// Points a plan's kernel arguments at the handle's current scratch (another
+// plan may have grown, and so moved, it since this one was built).
+int bind_scratch(panns_plan *pl)
+{
+    panns_index *h = pl->h;
+    int rc = ensure_scratch(h, pl->slot0 + pl->grid);
+    if (rc) return rc;
+    panns::KArgs &a = pl->args;
+    a.bitmaps = h->bitmaps.p + (size_t)pl->slot0 * h->bitmap_words;
+    a.spill_log = h->spill_log.p + (size_t)pl->slot0 * kLogCap;
+    a.work_counter = h->counter.p + pl->counter_idx;
+    return PANNS_OK;
+}
+
+hipError_t launch_one(const panns_plan *pl, hipStream_t st)
+{
+    LaunchFn fn;
+    KernelPtr kp;
+    kernel_for_dim(pl->h->dim, pl->args.stop_learned != 0, &fn, &kp);
+    return fn(pl->args, pl->grid, pl->lds, st);
+}
+
+// Enqueues a bucketed batch on the handle's stream (which may be capturing):
+// one counter reset for all buckets, then the bucket launches, in order or
+// as side-by-side branches joined back into the stream.
+int launch_parts(panns_plan *pl)
+{
+    panns_index *h = pl->h;
+    const uint32_t nb = (uint32_t)pl->parts.size();
+    hipLaunchKernelGGL(panns::reset_counter_kernel, dim3(1), dim3(64), 0, h->stream, h->counter.p, nb);
+    HIP_TRY(hipGetLastError());
+    if (!pl->opt.concurrent) {
+        for (auto &part : pl->parts) HIP_TRY(launch_one(part.get(), h->stream));
+        return PANNS_OK;
+    }
+    HIP_TRY(hipEventRecord(pl->fork_ev, h->stream));
+    for (uint32_t b = 0; b < nb; ++b) {
+        HIP_TRY(hipStreamWaitEvent(pl->side[b], pl->fork_ev, 0));
+        HIP_TRY(launch_one(pl->parts[b].get(), pl->side[b]));
+        HIP_TRY(hipEventRecord(pl->side_ev[b], pl->side[b]));
+    }
+    for (uint32_t b = 0; b < nb; ++b) HIP_TRY(hipStreamWaitEvent(h->stream, pl->side_ev[b], 0));
+    return PANNS_OK;
+}
+
 int run_plan(panns_plan *pl, int sync, float *ms)
 {
     panns_index *h = pl->h;
-    LaunchFn fn;
-    KernelPtr kp;
-    kernel_for_dim(h->dim, &fn, &kp);
-    hipLaunchKernelGGL(panns::reset_counter_kernel, dim3(1), dim3(64), 0, h->stream, h->counter.p);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    HIP_TRY(fn(pl->args, pl->grid, pl->lds, h->stream));
-    HIP_TRY(hipEventRecord(h->ev1, h->stream));
+    if (pl->parts.empty()) {
+        int rc = bind_scratch(pl);
+        if (rc) return rc;
+        hipLaunchKernelGGL(panns::reset_counter_kernel, dim3(1), dim3(64), 0, h->stream,
+                           h->counter.p + pl->counter_idx, 1u);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(h->ev0, h->stream));
+        HIP_TRY(launch_one(pl, h->stream));
+        HIP_TRY(hipEventRecord(h->ev1, h->stream));
+    } else {
+        for (auto &part : pl->parts) {
+            int rc = bind_scratch(part.get());
+            if (rc) return rc;
+        }
+        if (pl->opt.use_graph && (!pl->gexec || pl->graph_scratch != (const void *)h->bitmaps.p)) {
+            if (pl->gexec) HIP_TRY(hipGraphExecDestroy(pl->gexec));
+            if (pl->graph) HIP_TRY(hipGraphDestroy(pl->graph));
+            pl->gexec = nullptr;
+            pl->graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            int rc = launch_parts(pl);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(h->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            HIP_TRY(e);
+            pl->graph = g;
+            HIP_TRY(hipGraphInstantiate(&pl->gexec, pl->graph, nullptr, nullptr, 0));
+            pl->graph_scratch = h->bitmaps.p;
+        }
+        HIP_TRY(hipEventRecord(h->ev0, h->stream));
+        if (pl->opt.use_graph) {
+            HIP_TRY(hipGraphLaunch(pl->gexec, h->stream));
+        } else {
+            int rc = launch_parts(pl);
+            if (rc) return rc;
+        }
+        HIP_TRY(hipEventRecord(h->ev1, h->stream));
+        for (auto &part : pl->parts) part->ran = true;
+    }
     pl->ran = true;
     if (sync) {
         HIP_TRY(hipEventSynchronize(h->ev1));
@@ -372,7 +481,7 @@ int run_plan(panns_plan *pl, int sync, float *ms)
     return PANNS_OK;
 }
 
-int fetch_plan(panns_plan *pl, panns_outputs *out, panns_counters *cnt, float kernel_ms)
+int fetch_single(panns_plan *pl, panns_outputs *out, panns_counters *cnt, float kernel_ms)
 {
     panns_index *h = pl->h;
     if (!pl->ran) return fail(PANNS_ERR_STATE, "plan has not run");
@@ -417,6 +526,63 @@ int fetch_plan(panns_plan *pl, panns_outputs *out, panns_counters *cnt, float ke
         cnt->kernel_ms += kernel_ms;
     }
     return PANNS_OK;
+}
+
+// Results of a bucketed batch in the caller's order: each bucket's rows are
+// fetched and scattered; ids past a bucket's largest K read 0xFFFFFFFF and
+// distances +inf, as in a single plan.
+int fetch_parts(panns_plan *pl, panns_outputs *out, panns_counters *cnt, float kernel_ms)
+{
+    if (!pl->ran) return fail(PANNS_ERR_STATE, "plan has not run");
+    HIP_TRY(hipStreamSynchronize(pl->h->stream));
+    const uint32_t K = pl->maxK, L = pl->maxL;
+    const float inf = std::numeric_limits<float>::infinity();
+    for (size_t b = 0; b < pl->parts.size(); ++b) {
+        panns_plan *pt = pl->parts[b].get();
+        const std::vector<uint32_t> &rows = pl->part_rows[b];
+        const uint32_t n = pt->nq, kb = pt->maxK, lb = pt->maxL;
+        std::vector<uint32_t> ids, qids, nd, ne, ni;
+        std::vector<float> d, qd;
+        std::vector<uint8_t> qc;
+        panns_outputs o{};
+        const bool q = out && pl->opt.want_queue;
+        if (out) {
+            if (out->ids) { ids.resize((size_t)n * kb); o.ids = ids.data(); }
+            if (out->dists) { d.resize((size_t)n * kb); o.dists = d.data(); }
+            if (q && out->queue_ids) { qids.resize((size_t)n * lb); o.queue_ids = qids.data(); }
+            if (q && out->queue_dists) { qd.resize((size_t)n * lb); o.queue_dists = qd.data(); }
+            if (q && out->queue_checked) { qc.resize((size_t)n * lb); o.queue_checked = qc.data(); }
+            if (out->ndist) { nd.resize(n); o.ndist = nd.data(); }
+            if (out->nexp) { ne.resize(n); o.nexp = ne.data(); }
+            if (out->niter) { ni.resize(n); o.niter = ni.data(); }
+        }
+        int rc = fetch_single(pt, out ? &o : nullptr, cnt, 0.f);
+        if (rc) return rc;
+        if (!out) continue;
+        for (uint32_t i = 0; i < n; ++i) {
+            const size_t r = rows[i];
+            for (uint32_t j = 0; j < K; ++j) {
+                if (out->ids) out->ids[r * K + j] = j < kb ? ids[(size_t)i * kb + j] : 0xFFFFFFFFu;
+                if (out->dists) out->dists[r * K + j] = j < kb ? d[(size_t)i * kb + j] : inf;
+            }
+            if (q)
+                for (uint32_t j = 0; j < L; ++j) {
+                    if (out->queue_ids) out->queue_ids[r * L + j] = j < lb ? qids[(size_t)i * lb + j] : 0xFFFFFFFFu;
+                    if (out->queue_dists) out->queue_dists[r * L + j] = j < lb ? qd[(size_t)i * lb + j] : inf;
+                    if (out->queue_checked) out->queue_checked[r * L + j] = j < lb ? qc[(size_t)i * lb + j] : 0;
+                }
+            if (out->ndist) out->ndist[r] = nd[i];
+            if (out->nexp) out->nexp[r] = ne[i];
+            if (out->niter) out->niter[r] = ni[i];
+        }
+    }
+    if (cnt) cnt->kernel_ms += kernel_ms;
+    return PANNS_OK;
+}
+
+int fetch_plan(panns_plan *pl, panns_outputs *out, panns_counters *cnt, float kernel_ms)
+{
+    return pl->parts.empty() ? fetch_single(pl, out, cnt, kernel_ms) : fetch_parts(pl, out, cnt, kernel_ms);
 }
 
 int upload_graph(panns_index *h, uint32_t width, uint32_t ep, uint32_t n, uint32_t stride_in,
@@ -626,6 +792,206 @@ int panns_plan_create(panns_index *h, const panns_params *p, const uint32_t *que
     return PANNS_OK;
 }
 
+int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *query_ids, uint32_t nq,
+                         const uint32_t *L_per_query, const uint32_t *K_per_query, const uint32_t *init_ids,
+                         uint32_t n_init, const panns_plan_options *opt, panns_plan **out)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!p || !query_ids || !init_ids || !out || nq == 0) return fail(PANNS_ERR_ARG, "bad plan arguments");
+    if (!h->nq) return fail(PANNS_ERR_STATE, "queries not loaded");
+    panns_plan_options o{};
+    if (opt) o = *opt;
+    if (o.n_buckets == 0) o.n_buckets = 1;
+    if (o.n_buckets > PANNS_MAX_BUCKETS) return fail(PANNS_ERR_ARG, "at most 8 buckets");
+    if (o.table_shift > 2) return fail(PANNS_ERR_ARG, "table_shift must be 0, 1 or 2");
+    HIP_TRY(hipSetDevice(h->device));
+    std::unique_ptr<panns_plan> pl(new panns_plan());
+    pl->h = h;
+    pl->p = *p;
+    pl->nq = nq;
+    pl->opt = o;
+    pl->want_queue = o.want_queue != 0;
+    pl->table_shift = o.table_shift;
+    if (o.n_buckets == 1 && !o.use_graph && !o.concurrent) {
+        rc = build_plan(pl.get(), query_ids, L_per_query, K_per_query, init_ids, n_init);
+        if (rc) return rc;
+        *out = pl.release();
+        return PANNS_OK;
+    }
+    // L and K of every query, checked as a single plan would
+    std::vector<uint32_t> Lall(nq), Kall(nq);
+    pl->maxL = 0;
+    pl->maxK = 0;
+    for (uint32_t i = 0; i < nq; ++i) {
+        Lall[i] = L_per_query ? L_per_query[i] : p->L;
+        Kall[i] = K_per_query ? K_per_query[i] : p->K;
+        rc = check_params(h, p, Lall[i], Kall[i]);
+        if (rc) return rc;
+        pl->maxL = std::max(pl->maxL, Lall[i]);
+        pl->maxK = std::max(pl->maxK, Kall[i]);
+    }
+    // bucket bounds: given, or equal summed L over the queries sorted by L
+    std::vector<uint32_t> bounds;
+    bool given = false;
+    for (uint32_t b = 0; b < o.n_buckets; ++b) given = given || o.bucket_max_L[b] != 0;
+    if (given) {
+        for (uint32_t b = 0; b < o.n_buckets; ++b) {
+            if (b > 0 && o.bucket_max_L[b] <= o.bucket_max_L[b - 1])
+                return fail(PANNS_ERR_ARG, "bucket_max_L must be ascending");
+            bounds.push_back(o.bucket_max_L[b]);
+        }
+        if (bounds.back() < pl->maxL) return fail(PANNS_ERR_ARG, "the last bucket bound is below the largest L");
+    } else {
+        std::vector<uint32_t> sorted(Lall);
+        std::sort(sorted.begin(), sorted.end());
+        double total = 0;
+        for (uint32_t v : sorted) total += v;
+        double acc = 0;
+        uint32_t next = 1;
+        for (uint32_t v : sorted) {
+            acc += v;
+            while (next < o.n_buckets && acc >= total * next / o.n_buckets) {
+                if (bounds.empty() || bounds.back() < v) bounds.push_back(v);
+                ++next;
+            }
+        }
+        if (bounds.empty() || bounds.back() < pl->maxL) bounds.push_back(pl->maxL);
+    }
+    // queries per bucket, longest first; buckets with the largest L first
+    std::vector<std::vector<uint32_t>> members(bounds.size());
+    std::vector<uint32_t> order(nq);
+    for (uint32_t i = 0; i < nq; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return Lall[x] > Lall[y]; });
+    for (uint32_t i : order) {
+        const size_t b = std::lower_bound(bounds.begin(), bounds.end(), Lall[i]) - bounds.begin();
+        members[b].push_back(i);
+    }
+    uint32_t slot = 0, counter = 0;
+    for (size_t bi = bounds.size(); bi-- > 0;) {
+        const std::vector<uint32_t> &m = members[bi];
+        if (m.empty()) continue;
+        std::unique_ptr<panns_plan> part(new panns_plan());
+        part->h = h;
+        part->p = *p;
+        part->nq = (uint32_t)m.size();
+        part->want_queue = pl->want_queue;
+        part->table_shift = o.table_shift;
+        part->slot0 = o.concurrent ? slot : 0u;
+        part->counter_idx = counter++;
+        std::vector<uint32_t> q(m.size()), lq(m.size()), kq(m.size());
+        for (size_t i = 0; i < m.size(); ++i) {
+            q[i] = query_ids[m[i]];
+            lq[i] = Lall[m[i]];
+            kq[i] = Kall[m[i]];
+        }
+        rc = build_plan(part.get(), q.data(), lq.data(), kq.data(), init_ids, n_init);
+        if (rc) return rc;
+        slot += part->grid;
+        pl->parts.push_back(std::move(part));
+        pl->part_rows.push_back(m);
+    }
+    if (o.concurrent) {
+        HIP_TRY(hipEventCreateWithFlags(&pl->fork_ev, hipEventDisableTiming));
+        for (size_t b = 0; b < pl->parts.size(); ++b) {
+            hipStream_t st = nullptr;
+            HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            pl->side.push_back(st);
+            hipEvent_t e = nullptr;
+            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            pl->side_ev.push_back(e);
+        }
+    }
+    pl->grid = 0;
+    for (auto &part : pl->parts) {
+        pl->grid += part->grid;
+        pl->lds = std::max(pl->lds, part->lds);
+        pl->hash_slots = std::max(pl->hash_slots, part->hash_slots);
+    }
+    *out = pl.release();
+    return PANNS_OK;
+}
+
+int panns_plan_describe_buckets(panns_plan *plan, uint32_t *n_buckets, uint32_t *max_L, uint32_t *nq,
+                                uint32_t *workgroups, uint32_t *lds_bytes, uint32_t *hash_slots)
+{
+    if (!plan) return fail(PANNS_ERR_ARG, "null plan");
+    std::vector<const panns_plan *> ps;
+    if (plan->parts.empty()) ps.push_back(plan);
+    for (auto &part : plan->parts) ps.push_back(part.get());
+    if (n_buckets) *n_buckets = (uint32_t)ps.size();
+    for (size_t b = 0; b < ps.size(); ++b) {
+        if (max_L) max_L[b] = ps[b]->maxL;
+        if (nq) nq[b] = ps[b]->nq;
+        if (workgroups) workgroups[b] = ps[b]->grid;
+        if (lds_bytes) lds_bytes[b] = ps[b]->lds;
+        if (hash_slots) hash_slots[b] = ps[b]->hash_slots;
+    }
+    return PANNS_OK;
+}
+
+int panns_plan_autotune(panns_index *h, const panns_params *p, const uint32_t *query_ids, uint32_t nq,
+                        const uint32_t *L_per_query, const uint32_t *K_per_query, const uint32_t *init_ids,
+                        uint32_t n_init, uint32_t sample_stride, panns_plan_options *best, float *best_ms)
+{
+    if (!p || !query_ids || !best || nq == 0) return fail(PANNS_ERR_ARG, "bad autotune arguments");
+    const uint32_t stride = sample_stride ? sample_stride : std::max<uint32_t>(1u, nq / 20000u);
+    std::vector<uint32_t> sq, sl, sk, Lall(nq);
+    for (uint32_t i = 0; i < nq; ++i) Lall[i] = L_per_query ? L_per_query[i] : p->L;
+    for (uint32_t i = 0; i < nq; i += stride) {
+        sq.push_back(query_ids[i]);
+        sl.push_back(Lall[i]);
+        if (K_per_query) sk.push_back(K_per_query[i]);
+    }
+    std::vector<uint32_t> sorted(Lall);
+    std::sort(sorted.begin(), sorted.end());
+    double total = 0;
+    for (uint32_t v : sorted) total += v;
+    float best_t = std::numeric_limits<float>::infinity();
+    static const uint32_t kCounts[] = {1, 2, 3, 4, 6};
+    for (uint32_t nb : kCounts) {
+        // equal summed L over the whole batch, as explicit bounds
+        panns_plan_options o{};
+        uint32_t k = 0;
+        double acc = 0;
+        uint32_t next = 1;
+        for (uint32_t v : sorted) {
+            acc += v;
+            while (next < nb && acc >= total * next / nb) {
+                if (k == 0 || o.bucket_max_L[k - 1] < v) o.bucket_max_L[k++] = v;
+                ++next;
+            }
+        }
+        if (k == 0 || o.bucket_max_L[k - 1] < sorted.back()) o.bucket_max_L[k++] = sorted.back();
+        if (nb > 1 && k < 2) continue;
+        o.n_buckets = k;
+        o.use_graph = 1;
+        for (uint32_t conc = 0; conc < (k > 1 ? 2u : 1u); ++conc)
+            for (uint32_t shift = 0; shift < 2; ++shift) {
+                o.concurrent = conc;
+                o.table_shift = shift;
+                panns_plan *pl = nullptr;
+                int rc = panns_plan_create_ex(h, p, sq.data(), (uint32_t)sq.size(), L_per_query ? sl.data() : nullptr,
+                                              K_per_query ? sk.data() : nullptr, init_ids, n_init, &o, &pl);
+                if (rc) return rc;
+                float t = std::numeric_limits<float>::infinity(), ms = 0.f;
+                rc = run_plan(pl, 1, nullptr);
+                for (int r = 0; r < 3 && !rc; ++r) {
+                    rc = run_plan(pl, 1, &ms);
+                    t = std::min(t, ms);
+                }
+                panns_plan_destroy(pl);
+                if (rc) return rc;
+                if (t < best_t) {
+                    best_t = t;
+                    *best = o;
+                }
+            }
+    }
+    if (best_ms) *best_ms = best_t;
+    return PANNS_OK;
+}
+
 int panns_plan_run(panns_plan *plan, int sync, float *kernel_ms)
 {
     if (!plan) return fail(PANNS_ERR_ARG, "null plan");
@@ -796,25 +1162,8 @@ int panns_set_stop_model(panns_index *h, const float *params, uint32_t n_floats)
     for (uint32_t i = 0; i < panns::kFeat; ++i)
         if (!(params[4812 + i] > 0.f)) return fail(PANNS_ERR_ARG, "feature scales must be positive");
     HIP_TRY(hipSetDevice(h->device));
-    // Fold the feature normalisation into the first layer:
-    // W1 ((f - mu) / sd) + b1 = (W1 / sd) f + (b1 - (W1 / sd) mu).
-    std::vector<float> folded(params, params + n_floats);
-    const float *mu = params + 4804, *sd = params + 4812;
-    for (int j = 0; j < panns::kHid; ++j) {
-        double b = params[512 + j];
-        for (int i = 0; i < panns::kFeat; ++i) {
-            const float w = params[j * panns::kFeat + i] / sd[i];
-            folded[j * panns::kFeat + i] = w;
-            b -= (double)w * (double)mu[i];
-        }
-        folded[512 + j] = (float)b;
-    }
-    for (int i = 0; i < panns::kFeat; ++i) {
-        folded[4804 + i] = 0.f;
-        folded[4812 + i] = 1.f;
-    }
     HIP_TRY(h->model.ensure(n_floats));
-    HIP_TRY(hipMemcpy(h->model.p, folded.data(), n_floats * 4ull, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->model.p, params, n_floats * 4ull, hipMemcpyHostToDevice));
     h->has_model = true;
     return PANNS_OK;
 }
@@ -959,6 +1308,8 @@ int panns_mfma_probe(panns_index *h, const float *A, const float *B, float *C)
 
 int panns_plan_phase_cycles(panns_plan *plan, uint64_t *out8)
 {
+    if (plan && !plan->parts.empty())
+        return fail(PANNS_ERR_UNSUPPORTED, "phase cycles are kept for single-bucket plans only");
     if (!plan || !out8) return fail(PANNS_ERR_ARG, "null argument");
 #ifdef PANNS_PHASE_TIMERS
     HIP_TRY(hipSetDevice(plan->h->device));

@@ -58,6 +58,8 @@ struct StopModel {
     const float *b2;  // [64]
     const float *w3;  // [64]
     const float *b3;  // [1]
+    const float *mu;  // [8] feature means
+    const float *sd;  // [8] feature scales
 };
 
 struct KArgs {
@@ -173,54 +175,48 @@ __device__ __forceinline__ constexpr int blk_of(int i)
     return (nb & 1) ? (i == 0 ? nb - 1 : i - 1) : i;
 }
 
-// Squared L2 of two base rows (A and B) against the query, each computed by
-// a lane pair: the even lane owns accumulators 0-3, the odd lane 4-7 of every
+// Squared L2 of NV base rows against the query, each computed by a lane
+// pair: the even lane owns accumulators 0-3, the odd lane 4-7 of every
 // 8-wide block; each accumulator is an fma chain in the reference's block
-// order and the horizontal sum is ((((((a0+a1)+a2)+a3)+a4)+a5)+a6)+a7.  Both
-// rows' loads are issued before either is reduced.  The exact values are
-// returned in the odd lane.  qs is the query row in LDS.
-template <int D>
-__device__ __forceinline__ void pair_dist2(const float *__restrict__ base, uint32_t idA, bool vA,
-                                           uint32_t idB, bool vB, int half, const float4 *qs,
-                                           float &dA, float &dB)
+// order and the horizontal sum is ((((((a0+a1)+a2)+a3)+a4)+a5)+a6)+a7.  All
+// rows' loads are issued before any is reduced.  The exact values are
+// returned in the odd lane.  qs is the query row in LDS.  Loads are
+// unconditional: callers pass a valid row (id 0) for inactive lanes and
+// discard those results, because exec-masked loads make the compiler wait
+// for each one where the branches join.
+template <int D, int NV>
+__device__ __forceinline__ void pair_dist(const float *__restrict__ base, const uint32_t (&id)[NV], int half,
+                                          const float4 *qs, float (&d)[NV])
 {
     constexpr int NB = D / 8;
-    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 *ra = reinterpret_cast<const float4 *>(base + (size_t)idA * D);
-    const float4 *rb = reinterpret_cast<const float4 *>(base + (size_t)idB * D);
-    // Loads are unconditional: callers pass a valid row (id 0) for inactive
-    // lanes and discard those results, because exec-masked loads make the
-    // compiler wait for each one where the branches join.
-    (void)vA;
-    (void)vB;
-    (void)z;
-    float4 va[NB], vb[NB];
+    float4 v[NV][NB];
 #pragma unroll
-    for (int i = 0; i < NB; ++i) va[i] = ra[2 * blk_of<D>(i) + half];
+    for (int j = 0; j < NV; ++j) {
+        const float4 *r = reinterpret_cast<const float4 *>(base + (size_t)id[j] * D);
 #pragma unroll
-    for (int i = 0; i < NB; ++i) vb[i] = rb[2 * blk_of<D>(i) + half];
-    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-    float b0 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f;
+        for (int i = 0; i < NB; ++i) v[j][i] = r[2 * blk_of<D>(i) + half];
+    }
+    float acc[NV][4];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 0.f;
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
         const float4 q = qs[2 * blk_of<D>(i) + half];
-        const float tx = va[i].x - q.x, ty = va[i].y - q.y, tz = va[i].z - q.z, tw = va[i].w - q.w;
-        a0 = __builtin_fmaf(tx, tx, a0);
-        a1 = __builtin_fmaf(ty, ty, a1);
-        a2 = __builtin_fmaf(tz, tz, a2);
-        a3 = __builtin_fmaf(tw, tw, a3);
-        const float ux = vb[i].x - q.x, uy = vb[i].y - q.y, uz = vb[i].z - q.z, uw = vb[i].w - q.w;
-        b0 = __builtin_fmaf(ux, ux, b0);
-        b1 = __builtin_fmaf(uy, uy, b1);
-        b2 = __builtin_fmaf(uz, uz, b2);
-        b3 = __builtin_fmaf(uw, uw, b3);
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            const float tx = v[j][i].x - q.x, ty = v[j][i].y - q.y, tz = v[j][i].z - q.z, tw = v[j][i].w - q.w;
+            acc[j][0] = __builtin_fmaf(tx, tx, acc[j][0]);
+            acc[j][1] = __builtin_fmaf(ty, ty, acc[j][1]);
+            acc[j][2] = __builtin_fmaf(tz, tz, acc[j][2]);
+            acc[j][3] = __builtin_fmaf(tw, tw, acc[j][3]);
+        }
     }
-    const float sa = ((a0 + a1) + a2) + a3;
-    const float sb = ((b0 + b1) + b2) + b3;
-    const float oa = __shfl_xor(sa, 1);
-    const float ob = __shfl_xor(sb, 1);
-    dA = (((oa + a0) + a1) + a2) + a3;
-    dB = (((ob + b0) + b1) + b2) + b3;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const float sj = ((acc[j][0] + acc[j][1]) + acc[j][2]) + acc[j][3];
+        const float o = __shfl_xor(sj, 1);
+        d[j] = (((o + acc[j][0]) + acc[j][1]) + acc[j][2]) + acc[j][3];
+    }
 }
 
 __device__ __forceinline__ uint32_t lb_queue(const uint64_t *Q, uint32_t S, uint64_t x)
@@ -301,10 +297,11 @@ __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32
 // (multiplication by an odd constant, then an xor-shift); the low lbits of the
 // result pick the bucket and the remaining idbits - lbits <= 15 bits are the
 // tag stored in it (0xFFFF marks an empty slot), so bucket and tag together
-// identify the id exactly.  Slots are only ever filled during a query, so
-// once a bucket is full every later id of that bucket whose tag is absent is
-// decided by the bitmap, and an id is never both in the table and in the
-// bitmap.  spills is a wave-uniform count kept in registers.
+// identify the id exactly.  An id whose bucket is full tries one second
+// bucket (probe()).  Slots are only ever filled during a query, so once both
+// buckets of an id are full, an id absent from them is decided by the
+// bitmap, and an id is never both in the table and in the bitmap.  spills is
+// a wave-uniform count kept in registers.
 struct Visited {
     uint32_t *tbl;      // LDS: buckets x 4 words
     uint32_t bmask, lbits, idbits, idmask, xshift, use;
@@ -319,15 +316,12 @@ struct Visited {
         return h ^ (h >> xshift);
     }
 
-    // 0: visited before; 1: new and now in the table; 2: not in the table,
-    // whose bucket is full -- the bitmap decides.
-    __device__ __forceinline__ int probe(uint32_t id) const
+    // One bucket: 0 if `tag` is there, 1 if it was inserted, 2 if the
+    // bucket is full without it.  Every failed CAS means another slot of the
+    // bucket was filled, so at most 8 retries precede a full bucket or a
+    // success.
+    __device__ __forceinline__ int probe_bucket(uint32_t *w, uint32_t tag) const
     {
-        const uint32_t h = scramble(id);
-        const uint32_t tag = h >> lbits;
-        uint32_t *w = tbl + ((h & bmask) << 2);
-        // every failed CAS means another slot of the bucket was filled, so
-        // at most 8 retries precede a full bucket or a success
 #pragma unroll 1
         for (int attempt = 0; attempt < 9; ++attempt) {
             const uint4 v = *reinterpret_cast<const uint4 *>(w);
@@ -349,6 +343,26 @@ struct Visited {
             if (atomicCAS(&w[ew], eold, nw) == eold) return 1;
         }
         return 2;
+    }
+
+    // 0: visited before; 1: new and now in the table; 2: in neither of its
+    // two buckets, both full -- the bitmap decides.
+    // The id's primary bucket is b1 = h & bmask with tag t = h >> lbits
+    // (t < 2^15).  Only when b1 is full does the id go to its second bucket
+    // b2 = b1 ^ alt(t), as t | 0x8000: slots never empty during a query, so
+    // an id can be in b2 only if b1 is full, and (bucket, stored tag) still
+    // determines the id (b1 = b2 ^ alt(t), then h, then the id, since the
+    // scramble is a bijection).  t = 0x7FFF has no second bucket (its
+    // secondary tag would read as an empty slot).
+    __device__ __forceinline__ int probe(uint32_t id) const
+    {
+        const uint32_t h = scramble(id);
+        const uint32_t tag = h >> lbits;
+        const uint32_t b1 = h & bmask;
+        const int r = probe_bucket(tbl + (b1 << 2), tag);
+        if (r != 2 || tag == 0x7FFFu) return r;
+        const uint32_t b2 = (b1 ^ (((tag * 0x85EBCA77u) >> 15) | 1u)) & bmask;
+        return probe_bucket(tbl + (b2 << 2), tag | 0x8000u);
     }
 };
 
@@ -414,12 +428,22 @@ __device__ __forceinline__ uint32_t claim_chunks(Visited &vis, uint32_t (&ids)[k
 }
 
 // The per-lane part of the stop model, kept in registers for the kernel's
-// life: lane j's row of W1 (feature normalisation folded in by the host), its
-// biases and its output weight.
+// life: lane j's row of W1, its biases, its output weight, and its row of
+// the 64x64 layer as 32 pairs of bf16 (round to nearest even; element 2i in
+// the low half of w2[i]); the feature means and scales are the same for all
+// lanes.
 struct LaneModel {
     float w1[kFeat];
     float b1, b2, w3, b3;
+    float mu[kFeat], sd[kFeat];
+    uint32_t w2[kHid / 2];
 };
+
+__device__ __forceinline__ uint32_t bf16_bits(float x)
+{
+    const uint32_t u = __float_as_uint(x);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
 
 __device__ __forceinline__ LaneModel load_lane_model(const StopModel &m, int lane)
 {
@@ -430,22 +454,31 @@ __device__ __forceinline__ LaneModel load_lane_model(const StopModel &m, int lan
     r.b2 = m.b2[lane];
     r.w3 = m.w3[lane];
     r.b3 = m.b3[0];
+#pragma unroll
+    for (int i = 0; i < kFeat; ++i) {
+        r.mu[i] = m.mu[i];
+        r.sd[i] = m.sd[i];
+    }
+    const float2 *row = reinterpret_cast<const float2 *>(m.w2 + lane * kHid);
+#pragma unroll
+    for (int i = 0; i < kHid / 2; ++i) {
+        const float2 v = row[i];
+        r.w2[i] = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
+    }
     return r;
 }
 
-// One-wave fp32 MLP: lane j computes hidden unit j of both layers; only the
-// 64x64 layer's row is read from memory (L1-resident, shared by all waves).
-// Returns the sigmoid output in every lane.
-__device__ __forceinline__ float mlp_eval(const LaneModel &lm, const float *__restrict__ w2,
-                                          const float (&f)[kFeat], float *h1s, int lane)
+// One-wave MLP: lane j computes hidden unit j of both layers, in fp32 with
+// the 64x64 layer's weights in bf16 (as the training kernels use them);
+// features enter as (f - mu) / sd, the sums run in index order.  Returns the
+// sigmoid output in every lane.  (Folding the scaling into the first layer
+// would cancel catastrophically for a feature of near-zero spread, such as
+// the norm of normalised queries.)
+__device__ __forceinline__ float mlp_eval(const LaneModel &lm, const float (&f)[kFeat], float *h1s, int lane)
 {
-    const float4 *w2r = reinterpret_cast<const float4 *>(w2 + lane * kHid);
-    float4 wv[kHid / 4];
-#pragma unroll
-    for (int i = 0; i < kHid / 4; ++i) wv[i] = w2r[i];
     float h = lm.b1;
 #pragma unroll
-    for (int i = 0; i < kFeat; ++i) h = __builtin_fmaf(lm.w1[i], f[i], h);
+    for (int i = 0; i < kFeat; ++i) h = __builtin_fmaf(lm.w1[i], (f[i] - lm.mu[i]) / lm.sd[i], h);
     h = h > 0.f ? h : 0.f;
     h1s[lane] = h;
     __syncthreads();
@@ -454,10 +487,11 @@ __device__ __forceinline__ float mlp_eval(const LaneModel &lm, const float *__re
 #pragma unroll
     for (int i = 0; i < kHid / 4; ++i) {
         const float4 x = h1v[i];
-        g = __builtin_fmaf(wv[i].x, x.x, g);
-        g = __builtin_fmaf(wv[i].y, x.y, g);
-        g = __builtin_fmaf(wv[i].z, x.z, g);
-        g = __builtin_fmaf(wv[i].w, x.w, g);
+        const uint32_t p0 = lm.w2[2 * i], p1 = lm.w2[2 * i + 1];
+        g = __builtin_fmaf(__uint_as_float(p0 << 16), x.x, g);
+        g = __builtin_fmaf(__uint_as_float(p0 & 0xFFFF0000u), x.y, g);
+        g = __builtin_fmaf(__uint_as_float(p1 << 16), x.z, g);
+        g = __builtin_fmaf(__uint_as_float(p1 & 0xFFFF0000u), x.w, g);
     }
     g = g > 0.f ? g : 0.f;
     float o = lm.w3 * g;
@@ -468,36 +502,62 @@ __device__ __forceinline__ float mlp_eval(const LaneModel &lm, const float *__re
     return 1.f / (1.f + __expf(-o));
 }
 
-// Distances for ids[0, cnt) (in LDS), 64 per pass; keys below `bound` are
+// Distances for ids[cnt0, cnt0 + 32 * NV) of ids[0, cnt) (in LDS), one
+// lane pair per id with NV rows per pair in flight; keys below `bound` are
 // appended to cand[ncand...].  Returns the new ncand.
-template <int D>
-__device__ __forceinline__ uint32_t distance_pass(const float *__restrict__ base, const uint32_t *ids,
-                                                  uint32_t cnt, const float4 *qs, uint64_t bound,
-                                                  uint64_t *cand, uint32_t ncand, int lane, uint64_t lt)
+template <int D, int NV>
+__device__ __forceinline__ uint32_t distance_group(const float *__restrict__ base, const uint32_t *ids,
+                                                   uint32_t cnt0, uint32_t cnt, const float4 *qs, uint64_t bound,
+                                                   uint64_t *cand, uint32_t ncand, int lane, uint64_t lt)
 {
     const int half = lane & 1;
-    for (uint32_t p = 0; p < cnt; p += 64) {
-        const uint32_t ia = p + (uint32_t)(lane >> 1), ib = ia + 32;
-        const bool va = ia < cnt, vb = ib < cnt;
-        const uint32_t ra = ids[va ? ia : 0u], rb = ids[vb ? ib : 0u];
-        const uint32_t ida = va ? ra : 0u, idb = vb ? rb : 0u;
-        float da, db;
-        pair_dist2<D>(base, ida, va, idb, vb, half, qs, da, db);
-        const uint64_t ka = mk_key(da, ida), kb = mk_key(db, idb);
-        const bool sa = va && half && ka < bound;
-        const bool sb = vb && half && kb < bound;
-        const uint64_t ma = __ballot(sa);
-        if (sa) cand[ncand + (uint32_t)__popcll(ma & lt)] = ka;
-        ncand += (uint32_t)__popcll(ma);
-        const uint64_t mb = __ballot(sb);
-        if (sb) cand[ncand + (uint32_t)__popcll(mb & lt)] = kb;
-        ncand += (uint32_t)__popcll(mb);
+    uint32_t id[NV];
+    bool ok[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const uint32_t i = cnt0 + (uint32_t)(lane >> 1) + 32u * (uint32_t)j;
+        ok[j] = i < cnt;
+        const uint32_t r = ids[ok[j] ? i : 0u];
+        id[j] = ok[j] ? r : 0u;
+    }
+    float d[NV];
+    pair_dist<D, NV>(base, id, half, qs, d);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const uint64_t kj = mk_key(d[j], id[j]);
+        const bool sj = ok[j] && half && kj < bound;
+        const uint64_t mj = __ballot(sj);
+        if (sj) cand[ncand + (uint32_t)__popcll(mj & lt)] = kj;
+        ncand += (uint32_t)__popcll(mj);
     }
     return ncand;
 }
 
-template <int D>
-__global__ __launch_bounds__(64, 3) void search_kernel(KArgs a)
+// Distances for ids[0, cnt): groups of 96 ids (three rows per lane pair)
+// while more than 64 remain, then groups of 64.  kWide = false keeps to
+// groups of 64 (fewer registers; the learned-stop kernel holds its model).
+template <int D, bool kWide>
+__device__ __forceinline__ uint32_t distance_pass(const float *__restrict__ base, const uint32_t *ids,
+                                                  uint32_t cnt, const float4 *qs, uint64_t bound,
+                                                  uint64_t *cand, uint32_t ncand, int lane, uint64_t lt)
+{
+    uint32_t p = 0;
+    while (p < cnt) {
+        if (kWide && cnt - p > 64u) {
+            ncand = distance_group<D, 3>(base, ids, p, cnt, qs, bound, cand, ncand, lane, lt);
+            p += 96u;
+        } else {
+            ncand = distance_group<D, 2>(base, ids, p, cnt, qs, bound, cand, ncand, lane, lt);
+            p += 64u;
+        }
+    }
+    return ncand;
+}
+
+// kLearned: the variant that evaluates the stop model (a.stop_learned);
+// the other ignores it.
+template <int D, bool kLearned>
+__global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
 {
     extern __shared__ uint64_t smem[];
     const int lane = threadIdx.x;
@@ -526,7 +586,7 @@ __global__ __launch_bounds__(64, 3) void search_kernel(KArgs a)
     vis.log = a.shared_bitmap ? nullptr : a.spill_log + (size_t)blockIdx.x * a.log_cap;
     vis.log_cap = a.log_cap;
     LaneModel lm{};
-    if (a.stop_learned) lm = load_lane_model(a.model, lane);
+    if constexpr (kLearned) lm = load_lane_model(a.model, lane);
     PANNS_PT_DECL
 
     for (;;) {
@@ -566,7 +626,7 @@ __global__ __launch_bounds__(64, 3) void search_kernel(KArgs a)
             }
             for (uint32_t c0 = 0; c0 < L; c0 += 64) {
                 const uint32_t cnt = min(64u, L - c0);
-                const uint32_t n0 = distance_pass<D>(a.base, a.init_ids + c0, cnt, qs, kKeyMax, cand, 0,
+                const uint32_t n0 = distance_pass<D, !kLearned>(a.base, a.init_ids + c0, cnt, qs, kKeyMax, cand, 0,
                                                      lane, lt);
                 (void)n0;
                 __syncthreads();
@@ -582,6 +642,16 @@ __global__ __launch_bounds__(64, 3) void search_kernel(KArgs a)
             __syncthreads();
         }
         ep_dist = __uint_as_float((uint32_t)(Q[0] >> 32));
+        float qnorm = 0.f;
+        if (a.stop_learned || a.trace_feat) {
+            if (lane < D / 4) {
+                const float4 q = qs[lane];
+                qnorm = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
+            }
+#pragma unroll
+            for (int s = 32; s > 0; s >>= 1) qnorm += __shfl_xor(qnorm, s);
+            qnorm = sqrtf(qnorm);
+        }
         PANNS_PT(kPhInit);
 
         uint32_t k = 0;
@@ -639,14 +709,18 @@ __global__ __launch_bounds__(64, 3) void search_kernel(KArgs a)
                 __syncthreads();
                 PANNS_PT(kPhClaim);
                 ndist += nnew;
-                const uint32_t ncand = distance_pass<D>(a.base, newids, nnew, qs, km(Q[S - 1]), cand, 0, lane, lt);
+                const uint32_t ncand = distance_pass<D, !kLearned>(a.base, newids, nnew, qs, km(Q[S - 1]), cand, 0, lane, lt);
+                // stop feature: new keys that beat the queue's last key (a set
+                // fixed by the group, unlike the keys a batch-wise merge
+                // inserts and a later batch evicts)
+                ins += ncand;
                 __syncthreads();
                 PANNS_PT(kPhDist);
                 for (uint32_t b0 = 0; b0 < ncand; b0 += 64) {
                     const uint32_t cnt = min(64u, ncand - b0);
                     uint64_t x = (uint32_t)lane < cnt ? cand[b0 + lane] : kKeyMax;
                     x = sort64(x, lane);
-                    const uint32_t p0 = merge_batch(Q, S, L, x, cnt, bb, lane, &ins);
+                    const uint32_t p0 = merge_batch(Q, S, L, x, cnt, bb, lane);
                     if (p0 != kSent) nk = min(nk, p0);
                 }
                 PANNS_PT(kPhMerge);
@@ -656,19 +730,12 @@ __global__ __launch_bounds__(64, 3) void search_kernel(KArgs a)
             if (a.mode == kModeScaleM && Mcur < a.M) Mcur <<= 1;
 
             if (a.stop_learned || a.trace_feat) {
-                // features: step, top-1 dist, K-th dist, ratio, insertions in
-                // the last step, first-init (entry) dist, query norm, unchecked count
+                // features: step, top-1 dist, K-th dist, ratio, new keys below
+                // the queue's last key in the last step, first-init (entry) dist, query norm, unchecked count
                 uint32_t unc = 0;
                 for (uint32_t j = lane; j < S; j += 64) unc += (Q[j] & kChk) ? 0u : 1u;
 #pragma unroll
                 for (int s = 32; s > 0; s >>= 1) unc += __shfl_xor(unc, s);
-                float qn = 0.f;
-                if (lane < D / 4) {
-                    const float4 q = qs[lane];
-                    qn = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
-                }
-#pragma unroll
-                for (int s = 32; s > 0; s >>= 1) qn += __shfl_xor(qn, s);
                 float f[kFeat];
                 const float d1 = __uint_as_float((uint32_t)(Q[0] >> 32));
                 const float dk = __uint_as_float((uint32_t)(Q[min(K, S) - 1] >> 32));
@@ -678,7 +745,7 @@ __global__ __launch_bounds__(64, 3) void search_kernel(KArgs a)
                 f[3] = dk / fmaxf(d1, 1e-12f);
                 f[4] = (float)ins_last;
                 f[5] = ep_dist;
-                f[6] = sqrtf(qn);
+                f[6] = qnorm;
                 f[7] = (float)unc;
                 if (a.trace_feat && niter <= a.trace_cap) {
                     float *tf = a.trace_feat + ((size_t)w * a.trace_cap + (niter - 1)) * kFeat;
@@ -693,8 +760,8 @@ __global__ __launch_bounds__(64, 3) void search_kernel(KArgs a)
                     for (int s = 32; s > 0; s >>= 1) hsh += __shfl_xor(hsh, s);
                     if (lane == 0) a.trace_topk[(size_t)w * a.trace_cap + (niter - 1)] = hsh;
                 }
-                if (a.stop_learned) {
-                    const float pstop = mlp_eval(lm, a.model.w2, f, h1s, lane);
+                if constexpr (kLearned) {
+                    const float pstop = mlp_eval(lm, f, h1s, lane);
                     if (pstop >= a.stop_threshold) break;
                 }
                 PANNS_PT(kPhStop);
@@ -755,9 +822,11 @@ __global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ d
         dst[i] = src[i];
 }
 
-__global__ void reset_counter_kernel(uint32_t *c)
+// Zeroes the n work counters of a launch sequence (a kernel, not a memset,
+// so the reset can live inside a captured graph).
+__global__ void reset_counter_kernel(uint32_t *c, uint32_t n)
 {
-    if (threadIdx.x == 0) *c = 0;
+    if (threadIdx.x < n) c[threadIdx.x] = 0;
 }
 
 }  // namespace panns

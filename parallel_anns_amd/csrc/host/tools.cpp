@@ -202,6 +202,18 @@ extern "C" int pt_gen_mixture(uint64_t seed_centers, uint64_t seed_rows, uint64_
     return PT_OK;
 }
 
+extern "C" int pt_mixture_labels(uint64_t seed_rows, uint64_t row0, uint32_t n, uint32_t n_comp,
+                                 uint32_t *labels, int nthreads)
+{
+    if (!labels || n_comp == 0) return fail(PT_ERR_ARG, "pt_mixture_labels: bad argument");
+#pragma omp parallel for schedule(static) num_threads(nthreads_or_default(nthreads))
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        Normal g(seed_rows, row0 + (uint64_t)i);
+        labels[i] = g.rng.below(n_comp);
+    }
+    return PT_OK;
+}
+
 // ---------------------------------------------------------------------------
 // fvecs: per row int32 D followed by D float32 (include/utils.h:95-142).
 extern "C" int pt_fvecs_info(const char *path, uint32_t *n, uint32_t *dim)

@@ -23,6 +23,11 @@ CONFIGS = {
     "c0": dict(n=100_000, nq=1_000, dim=96, comps=64, sigma=0.3, knn=64, R=32, K=100, L=100),
     # C1: 1M base, 10k queries, exact 100-NN -> R <= 50
     "c1": dict(n=1_000_000, nq=10_000, dim=96, comps=64, sigma=0.3, knn=100, R=50, K=100, L=100),
+    # C2: 10M base from 256 components, 100k queries of uneven L and K; the
+    # exact 64-NN lists are computed per mixture component (checked against
+    # global brute force on a sample of rows)
+    "c2": dict(n=10_000_000, nq=100_000, dim=96, comps=256, sigma=0.3, knn=64, R=32, K=100, L=100,
+               knn_by_component=True, L_range=(100, 2000), K_choices=(10, 100)),
     # small cases for tests
     "t20k": dict(n=20_000, nq=200, dim=96, comps=64, sigma=0.3, knn=64, R=32, K=100, L=100),
     "t5k": dict(n=5_000, nq=64, dim=96, comps=16, sigma=0.3, knn=32, R=24, K=50, L=64),
@@ -47,6 +52,37 @@ def knn_graph(base, k, nthreads=0, gpu_knn=None):
         return gpu_knn(base, k)
     ids, _ = tools.exact_knn(base, base, k, exclude_self=True, nthreads=nthreads)
     return ids
+
+
+def knn_by_component(cfg, base, k, gpu_knn, check_rows=2000, verbose=True):
+    """Exact k-NN lists computed within each mixture component (ids stay in
+    ascending order inside a component, so ties still go to the lower id).
+    On the configured mixtures a row's k nearest rows all share its
+    component; `check_rows` evenly spaced rows are compared with global
+    brute force and the agreement is logged (and returned)."""
+    n = base.shape[0]
+    labels = tools.mixture_labels(n, cfg["comps"], SEED_BASE)
+    order = np.argsort(labels, kind="stable")
+    counts = np.bincount(labels, minlength=cfg["comps"])
+    if counts.min() <= k:
+        raise ValueError("a mixture component holds no more than k rows")
+    ids = np.empty((n, k), np.uint32)
+    start = 0
+    for c in range(cfg["comps"]):
+        members = order[start:start + counts[c]]
+        start += counts[c]
+        local = gpu_knn(np.ascontiguousarray(base[members]), k)
+        ids[members] = members[local]
+    rows = np.linspace(0, n - 1, min(check_rows, n)).astype(np.int64)
+    g_ids, _ = gpu_knn.truth(base, np.ascontiguousarray(base[rows]), k + 1)
+    agree = 0
+    for r, row in zip(rows, g_ids):
+        row = row[row != r][:k]
+        agree += int((row == ids[r]).all())
+    frac = agree / rows.shape[0]
+    if verbose:
+        log(f"per-component {k}-NN equals global brute force on {agree}/{rows.shape[0]} sampled rows")
+    return ids, frac
 
 
 class GpuKnn:
@@ -80,6 +116,17 @@ def cfg_key(cfg):
     return hashlib.sha256(json.dumps(cfg, sort_keys=True).encode()).hexdigest()[:12]
 
 
+def uneven_L_K(cfg, nq, seed=11):
+    """Per-query L log-uniform in cfg["L_range"] and K drawn from
+    cfg["K_choices"] (K <= L), seeded."""
+    rng = np.random.default_rng(seed)
+    lo, hi = cfg["L_range"]
+    L = np.exp(rng.uniform(np.log(lo), np.log(hi), nq)).astype(np.uint32)
+    L = np.clip(L, lo, hi)
+    K = rng.choice(np.asarray(cfg["K_choices"], np.uint32), nq)
+    return L, np.minimum(K, L).astype(np.uint32)
+
+
 def prepare(name_or_cfg, cache_dir=None, nthreads=0, gpu_knn=None, truth_k=100, verbose=True):
     """Return dict(base, queries, adj, deg, ep, gt_ids, gt_dists, cfg)."""
     cfg = CONFIGS[name_or_cfg] if isinstance(name_or_cfg, str) else dict(name_or_cfg)
@@ -105,7 +152,10 @@ def prepare(name_or_cfg, cache_dir=None, nthreads=0, gpu_knn=None, truth_k=100, 
     if verbose:
         log(f"generated {base.shape} + {queries.shape} in {time.time() - t0:.1f}s")
     t0 = time.time()
-    knn = knn_graph(base, cfg["knn"], nthreads, gpu_knn)
+    if cfg.get("knn_by_component"):
+        knn, _ = knn_by_component(cfg, base, cfg["knn"], gpu_knn, verbose=verbose)
+    else:
+        knn = knn_graph(base, cfg["knn"], nthreads, gpu_knn)
     if verbose:
         log(f"exact {cfg['knn']}-NN in {time.time() - t0:.1f}s")
     t0 = time.time()

@@ -67,7 +67,7 @@ def train(index, X, Y, epochs=4, batch=4096, lr=2e-3, seed=0):
 
 def recall_work(index, qids, gt, K, L, M, threshold):
     """(recall@K, mean distance computations, kernel ms) of one learned-stop run."""
-    plan = index.plan(qids, K, L, M, stop="learned", stop_threshold=threshold)
+    plan = index.plan(qids, K, L, M, stop="learned", stop_threshold=threshold, want_queue=False)
     ms = plan.run()
     res = plan.fetch()
     plan.close()
@@ -93,17 +93,34 @@ def calibrate(index, qids, gt, K, L, M, target, lo=1e-4, hi=1.0, steps=14):
 
 
 def oracle_bound(per_query_recall, per_query_ndist, target):
-    """Mean distance computations when every query may pick its own L from the
-    measured grid (columns), the best any per-query stop rule can do on that
-    grid: the cheapest per-query recall floor whose mean reaches target."""
-    R = np.asarray(per_query_recall)
+    """Least mean distance computations at mean recall >= target when every
+    query may take its own L from the measured grid (columns): a bound for
+    any per-query stop rule that stops at one of those L.
+
+    Lagrangian relaxation: at price lam each query takes the column that
+    minimises ndist - lam * recall, and the mean recall of that choice rises
+    with lam; bisect for the smallest lam that reaches target.  This is
+    exact at the vertices of the convex hull of the (recall, ndist) choices,
+    which is where the bisection ends up.  Returns (ndist, recall, lam)."""
+    R = np.asarray(per_query_recall, np.float64)
     N = np.asarray(per_query_ndist, np.float64)
-    best = None
-    for r0 in np.linspace(0.0, 1.0, 101):
-        ok = R >= r0 - 1e-9
-        pick = np.where(ok.any(1), ok.argmax(1), R.shape[1] - 1)
-        rec = R[np.arange(R.shape[0]), pick].mean()
-        nd = N[np.arange(R.shape[0]), pick].mean()
-        if rec >= target and (best is None or nd < best[0]):
-            best = (float(nd), float(rec), float(r0))
-    return best
+    rows = np.arange(R.shape[0])
+
+    def choose(lam):
+        c = np.argmin(N - lam * R, axis=1)
+        return float(N[rows, c].mean()), float(R[rows, c].mean())
+
+    lo, hi = 0.0, 1.0
+    while choose(hi)[1] < target and hi < 1e12:
+        hi *= 4.0
+    nd, rec = choose(hi)
+    if rec < target:
+        return None
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        if choose(mid)[1] >= target:
+            hi = mid
+        else:
+            lo = mid
+    nd, rec = choose(hi)
+    return (nd, rec, hi)

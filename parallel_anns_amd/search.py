@@ -45,8 +45,9 @@ def counters_dict(c):
 class Plan:
     """A resident batch: inputs in HBM, re-runnable, results fetched on demand."""
 
-    def __init__(self, index, handle, nq, max_k, max_l):
+    def __init__(self, index, handle, nq, max_k, max_l, keeps_queue=True):
         self.index, self.h, self.nq, self.max_k, self.max_l = index, handle, nq, max_k, max_l
+        self.keeps_queue = keeps_queue
 
     def run(self, sync=True):
         ms = _native.c_float(0.0)
@@ -58,7 +59,18 @@ class Plan:
         _chk(_native.amd().panns_plan_describe(self.h, g, lds, hs, ml))
         return {"workgroups": g.value, "lds_bytes": lds.value, "hash_slots": hs.value, "max_L": ml.value}
 
+    def buckets(self):
+        """Per queue-size bucket: largest L, queries, workgroups, LDS bytes,
+        visited-table slots."""
+        n = _native.c_u32()
+        arrs = [np.zeros(_native.MAX_BUCKETS, np.uint32) for _ in range(5)]
+        _chk(_native.amd().panns_plan_describe_buckets(self.h, n, *(ptr(a) for a in arrs)))
+        keys = ("max_L", "nq", "workgroups", "lds_bytes", "hash_slots")
+        return [{k: int(a[b]) for k, a in zip(keys, arrs)} for b in range(n.value)]
+
     def fetch(self, want_queue=False):
+        if want_queue and not self.keeps_queue:
+            raise ValueError("this plan was created with want_queue=False")
         nq, K, L = self.nq, self.max_k, self.max_l
         res = {"ids": np.empty((nq, K), np.uint32), "dists": np.empty((nq, K), np.float32),
                "ndist": np.empty(nq, np.uint32), "nexp": np.empty(nq, np.uint32),
@@ -176,21 +188,59 @@ class Index:
         res["counters"] = counters_dict(c)
         return res
 
-    def plan(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, L_per_query=None,
-             K_per_query=None, stop="fixed", stop_threshold=0.5, hash_slots=0):
+    @staticmethod
+    def _batch_args(query_ids, K, L, L_per_query, K_per_query):
         qids = np.ascontiguousarray(np.atleast_1d(query_ids), dtype=np.uint32)
         Lq = None if L_per_query is None else np.ascontiguousarray(L_per_query, dtype=np.uint32)
         Kq = None if K_per_query is None else np.ascontiguousarray(K_per_query, dtype=np.uint32)
         maxL = int(Lq.max()) if Lq is not None else L
         maxK = int(Kq.max()) if Kq is not None else K
+        return qids, Lq, Kq, maxL, maxK
+
+    @staticmethod
+    def plan_options(buckets=1, bucket_max_L=None, table_shift=0, use_graph=False, concurrent=False,
+                     want_queue=False):
+        o = _native.PannsPlanOptions()
+        o.n_buckets = len(bucket_max_L) if bucket_max_L else buckets
+        for i, v in enumerate(bucket_max_L or ()):
+            o.bucket_max_L[i] = int(v)
+        o.table_shift, o.use_graph, o.concurrent = table_shift, int(use_graph), int(concurrent)
+        o.want_queue = int(want_queue)
+        return o
+
+    def plan(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, L_per_query=None,
+             K_per_query=None, stop="fixed", stop_threshold=0.5, hash_slots=0, options=None,
+             want_queue=True):
+        """A resident batch (panns_plan_create_ex).  options: a
+        PannsPlanOptions (plan_options() or autotune()) for queue-size
+        buckets, graph replay and side-by-side buckets; want_queue keeps the
+        final queues for fetch(want_queue=True)."""
+        qids, Lq, Kq, maxL, maxK = self._batch_args(query_ids, K, L, L_per_query, K_per_query)
         if init_ids is None:
             init_ids = self.prepare_init_ids(maxL)
         init_ids = np.ascontiguousarray(init_ids, dtype=np.uint32)
         p = make_params(K, L, M, mode, 1, stop, stop_threshold, hash_slots)
+        o = _native.PannsPlanOptions() if options is None else options
+        o.want_queue = int(want_queue)
         h = _native.vp()
-        _chk(self.lib.panns_plan_create(self.h, p, ptr(qids), qids.shape[0], ptr(Lq), ptr(Kq),
-                                        ptr(init_ids), init_ids.shape[0], h))
-        return Plan(self, h, qids.shape[0], maxK, maxL)
+        _chk(self.lib.panns_plan_create_ex(self.h, p, ptr(qids), qids.shape[0], ptr(Lq), ptr(Kq),
+                                           ptr(init_ids), init_ids.shape[0], o, h))
+        return Plan(self, h, qids.shape[0], maxK, maxL, keeps_queue=bool(want_queue))
+
+    def autotune(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, L_per_query=None,
+                 K_per_query=None, stop="fixed", stop_threshold=0.5, sample_stride=0):
+        """The fastest bucketing for this batch (panns_plan_autotune): returns
+        (PannsPlanOptions, device ms on the sample)."""
+        qids, Lq, Kq, maxL, maxK = self._batch_args(query_ids, K, L, L_per_query, K_per_query)
+        if init_ids is None:
+            init_ids = self.prepare_init_ids(maxL)
+        init_ids = np.ascontiguousarray(init_ids, dtype=np.uint32)
+        p = make_params(K, L, M, mode, 1, stop, stop_threshold, 0)
+        best = _native.PannsPlanOptions()
+        ms = _native.c_float(0.0)
+        _chk(self.lib.panns_plan_autotune(self.h, p, ptr(qids), qids.shape[0], ptr(Lq), ptr(Kq),
+                                          ptr(init_ids), init_ids.shape[0], sample_stride, best, ms))
+        return best, ms.value
 
     def search_trace(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, trace_cap=256):
         """Fixed-L search that also records the stop features and the top-K

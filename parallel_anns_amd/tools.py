@@ -40,6 +40,13 @@ def gen_mixture(n, dim, n_comp, sigma, seed_centers, seed_rows, row0=0, normaliz
     return out
 
 
+def mixture_labels(n, n_comp, seed_rows, row0=0, nthreads=0):
+    """Mixture component of rows [row0, row0+n) of gen_mixture's stream."""
+    out = np.empty(n, dtype=np.uint32)
+    _chk(_native.tools().pt_mixture_labels(seed_rows, row0, n, n_comp, ptr(out), nthreads), "mixture_labels")
+    return out
+
+
 def fvecs_write(path, data):
     data = _f32(data)
     _chk(_native.tools().pt_fvecs_write(str(path).encode(), ptr(data), data.shape[0], data.shape[1]),

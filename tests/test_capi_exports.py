@@ -49,3 +49,20 @@ def test_product_does_not_import_the_oracle():
             if f.endswith((".py", ".cpp", ".hip", ".h")):
                 src = open(os.path.join(dirpath, f)).read()
                 assert "pyoracle" not in src and "panns_oracle" not in src, f
+
+
+def test_dropin_header_and_driver_build(tmp_path):
+    """include/panns/Searching.h compiles warning-free, its value types behave
+    like the reference's (Candidate order, Boost bitset block layout), and
+    the find-L driver links and prints its usage without touching a GPU."""
+    import subprocess
+    exe = os.path.join(str(tmp_path), "dropin_types")
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "native", "dropin_types.cpp"), "-o", exe], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr
+    from parallel_anns_amd import build
+    driver = os.path.join(build.BIN, "find_L_top_m")
+    assert os.path.exists(driver)
+    r = subprocess.run([driver], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage:" in r.stderr

@@ -145,3 +145,15 @@ def test_cpu_search_recall_sanity(tiny_set):
     assert tools.recall(ds["gt_ids"], ids, 64, R=50) > 0.9
     # the config table is what the benchmark uses
     assert datasets.CONFIGS["c1"]["n"] == 1_000_000 and datasets.CONFIGS["c1"]["R"] == 50
+
+
+def test_mixture_labels_are_the_generators_components():
+    """Every row lies nearest to the mean of the rows sharing its label (the
+    components are far apart), and labels are reproducible per row."""
+    X = tools.gen_mixture(2000, 96, 16, 0.3, 0, 1)
+    lab = tools.mixture_labels(2000, 16, 1)
+    assert lab.max() < 16 and len(set(lab.tolist())) == 16
+    C = np.stack([X[lab == c].mean(0) for c in range(16)])
+    d = ((X[:, None, :] - C[None]) ** 2).sum(-1)
+    assert (d.argmin(1) == lab).all()
+    assert (tools.mixture_labels(500, 16, 1, row0=1500) == lab[1500:]).all()

@@ -88,3 +88,35 @@ def test_learned_stop_semantics(small_set, gpu_index):
     # stopping early never reorders what was found: the partial queue is sorted
     assert (np.diff(mid["dists"], axis=1) >= 0).all()
     assert tools.recall(ds["gt_ids"], mid["ids"], L) <= tools.recall(ds["gt_ids"], fixed["ids"], L) + 1e-9
+
+
+def test_in_loop_model_matches_oracle(small_set, gpu_index):
+    """Each query stops at the first iteration where the FP64 oracle of the
+    network (64x64 layer weights rounded to bf16, as the kernel holds them),
+    evaluated on the traced features, reaches the threshold; queries with a
+    step within 1e-3 of the threshold before that are not judged (the kernel
+    folds the normalisation into the first layer in fp32)."""
+    ds = small_set
+    K, L, M = 100, 160, 4
+    q = np.arange(ds["queries"].shape[0], dtype=np.uint32)
+    feat, hsh, niter, _ = gpu_index.search_trace(q, K, L, M, trace_cap=256)
+    assert int(niter.max()) <= 256
+    X, Y = learned.samples_from_trace(feat, hsh, niter)
+    P, _ = learned.train(gpu_index, X, Y, epochs=2, batch=1024)
+    gpu_index.set_stop_model(P)
+    Pb = np.array(P, np.float64)
+    Pb[576:4672] = mo.bf16(P[576:4672])
+    for thr in (0.3, 0.5, 0.7):
+        print("case in-loop model threshold", thr, flush=True)
+        res = gpu_index.search(q, K, L, M, stop="learned", stop_threshold=thr)
+        judged = 0
+        for i in range(q.shape[0]):
+            n = int(niter[i])
+            _, p, _ = mo.forward_backward(Pb[:mo.N_PARAMS], mo.normalise(feat[i, :n], P), np.zeros(n))
+            fire = np.nonzero(p >= thr)[0]
+            expect = int(fire[0]) + 1 if fire.size else n
+            if np.any(np.abs(p[:expect] - thr) < 1e-3):
+                continue
+            judged += 1
+            assert int(res["niter"][i]) == expect, f"query {i} threshold {thr}"
+        assert judged >= 0.9 * q.shape[0]
