@@ -40,7 +40,7 @@ __device__ __forceinline__ void merge_batch_wave(uint64_t *Q, uint32_t &S, uint3
         valid = pos < L;
     }
     const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
-    const uint32_t p0 = __shfl(pos, 0);
+    const uint32_t p0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)pos);
     if (nv == 0) return;
     if (valid) bb[lane] = x;
     wave_sync();
@@ -95,6 +95,7 @@ __global__ __launch_bounds__(256) void knn_kernel(const float *__restrict__ base
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
+    const XorSel xs = make_xor_sel(lane);
     const int ql = tid / kKnnTPQ;
     const int sub = tid % kKnnTPQ;
     const uint32_t q = blockIdx.x * kKnnQT + ql;
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(256) void knn_kernel(const float *__restrict__ base
             const uint32_t cnt = ccnt[qq];
             if (cnt == 0) continue;
             uint64_t x = (uint32_t)lane < cnt ? cbuf[qq * 64 + lane] : kKeyMax;
-            x = sort64(x, lane);
+            x = sort64(x, lane, xs);
             uint32_t S = lsz[qq];
             merge_batch_wave(lists + qq * kKnnKCap, S, k, x, cnt, bbs + wave * 64, lane);
             if (lane == 0) {

@@ -37,6 +37,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace panns {
 
 constexpr uint32_t kSent = 0xFFFFFFFFu;
@@ -140,30 +142,129 @@ __device__ __forceinline__ uint64_t mk_key(float d, uint32_t id)
 }
 __device__ __forceinline__ uint64_t km(uint64_t e) { return e & ~kChk; }
 
-__device__ __forceinline__ uint64_t shfl_xor64(uint64_t x, int m)
+// Lane exchanges v -> v[lane ^ M] without the LDS crossbar: DPP moves for
+// M <= 8 and gfx950's permlane swaps for 16 and 32.  For M = 4 (two row
+// rotations) and M = 16, 32 (two halves of a swap) each lane picks one of
+// two results; which one is fixed by the hardware's lane order, so it is
+// measured once per kernel on the lane ids themselves (make_xor_sel)
+// rather than assumed.
+struct XorSel {
+    bool s4, s16, s32;
+};
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t v)
 {
-    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-    lo = __shfl_xor(lo, m);
-    hi = __shfl_xor(hi, m);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
+}
+
+constexpr int kDppXor1 = 0xB1;   // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;   // quad_perm [2,3,0,1]
+constexpr int kDppRor4 = 0x124;  // row_ror:4
+constexpr int kDppRor8 = 0x128;  // row_ror:8
+constexpr int kDppRor12 = 0x12C; // row_ror:12
+
+__device__ __forceinline__ XorSel make_xor_sel(int lane)
+{
+    const uint32_t l = (uint32_t)lane;
+    XorSel s;
+    s.s4 = dpp_mov<kDppRor4>(l) == (l ^ 4u);
+    const auto p16 = __builtin_amdgcn_permlane16_swap(l, l, false, false);
+    s.s16 = p16[0] == (l ^ 16u);
+    const auto p32 = __builtin_amdgcn_permlane32_swap(l, l, false, false);
+    s.s32 = p32[0] == (l ^ 32u);
+    return s;
+}
+
+template <int M>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v, const XorSel &s)
+{
+    if constexpr (M == 1) {
+        return dpp_mov<kDppXor1>(v);
+    } else if constexpr (M == 2) {
+        return dpp_mov<kDppXor2>(v);
+    } else if constexpr (M == 4) {
+        const uint32_t a = dpp_mov<kDppRor4>(v), b = dpp_mov<kDppRor12>(v);
+        return s.s4 ? a : b;
+    } else if constexpr (M == 8) {
+        return dpp_mov<kDppRor8>(v);
+    } else if constexpr (M == 16) {
+        const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return s.s16 ? p[0] : p[1];
+    } else {
+        static_assert(M == 32, "lane exchange distance");
+        const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return s.s32 ? p[0] : p[1];
+    }
+}
+
+template <int M>
+__device__ __forceinline__ float xor_lane_f(float v, const XorSel &s)
+{
+    return __uint_as_float(xor_lane<M>(__float_as_uint(v), s));
+}
+
+template <int M>
+__device__ __forceinline__ uint64_t xor_lane64(uint64_t x, const XorSel &s)
+{
+    const uint32_t lo = xor_lane<M>((uint32_t)x, s), hi = xor_lane<M>((uint32_t)(x >> 32), s);
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Bitonic sort of one key per lane, ascending across the 64 lanes.
-__device__ __forceinline__ uint64_t sort64(uint64_t x, int lane)
+// Butterfly sum over the wave (exchange distances 32, 16, ..., 1, in that
+// order); every lane gets the total.
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x, const XorSel &s)
 {
-#pragma unroll
-    for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            const uint64_t y = shfl_xor64(x, j);
-            const bool up = (lane & k) == 0;
-            const bool lower = (lane & j) == 0;
-            const uint64_t mn = x < y ? x : y;
-            const uint64_t mx = x < y ? y : x;
-            x = (lower == up) ? mn : mx;
-        }
+    if constexpr (sizeof(T) == 4 && std::is_floating_point<T>::value) {
+        x += xor_lane_f<32>(x, s);
+        x += xor_lane_f<16>(x, s);
+        x += xor_lane_f<8>(x, s);
+        x += xor_lane_f<4>(x, s);
+        x += xor_lane_f<2>(x, s);
+        x += xor_lane_f<1>(x, s);
+    } else {
+        x += xor_lane<32>(x, s);
+        x += xor_lane<16>(x, s);
+        x += xor_lane<8>(x, s);
+        x += xor_lane<4>(x, s);
+        x += xor_lane<2>(x, s);
+        x += xor_lane<1>(x, s);
     }
     return x;
+}
+
+template <int K, int J>
+__device__ __forceinline__ uint64_t bitonic_step(uint64_t x, int lane, const XorSel &s)
+{
+    const uint64_t y = xor_lane64<J>(x, s);
+    const bool up = (lane & K) == 0;
+    const bool lower = (lane & J) == 0;
+    const uint64_t mn = x < y ? x : y;
+    const uint64_t mx = x < y ? y : x;
+    return (lower == up) ? mn : mx;
+}
+
+template <int K>
+__device__ __forceinline__ uint64_t bitonic_merge(uint64_t x, int lane, const XorSel &s)
+{
+    if constexpr (K >= 64) x = bitonic_step<K, 32>(x, lane, s);
+    if constexpr (K >= 32) x = bitonic_step<K, 16>(x, lane, s);
+    if constexpr (K >= 16) x = bitonic_step<K, 8>(x, lane, s);
+    if constexpr (K >= 8) x = bitonic_step<K, 4>(x, lane, s);
+    if constexpr (K >= 4) x = bitonic_step<K, 2>(x, lane, s);
+    return bitonic_step<K, 1>(x, lane, s);
+}
+
+// Bitonic sort of one key per lane, ascending across the 64 lanes.
+__device__ __forceinline__ uint64_t sort64(uint64_t x, int lane, const XorSel &s)
+{
+    x = bitonic_merge<2>(x, lane, s);
+    x = bitonic_merge<4>(x, lane, s);
+    x = bitonic_merge<8>(x, lane, s);
+    x = bitonic_merge<16>(x, lane, s);
+    x = bitonic_merge<32>(x, lane, s);
+    return bitonic_merge<64>(x, lane, s);
 }
 
 // Block order of the reference's 8-wide accumulation: the tail block first
@@ -214,7 +315,7 @@ __device__ __forceinline__ void pair_dist(const float *__restrict__ base, const 
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
         const float sj = ((acc[j][0] + acc[j][1]) + acc[j][2]) + acc[j][3];
-        const float o = __shfl_xor(sj, 1);
+        const float o = __uint_as_float(dpp_mov<kDppXor1>(__float_as_uint(sj)));
         d[j] = (((o + acc[j][0]) + acc[j][1]) + acc[j][2]) + acc[j][3];
     }
 }
@@ -251,7 +352,7 @@ __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32
         valid = pos < L;
     }
     const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
-    const uint32_t p0 = __shfl(pos, 0);
+    const uint32_t p0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)pos);
     if (nv == 0) return kSent;
     if (ninserted) *ninserted += nv;
     if (valid) bb[lane] = x;
@@ -474,7 +575,8 @@ __device__ __forceinline__ LaneModel load_lane_model(const StopModel &m, int lan
 // sigmoid output in every lane.  (Folding the scaling into the first layer
 // would cancel catastrophically for a feature of near-zero spread, such as
 // the norm of normalised queries.)
-__device__ __forceinline__ float mlp_eval(const LaneModel &lm, const float (&f)[kFeat], float *h1s, int lane)
+__device__ __forceinline__ float mlp_eval(const LaneModel &lm, const float (&f)[kFeat], float *h1s, int lane,
+                                          const XorSel &xs)
 {
     float h = lm.b1;
 #pragma unroll
@@ -494,9 +596,8 @@ __device__ __forceinline__ float mlp_eval(const LaneModel &lm, const float (&f)[
         g = __builtin_fmaf(__uint_as_float(p1 & 0xFFFF0000u), x.w, g);
     }
     g = g > 0.f ? g : 0.f;
-    float o = lm.w3 * g;
-#pragma unroll
-    for (int s = 32; s > 0; s >>= 1) o += __shfl_xor(o, s);
+    const float o0 = lm.w3 * g;
+    float o = wave_sum(o0, xs);
     __syncthreads();
     o += lm.b3;
     return 1.f / (1.f + __expf(-o));
@@ -562,6 +663,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
     extern __shared__ uint64_t smem[];
     const int lane = threadIdx.x;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const XorSel xs = make_xor_sel(lane);
 
     uint64_t *Q = smem;
     uint64_t *cand = Q + a.Lcap;
@@ -631,7 +733,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                 (void)n0;
                 __syncthreads();
                 uint64_t x = (uint32_t)lane < cnt ? cand[lane] : kKeyMax;
-                x = sort64(x, lane);
+                x = sort64(x, lane, xs);
                 merge_batch(Q, S, L, x, cnt, bb, lane);
             }
             ndist = L;
@@ -648,8 +750,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                 const float4 q = qs[lane];
                 qnorm = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
             }
-#pragma unroll
-            for (int s = 32; s > 0; s >>= 1) qnorm += __shfl_xor(qnorm, s);
+            qnorm = wave_sum(qnorm, xs);
             qnorm = sqrtf(qnorm);
         }
         PANNS_PT(kPhInit);
@@ -719,7 +820,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                 for (uint32_t b0 = 0; b0 < ncand; b0 += 64) {
                     const uint32_t cnt = min(64u, ncand - b0);
                     uint64_t x = (uint32_t)lane < cnt ? cand[b0 + lane] : kKeyMax;
-                    x = sort64(x, lane);
+                    x = sort64(x, lane, xs);
                     const uint32_t p0 = merge_batch(Q, S, L, x, cnt, bb, lane);
                     if (p0 != kSent) nk = min(nk, p0);
                 }
@@ -734,8 +835,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                 // the queue's last key in the last step, first-init (entry) dist, query norm, unchecked count
                 uint32_t unc = 0;
                 for (uint32_t j = lane; j < S; j += 64) unc += (Q[j] & kChk) ? 0u : 1u;
-#pragma unroll
-                for (int s = 32; s > 0; s >>= 1) unc += __shfl_xor(unc, s);
+                unc = wave_sum(unc, xs);
                 float f[kFeat];
                 const float d1 = __uint_as_float((uint32_t)(Q[0] >> 32));
                 const float dk = __uint_as_float((uint32_t)(Q[min(K, S) - 1] >> 32));
@@ -756,12 +856,11 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                         x ^= x >> 15;
                         hsh += x * 0x85EBCA77u;
                     }
-#pragma unroll
-                    for (int s = 32; s > 0; s >>= 1) hsh += __shfl_xor(hsh, s);
+                    hsh = wave_sum(hsh, xs);
                     if (lane == 0) a.trace_topk[(size_t)w * a.trace_cap + (niter - 1)] = hsh;
                 }
                 if constexpr (kLearned) {
-                    const float pstop = mlp_eval(lm, f, h1s, lane);
+                    const float pstop = mlp_eval(lm, f, h1s, lane, xs);
                     if (pstop >= a.stop_threshold) break;
                 }
                 PANNS_PT(kPhStop);
@@ -776,15 +875,17 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
             if (a.out_dists) a.out_dists[(size_t)w * a.Kstride + i] = have ? __uint_as_float((uint32_t)(e >> 32)) : __int_as_float(0x7f800000);
         }
         if (a.out_qids) {
-            for (uint32_t i = lane; i < L; i += 64) {
-                const uint64_t e = Q[i];
-                a.out_qids[(size_t)w * a.Lstride + i] = (uint32_t)e & kIdMask;
-                a.out_qdists[(size_t)w * a.Lstride + i] = __uint_as_float((uint32_t)(e >> 32));
-                a.out_qchk[(size_t)w * a.Lstride + i] = (e & kChk) ? 1 : 0;
+            // rows of queries with L below the batch's largest are padded
+            for (uint32_t i = lane; i < a.Lstride; i += 64) {
+                const bool have = i < L;
+                const uint64_t e = have ? Q[i] : 0ull;
+                a.out_qids[(size_t)w * a.Lstride + i] = have ? ((uint32_t)e & kIdMask) : kSent;
+                a.out_qdists[(size_t)w * a.Lstride + i] = have ? __uint_as_float((uint32_t)(e >> 32))
+                                                              : __int_as_float(0x7f800000);
+                a.out_qchk[(size_t)w * a.Lstride + i] = (have && (e & kChk)) ? 1 : 0;
             }
         }
-#pragma unroll
-        for (int s = 32; s > 0; s >>= 1) nadj += __shfl_xor(nadj, s);
+        nadj = wave_sum(nadj, xs);
         const uint32_t nsp = vis.spills;
         if (lane == 0) {
             uint32_t *st = a.out_stats + (size_t)w * 5;
