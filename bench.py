@@ -128,7 +128,7 @@ def cpu_baseline(ds, K, L, M, init_ids, budget_s=12.0):
 
 def fixed_operating_point(index, qids, K, gt, target, ms, nq, op=None, hi=1024):
     """Fastest (M, L) whose recall reaches target: L by bisection per M, each
-    candidate timed over 3 launches."""
+    candidate timed over 3 launches.  Returns (M, L, {M: L})."""
     if op:
         cands = [op]
     else:
@@ -147,12 +147,17 @@ def fixed_operating_point(index, qids, K, gt, target, ms, nq, op=None, hi=1024):
         log(f"target {target}: fixed candidate M={M} L={L}: {qps:,.0f} QPS")
         if best is None or qps > best[0]:
             best = (qps, M, L)
-    return None if best is None else best[1:]
+    return None if best is None else (best[1], best[2], dict(cands))
 
 
 def timed(index, qids, K, L, M, gt, steps, warmup, **kw):
-    plan = index.plan(qids, K, L, M, want_queue=False, **kw)
+    """One plan over the batch, launch shape from the autotuner (visited-table
+    size; buckets when L varies), timed over `steps` launches."""
+    opt, _ = index.autotune(qids, K, L, M, **kw)
+    plan = index.plan(qids, K, L, M, want_queue=False, options=opt, **kw)
     desc = plan.describe()
+    desc["autotuned"] = {"table_eighths": opt.table_eighths, "n_buckets": opt.n_buckets,
+                         "hash_slots": plan.buckets()[0]["hash_slots"]}
     wall, kms = time_plan(plan, steps, warmup)
     res = plan.fetch()
     plan.close()
@@ -160,7 +165,7 @@ def timed(index, qids, K, L, M, gt, steps, warmup, **kw):
     return res, desc, {"M": M, "L": L, "qps": round(steps * qids.shape[0] / wall, 1),
                        "recall": round(tools.recall(gt, res["ids"], L), 4),
                        "kernel_ms": round(float(np.mean(kms)), 4),
-                       "ndist": round(float(res["ndist"].mean()), 1)}
+                       "ndist": round(float(res["ndist"].mean()), 1), "table_eighths": int(opt.table_eighths)}
 
 
 def deficit_distribution(d):
@@ -224,14 +229,15 @@ def main():
 
     # ---- fixed-L operating points
     op = tuple(int(x) for x in args.op.split(",")) if args.op else None
-    fM, fL = fixed_operating_point(index, qids, K, gt, target, ms, nq, op)
-    fres, _, fixed = timed(index, qids, K, fL, fM, gt, args.steps, args.warmup)
+    fM, fL, per_M = fixed_operating_point(index, qids, K, gt, target, ms, nq, op)
+    fres, fdesc, fixed = timed(index, qids, K, fL, fM, gt, args.steps, args.warmup)
     log(f"fixed-L: {fixed}")
-    fixed_other = {}
+    fixed_other, L_at = {}, {target: per_M.get(fM, fL)}
     for t in others:
         o = fixed_operating_point(index, qids, K, gt, t, ms, nq, hi=2048)
         if o is not None:
             fixed_other[t] = timed(index, qids, K, o[1], o[0], gt, 3, 1)[2]
+            L_at[t] = o[2].get(fM, o[1])
             log(f"fixed-L at {t}: {fixed_other[t]}")
 
     # ---- per-query bound on an L grid around the operating point
@@ -283,8 +289,12 @@ def main():
             index.set_stop_model(model)
             picks = {}
             for t in [target] + sorted(fixed_other):
-                Lf = fL if t == target else fixed_other[t]["L"]
-                best_l = None
+                # L_max candidates around the fixed-L operating L of this
+                # target at the model's M; the plain fixed-L search at that L
+                # (a threshold above 1 never stops early) is the fallback
+                Lf = L_at[t]
+                _, nd_fixed, _ = learned.recall_work(index, vq, vgt, K, Lf, lM, 2.0)
+                best_l = (nd_fixed, Lf, 2.0)
                 for f in (float(x) for x in args.learned_L.split(",")):
                     lL = min(4096, int(round(Lf * f)))
                     thr = learned.calibrate(index, vq, vgt, K, lL, lM, t + 0.002)
@@ -292,18 +302,19 @@ def main():
                         continue
                     rv, nd, _ = learned.recall_work(index, vq, vgt, K, lL, lM, thr)
                     log(f"target {t}: calibrated L={lL} threshold {thr:.4g}: validation recall {rv:.4f}, ndist {nd:.0f}")
-                    if best_l is None or nd < best_l[0]:
+                    if nd < best_l[0]:
                         best_l = (nd, lL, thr)
-                if best_l is not None:
-                    picks[t] = best_l[1:]
+                if best_l[2] >= 1.0:
+                    log(f"target {t}: no threshold beats the fixed search on validation; learned = fixed L={Lf}")
+                picks[t] = best_l[1:]
             index.set_queries(ds["queries"])
         index.set_stop_model(model)
         if target in picks:
             lL, thr = picks[target]
-            lres, ldesc, lrn = timed(index, qids, K, lL, lM, gt, args.steps, args.warmup, stop="learned",
-                                     stop_threshold=thr)
+            stop = dict(stop="learned", stop_threshold=thr) if thr < 1.0 else {}
+            lres, ldesc, lrn = timed(index, qids, K, lL, lM, gt, args.steps, args.warmup, **stop)
             lrn = {"M": lM, "L_max": lL, "threshold": thr, **{k: v for k, v in lrn.items() if k not in ("M", "L")},
-                   "training": train_info}
+                   "fallback_to_fixed": thr >= 1.0, "training": train_info}
             fmax = index.search(qids, K, lL, lM)
             lrn["recall_deficit_vs_fixed_at_L_max"] = deficit_distribution(
                 tools.recall_per_query(gt, fmax["ids"], lL) - tools.recall_per_query(gt, lres["ids"], lL))
@@ -311,10 +322,12 @@ def main():
         for t in sorted(fixed_other):
             if t in picks:
                 lL, thr = picks[t]
-                r = timed(index, qids, K, lL, lM, gt, 3, 1, stop="learned", stop_threshold=thr)[2]
+                stop = dict(stop="learned", stop_threshold=thr) if thr < 1.0 else {}
+                r = timed(index, qids, K, lL, lM, gt, 3, 1, **stop)[2]
                 lrn_other[t] = {"fixed_L": fixed_other[t],
-                                "learned": {"L_max": lL, "threshold": thr, "qps": r["qps"], "recall": r["recall"],
-                                            "ndist": r["ndist"]},
+                                "learned": {"M": lM, "L_max": lL, "threshold": thr, "qps": r["qps"],
+                                            "recall": r["recall"], "ndist": r["ndist"],
+                                            "fallback_to_fixed": thr >= 1.0},
                                 "learned_over_fixed": round(r["qps"] / fixed_other[t]["qps"], 3)}
                 log(f"target {t}: {lrn_other[t]}")
 
@@ -329,7 +342,7 @@ def main():
         cpu, cpu_ids = cpu_baseline(ds, K, fL, fM, init)
         cpu["ids_identical_to_gpu"] = bool((cpu_ids == fres["ids"][:cpu_ids.shape[0]]).all())
         log(f"cpu baseline {cpu['value']:,.0f} QPS on {cpu['cores']} cores; ids identical: {cpu['ids_identical_to_gpu']}")
-    kernel = f"search_kernel<{dim}, {'true' if lrn else 'false'}>"
+    kernel = f"search_kernel<{dim}, {'true' if lrn and not lrn['fallback_to_fixed'] else 'false'}>"
     traffic, traffic_src = measured_traffic(args.config, kernel, lrn or fixed)
 
     out = {
@@ -356,8 +369,9 @@ def main():
                    "per_query_stop_bound": oracle,
                    "other_targets": lrn_other or ({str(t): {"fixed_L": v} for t, v in fixed_other.items()}
                                                   if fixed_other else None),
-                   "workgroups": (ldesc or {}).get("workgroups"),
-                   "lds_bytes": (ldesc or {}).get("lds_bytes")},
+                   "workgroups": (ldesc or fdesc).get("workgroups"),
+                   "lds_bytes": (ldesc or fdesc).get("lds_bytes"),
+                   "launch": (ldesc or fdesc).get("autotuned")},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": traffic_src,

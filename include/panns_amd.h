@@ -55,7 +55,7 @@ typedef struct {
     uint32_t num_threads;  /* PSS_V5: the reference's num_threads_; only 1 is offered */
     uint32_t stop;         /* panns_stop */
     float stop_threshold;  /* LEARNED: stop when sigmoid output >= threshold */
-    uint32_t hash_slots;   /* 0 = automatic; else a power of two in [64, 32768]: 16-bit slots of
+    uint32_t hash_slots;   /* 0 = automatic; else a multiple of 64 in [64, 32768]: 16-bit slots of
                               the per-query LDS visited table (raised when the id space needs) */
     uint32_t reserved[4];
 } panns_params;
@@ -170,8 +170,8 @@ typedef struct {
     uint32_t n_buckets;                        /* 1..PANNS_MAX_BUCKETS (0 means 1) */
     uint32_t bucket_max_L[PANNS_MAX_BUCKETS];  /* ascending upper bounds of L; all 0: split the
                                                   queries into buckets of equal summed L */
-    uint32_t table_shift;                      /* visited-table size halved this many times (0-2)
-                                                  from the automatic size, for every bucket */
+    uint32_t table_eighths;                    /* visited tables at this many eighths (1-8; 0 means
+                                                  8) of the automatic size, for every bucket */
     uint32_t use_graph;                        /* capture and replay as one hipGraph */
     uint32_t concurrent;                       /* buckets side by side */
     uint32_t want_queue;                       /* keep the final queues for panns_plan_fetch */
@@ -187,8 +187,8 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
 int panns_plan_describe_buckets(panns_plan *plan, uint32_t *n_buckets, uint32_t *max_L, uint32_t *nq,
                                 uint32_t *workgroups, uint32_t *lds_bytes, uint32_t *hash_slots);
 /* The autotuner: times candidate bucketings (1, 2, 3, 4, 6 buckets of equal
- * summed L; side by side or not; full or halved visited tables; always as
- * a graph) on every `sample_stride`-th query of the batch (0: a stride that
+ * summed L; side by side or not; visited tables at 8/8, 7/8, 6/8 or 5/8 of
+ * the automatic size; always as a graph) on every `sample_stride`-th query of the batch (0: a stride that
  * leaves about 20,000 queries) and returns the fastest in *best, with its
  * bucket bounds filled in, and its device time on the sample in *best_ms
  * (nullable).  Results do not depend on the choice. */

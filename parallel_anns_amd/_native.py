@@ -45,7 +45,7 @@ MAX_BUCKETS = 8
 
 
 class PannsPlanOptions(ctypes.Structure):
-    _fields_ = [("n_buckets", c_u32), ("bucket_max_L", c_u32 * MAX_BUCKETS), ("table_shift", c_u32),
+    _fields_ = [("n_buckets", c_u32), ("bucket_max_L", c_u32 * MAX_BUCKETS), ("table_eighths", c_u32),
                 ("use_graph", c_u32), ("concurrent", c_u32), ("want_queue", c_u32), ("reserved", c_u32 * 4)]
 
 

@@ -114,7 +114,7 @@ struct panns_plan {
     bool want_queue = false;
     uint32_t *shared_bitmap = nullptr;
     uint32_t grid = 0, lds = 0, hash_slots = 0;
-    uint32_t table_shift = 0;
+    uint32_t table_eighths = 8;
     // scratch placement: bitmap/log slots [slot0, slot0 + grid) and the work
     // counter index, bound to the handle's buffers before every run
     uint32_t slot0 = 0, counter_idx = 0;
@@ -200,8 +200,8 @@ int check_params(panns_index *h, const panns_params *p, uint32_t L, uint32_t K)
     if (p->M == 0) return fail(PANNS_ERR_ARG, "M must be positive");
     if (p->M > kMaxM) return fail(PANNS_ERR_UNSUPPORTED, "M above 1024 is not offered");
     if (p->stop == PANNS_STOP_LEARNED && !h->has_model) return fail(PANNS_ERR_STATE, "no stop model loaded");
-    if (p->hash_slots && ((p->hash_slots & (p->hash_slots - 1)) || p->hash_slots < 64 || p->hash_slots > 32768))
-        return fail(PANNS_ERR_ARG, "hash_slots must be a power of two in [64, 32768]");
+    if (p->hash_slots && (p->hash_slots % 64 || p->hash_slots < 64 || p->hash_slots > 32768))
+        return fail(PANNS_ERR_ARG, "hash_slots must be a multiple of 64 in [64, 32768]");
     return PANNS_OK;
 }
 
@@ -265,17 +265,19 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     const uint32_t stride = 1u << h->stride_log2;
     const uint32_t nb_cap = std::max<uint32_t>(panns::kGroupSlots, stride);
     const uint32_t Mcap = std::max<uint32_t>(4u, (p.M + 3u) & ~3u);
-    // visited table: hs 16-bit slots in buckets of 8; the tag of an id has
-    // id_bits - lbits bits, which must stay <= 15 (0xFFFF marks empty)
+    // visited table: hs 16-bit slots in nb buckets of 8; the tag of an id
+    // is its scrambled value divided by nb and must stay below 2^15 (0xFFFF
+    // marks an empty slot, bit 15 a second-bucket entry), so nb grows with N
     uint32_t hs = p.hash_slots;
-    if (!hs) hs = std::max<uint32_t>(64u, (pl->maxL <= 512 ? 8192u : 16384u) >> pl->table_shift);
+    if (!hs) hs = ((pl->maxL <= 512 ? 8192u : 16384u) * pl->table_eighths / 8u + 63u) & ~63u;
     const uint32_t id_bits = std::max<uint32_t>(1u, log2u(h->n));
-    uint32_t lbits = log2u(hs / 8u);
-    if (id_bits > lbits + 15u) lbits = id_bits - 15u;
-    hs = 8u << lbits;
+    const uint64_t min_nb = (((1ull << id_bits) - 1ull) / 0x7FFEull) + 1ull;
+    uint32_t nb = std::max<uint32_t>(hs / 8u, (uint32_t)((min_nb + 7ull) & ~7ull));
+    nb = std::max<uint32_t>(nb, 8u);
+    hs = nb * 8u;
     if (pl->shared_bitmap) {
         hs = 64;
-        lbits = 3;
+        nb = 8;
     }
     pl->hash_slots = hs;
     // Lcap rounded to 8 keys keeps every LDS region 16-byte aligned.
@@ -355,7 +357,8 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     a.log_cap = kLogCap;
     a.Lcap = Lcap;
     a.tbl_words = hs / 2u;
-    a.tbl_lbits = lbits;
+    a.tbl_buckets = nb;
+    a.tbl_magic = (uint32_t)((1ull << 32) / nb);
     a.id_bits = id_bits;
     a.use_table = pl->shared_bitmap ? 0u : 1u;
     a.Mcap = Mcap;
@@ -804,7 +807,8 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
     if (opt) o = *opt;
     if (o.n_buckets == 0) o.n_buckets = 1;
     if (o.n_buckets > PANNS_MAX_BUCKETS) return fail(PANNS_ERR_ARG, "at most 8 buckets");
-    if (o.table_shift > 2) return fail(PANNS_ERR_ARG, "table_shift must be 0, 1 or 2");
+    if (o.table_eighths == 0) o.table_eighths = 8;
+    if (o.table_eighths > 8) return fail(PANNS_ERR_ARG, "table_eighths must be 1..8");
     HIP_TRY(hipSetDevice(h->device));
     std::unique_ptr<panns_plan> pl(new panns_plan());
     pl->h = h;
@@ -812,7 +816,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
     pl->nq = nq;
     pl->opt = o;
     pl->want_queue = o.want_queue != 0;
-    pl->table_shift = o.table_shift;
+    pl->table_eighths = o.table_eighths;
     if (o.n_buckets == 1 && !o.use_graph && !o.concurrent) {
         rc = build_plan(pl.get(), query_ids, L_per_query, K_per_query, init_ids, n_init);
         if (rc) return rc;
@@ -876,7 +880,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
         part->p = *p;
         part->nq = (uint32_t)m.size();
         part->want_queue = pl->want_queue;
-        part->table_shift = o.table_shift;
+        part->table_eighths = o.table_eighths;
         part->slot0 = o.concurrent ? slot : 0u;
         part->counter_idx = counter++;
         std::vector<uint32_t> q(m.size()), lq(m.size()), kq(m.size());
@@ -967,9 +971,9 @@ int panns_plan_autotune(panns_index *h, const panns_params *p, const uint32_t *q
         o.n_buckets = k;
         o.use_graph = 1;
         for (uint32_t conc = 0; conc < (k > 1 ? 2u : 1u); ++conc)
-            for (uint32_t shift = 0; shift < 2; ++shift) {
+            for (uint32_t eighths : {8u, 7u, 6u, 5u}) {
                 o.concurrent = conc;
-                o.table_shift = shift;
+                o.table_eighths = eighths;
                 panns_plan *pl = nullptr;
                 int rc = panns_plan_create_ex(h, p, sq.data(), (uint32_t)sq.size(), L_per_query ? sl.data() : nullptr,
                                               K_per_query ? sk.data() : nullptr, init_ids, n_init, &o, &pl);

@@ -89,7 +89,7 @@ struct KArgs {
     uint32_t *spill_log;  // [grid][log_cap]
     uint32_t log_cap;
     uint32_t Lcap;
-    uint32_t tbl_words, tbl_lbits, id_bits, use_table;  // LDS visited table
+    uint32_t tbl_words, tbl_buckets, tbl_magic, id_bits, use_table;  // LDS visited table
     uint32_t Mcap, nb_cap, group_picks;
     // subsearch on a caller-held queue and visited bitmap
     const uint64_t *init_queue;  // [n_work][L] keys, or null
@@ -393,11 +393,11 @@ __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32
 // Visited set of one query: an LDS tag table first, a per-workgroup global
 // bitmap for what the table cannot hold.
 //
-// The table has 2^lbits buckets of eight 16-bit slots (one 16-byte LDS read
-// per lookup).  An id is scrambled by a bijection of [0, 2^idbits)
-// (multiplication by an odd constant, then an xor-shift); the low lbits of the
-// result pick the bucket and the remaining idbits - lbits <= 15 bits are the
-// tag stored in it (0xFFFF marks an empty slot), so bucket and tag together
+// The table has nb buckets of eight 16-bit slots (one 16-byte LDS read per
+// lookup).  An id is scrambled by a bijection of [0, 2^idbits)
+// (multiplication by an odd constant, then an xor-shift); the remainder of
+// the result by nb picks the bucket and the quotient (< 2^15) is the tag
+// stored in it (0xFFFF marks an empty slot), so bucket and tag together
 // identify the id exactly.  An id whose bucket is full tries one second
 // bucket (probe()).  Slots are only ever filled during a query, so once both
 // buckets of an id are full, an id absent from them is decided by the
@@ -405,7 +405,7 @@ __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32
 // a wave-uniform count kept in registers.
 struct Visited {
     uint32_t *tbl;      // LDS: buckets x 4 words
-    uint32_t bmask, lbits, idbits, idmask, xshift, use;
+    uint32_t nb, magic, idbits, idmask, xshift, use;  // buckets, floor(2^32 / nb)
     uint32_t *bitmap;   // global
     uint32_t *log;      // global: words touched in the bitmap, may be null
     uint32_t log_cap;
@@ -448,21 +448,29 @@ struct Visited {
 
     // 0: visited before; 1: new and now in the table; 2: in neither of its
     // two buckets, both full -- the bitmap decides.
-    // The id's primary bucket is b1 = h & bmask with tag t = h >> lbits
-    // (t < 2^15).  Only when b1 is full does the id go to its second bucket
-    // b2 = b1 ^ alt(t), as t | 0x8000: slots never empty during a query, so
-    // an id can be in b2 only if b1 is full, and (bucket, stored tag) still
-    // determines the id (b1 = b2 ^ alt(t), then h, then the id, since the
-    // scramble is a bijection).  t = 0x7FFF has no second bucket (its
-    // secondary tag would read as an empty slot).
+    // With h = scramble(id) = t * nb + b1, the id's primary bucket is b1 and
+    // its tag t (< 2^15: the host keeps nb large enough).  Only when b1 is
+    // full does the id go to its second bucket b2 = (b1 + alt(t)) mod nb,
+    // alt(t) in [1, nb), as t | 0x8000: slots never empty during a query,
+    // so an id can be in b2 only if b1 is full, and (bucket, stored tag)
+    // still determines the id (b1 = b2 - alt(t) mod nb, then h, then the id,
+    // since the scramble is a bijection).  t = 0x7FFF has no second bucket
+    // (its secondary tag would read as an empty slot).
     __device__ __forceinline__ int probe(uint32_t id) const
     {
         const uint32_t h = scramble(id);
-        const uint32_t tag = h >> lbits;
-        const uint32_t b1 = h & bmask;
+        // h / nb by multiply-high with magic = floor(2^32 / nb): the
+        // estimate is low by at most one
+        uint32_t tag = __umulhi(h, magic);
+        uint32_t b1 = h - tag * nb;
+        if (b1 >= nb) {
+            b1 -= nb;
+            ++tag;
+        }
         const int r = probe_bucket(tbl + (b1 << 2), tag);
         if (r != 2 || tag == 0x7FFFu) return r;
-        const uint32_t b2 = (b1 ^ (((tag * 0x85EBCA77u) >> 15) | 1u)) & bmask;
+        uint32_t b2 = b1 + 1u + __umulhi(tag * 0x85EBCA77u, nb - 1u);
+        if (b2 >= nb) b2 -= nb;
         return probe_bucket(tbl + (b2 << 2), tag | 0x8000u);
     }
 };
@@ -678,8 +686,8 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
     const uint32_t stride = 1u << a.stride_log2;
     Visited vis;
     vis.tbl = tbl;
-    vis.bmask = (1u << a.tbl_lbits) - 1u;
-    vis.lbits = a.tbl_lbits;
+    vis.nb = a.tbl_buckets;
+    vis.magic = a.tbl_magic;
     vis.idbits = a.id_bits;
     vis.idmask = a.id_bits >= 32 ? ~0u : (1u << a.id_bits) - 1u;
     vis.xshift = a.id_bits / 2 + 1;

@@ -73,6 +73,8 @@ def knn_by_component(cfg, base, k, gpu_knn, check_rows=2000, verbose=True):
         start += counts[c]
         local = gpu_knn(np.ascontiguousarray(base[members]), k)
         ids[members] = members[local]
+        if verbose and (c + 1) % 32 == 0:
+            log(f"per-component {k}-NN: {c + 1}/{cfg['comps']} components")
     rows = np.linspace(0, n - 1, min(check_rows, n)).astype(np.int64)
     g_ids, _ = gpu_knn.truth(base, np.ascontiguousarray(base[rows]), k + 1)
     agree = 0

@@ -1,0 +1,71 @@
+"""A fixed sequence of search launches for hardware profilers.
+
+  python -m parallel_anns_amd.profile_run --op M,L[,eighths] [--learned-op M,Lmax,threshold[,eighths]]
+                                          [--config c1] [--steps 5]
+
+Loads the prepared workload from the cache (and the stop model bench.py
+saved there), then runs the fixed-L plan `steps + 1` times and, when given,
+the learned-stop plan `steps + 1` times, nothing else, so that the last
+`steps` dispatches of each search-kernel variant in a kernel trace or counter
+run are exactly the timed launches of the benchmark's two operating points.
+Prints one JSON line with the device times and the algorithmic bytes per
+launch.
+"""
+import argparse
+import json
+import os
+import sys
+
+import numpy as np
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="c1")
+    ap.add_argument("--cache-dir", default=os.environ.get("PANNS_CACHE", "/tmp/panns_cache"))
+    ap.add_argument("--op", required=True)
+    ap.add_argument("--learned-op", default=None)
+    ap.add_argument("--steps", type=int, default=5)
+    args = ap.parse_args()
+
+    from . import datasets
+    from .search import Index
+
+    ds = datasets.prepare(args.config, cache_dir=args.cache_dir, gpu_knn=datasets.GpuKnn(0), verbose=False)
+    K = ds["cfg"]["K"]
+    nq = ds["queries"].shape[0]
+    dim = ds["base"].shape[1]
+    idx = Index(0)
+    idx.set_base(ds["base"])
+    idx.set_queries(ds["queries"])
+    idx.set_graph(ds["adj"], ds["deg"], ds["ep"])
+    qids = np.arange(nq, dtype=np.uint32)
+
+    def run(M, L, eighths, **kw):
+        pl = idx.plan(qids, K, L, M, want_queue=False, options=Index.plan_options(table_eighths=eighths), **kw)
+        pl.run()
+        ms = [pl.run() for _ in range(args.steps)]
+        res = pl.fetch()
+        pl.close()
+        c = res["counters"]
+        alg = 4.0 * dim * c["distance_computations"] + 4.0 * (c["expansions"] + c["adjacency_slots"]) \
+            + nq * (4.0 * dim + 8.0 * K)
+        return {"M": M, "L": L, "table_eighths": eighths, "kernel_ms": round(float(np.mean(ms)), 4),
+                "algorithmic_bytes_per_launch": int(alg)}
+
+    out = {"config": args.config, "steps": args.steps}
+    f = [x for x in args.op.split(",")]
+    out["fixed"] = run(int(f[0]), int(f[1]), int(f[2]) if len(f) > 2 else 8)
+    if args.learned_op:
+        g = args.learned_op.split(",")
+        model = np.load(os.path.join(ds["dir"] or args.cache_dir, "stop_model.npy"))
+        idx.set_stop_model(model)
+        out["learned"] = run(int(g[0]), int(g[1]), int(g[3]) if len(g) > 3 else 8, stop="learned",
+                             stop_threshold=float(g[2]))
+        out["learned"]["threshold"] = float(g[2])
+    print(json.dumps(out), flush=True)
+    idx.close()
+
+
+if __name__ == "__main__":
+    sys.exit(main())

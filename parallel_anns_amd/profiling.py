@@ -3,15 +3,22 @@
   python -m parallel_anns_amd.profiling trace   <results.db> [kernel-substring] [--last N]
   python -m parallel_anns_amd.profiling counter <results.db> <COUNTER> [kernel-substring] [--last N]
   python -m parallel_anns_amd.profiling scale   <copy_fetch.db> <copy_write.db> <bytes>
+  python -m parallel_anns_amd.profiling report  --trace T.db --fetch F.db --write W.db --cal-fetch CF.db
+                                                --cal-write CW.db --copy-bytes B --run run.json
+                                                --command "..." --md out.md --records out.json
 
 `trace` prints per-kernel call counts and mean/min/max durations (all
 dispatches, and the last N dispatches of the selected kernel, e.g. the timed
 steps of a benchmark); `counter` prints a hardware counter per dispatch;
 `scale` turns the FETCH_SIZE / WRITE_SIZE (KiB) readings of a copy kernel of
-known size into bytes-per-counted-KiB factors.
+known size into bytes-per-counted-KiB factors.  `report` combines a kernel
+trace and the two counter passes of a profile_run command (the last `steps`
+dispatches of each search-kernel variant) with the copy-kernel factors into a
+markdown summary and the HBM-traffic records bench.py reports.
 """
 import argparse
 import sqlite3
+import sys
 
 
 def kernel_durations(db, match=None):
@@ -31,7 +38,67 @@ def counter_values(db, counter, match=None):
     return rows
 
 
+def report(a):
+    import json
+    run = json.loads(open(a.run).read().strip().splitlines()[-1])
+    steps = run["steps"]
+    cf = [r[2] for r in counter_values(a.cal_fetch, "FETCH_SIZE", "copy_kernel")]
+    cw = [r[2] for r in counter_values(a.cal_write, "WRITE_SIZE", "copy_kernel")]
+    ff = a.copy_bytes / (sum(cf) / len(cf) * 1024)
+    wf = a.copy_bytes / (sum(cw) / len(cw) * 1024)
+    lines = ["# " + a.title, "", "Command (each pass):", "", "```", a.command, "```", "",
+             f"Copy-kernel calibration ({a.copy_bytes} bytes copied): FETCH_SIZE factor {ff:.4f}, "
+             f"WRITE_SIZE factor {wf:.4f} bytes per counted byte.", "",
+             "| variant | M | L | table 8ths | trace mean us (last %d) | run ms | FETCH KiB | WRITE KiB | "
+             "traffic GB | algorithmic GB | traffic / alg. | achieved GB/s |" % steps,
+             "|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    records = []
+    dim = a.dim
+    for name, variant in (("fixed", "false"), ("learned", "true")):
+        if name not in run:
+            continue
+        r = run[name]
+        match = f"search_kernel<{dim}, {variant}>"
+        d = [x[1] / 1e3 for x in kernel_durations(a.trace, match)][-steps:]
+        fv = [x[2] for x in counter_values(a.fetch, "FETCH_SIZE", match)][-steps:]
+        wv = [x[2] for x in counter_values(a.write, "WRITE_SIZE", match)][-steps:]
+        mean_us = sum(d) / len(d)
+        fk, wk = sum(fv) / len(fv), sum(wv) / len(wv)
+        traffic = fk * 1024 * ff + wk * 1024 * wf
+        alg = r["algorithmic_bytes_per_launch"]
+        lines.append(f"| {match} | {r['M']} | {r['L']} | {r['table_eighths']} | {mean_us:.1f} | {r['kernel_ms']} | "
+                     f"{fk:,.0f} | {wk:,.0f} | {traffic / 1e9:.3f} | {alg / 1e9:.3f} | {traffic / alg:.3f} | "
+                     f"{alg / (mean_us * 1e-6) / 1e9:,.1f} |")
+        rec = {"config": run["config"], "kernel": f"search_kernel<{dim}, {variant}>", "M": r["M"], "L": r["L"],
+               "traffic_bytes": int(traffic), "fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
+               "fetch_factor": round(ff, 4), "write_factor": round(wf, 4), "trace_mean_us": round(mean_us, 1),
+               "command": a.command}
+        if "threshold" in r:
+            rec["threshold"] = r["threshold"]
+        records.append(rec)
+    lines += ["", "Kernel trace, all kernels of the traced run:", "", "```"]
+    rows = kernel_durations(a.trace)
+    for n in sorted({x[0] for x in rows}):
+        dd = [x[1] / 1e3 for x in rows if x[0] == n]
+        lines.append(f"{n[:60]:60s} calls {len(dd):4d} mean {sum(dd) / len(dd):10.1f} us "
+                     f"min {min(dd):10.1f} max {max(dd):10.1f}")
+    lines += ["```", ""]
+    open(a.md, "w").write("\n".join(lines))
+    json.dump({"records": records}, open(a.records, "w"), indent=1)
+    print("\n".join(lines))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "report":
+        ap = argparse.ArgumentParser()
+        ap.add_argument("mode")
+        for k in ("--trace", "--fetch", "--write", "--cal-fetch", "--cal-write", "--run", "--command", "--md",
+                  "--records"):
+            ap.add_argument(k, required=True)
+        ap.add_argument("--copy-bytes", type=int, required=True)
+        ap.add_argument("--dim", type=int, default=96)
+        ap.add_argument("--title", default="search_kernel, 1M x 96, 10k queries")
+        return report(ap.parse_args())
     ap = argparse.ArgumentParser()
     ap.add_argument("mode", choices=["trace", "counter", "scale"])
     ap.add_argument("args", nargs="+")

@@ -198,13 +198,13 @@ class Index:
         return qids, Lq, Kq, maxL, maxK
 
     @staticmethod
-    def plan_options(buckets=1, bucket_max_L=None, table_shift=0, use_graph=False, concurrent=False,
+    def plan_options(buckets=1, bucket_max_L=None, table_eighths=8, use_graph=False, concurrent=False,
                      want_queue=False):
         o = _native.PannsPlanOptions()
         o.n_buckets = len(bucket_max_L) if bucket_max_L else buckets
         for i, v in enumerate(bucket_max_L or ()):
             o.bucket_max_L[i] = int(v)
-        o.table_shift, o.use_graph, o.concurrent = table_shift, int(use_graph), int(concurrent)
+        o.table_eighths, o.use_graph, o.concurrent = table_eighths, int(use_graph), int(concurrent)
         o.want_queue = int(want_queue)
         return o
 

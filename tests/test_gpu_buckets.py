@@ -45,7 +45,7 @@ def test_buckets_equal_single_plan_and_oracle(small_set, gpu_index, uneven):
         assert (ref["ids"][i, Ks[i]:] == 0xFFFFFFFF).all()
         assert ref["ndist"][i] == o["ndist"]
     for opts in (dict(buckets=3), dict(buckets=3, use_graph=True), dict(buckets=4, concurrent=True),
-                 dict(buckets=2, use_graph=True, concurrent=True, table_shift=1),
+                 dict(buckets=2, use_graph=True, concurrent=True, table_eighths=5),
                  dict(bucket_max_L=[150, 300, 600], use_graph=True)):
         print("case buckets", opts, flush=True)
         pl = gpu_index.plan(qids, 100, int(Ls.max()), M, init_ids=init, L_per_query=Ls, K_per_query=Ks,
