@@ -813,7 +813,34 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                                        (sc & (stride - 1u))];
                         okbits |= ok ? 1u << u : 0u;
                     }
-                    nnew = claim_chunks(vis, ids, okbits, nch, newids, nnew, lt, nadj);
+                    uint32_t nch_c = nch;
+                    if (nch > 2) {
+                        // compact the real neighbour ids (rows are padded to
+                        // the stride, mostly with empty slots) through the
+                        // idle candidate buffer, so the probes run on fewer,
+                        // full chunks
+                        uint32_t *stage = reinterpret_cast<uint32_t *>(cand);
+                        uint32_t nvalid = 0;
+#pragma unroll
+                        for (int u = 0; u < kClaimChunks; ++u) {
+                            const bool v = ((okbits >> u) & 1u) && ids[u] != kSent;
+                            const uint64_t vm = __ballot(v);
+                            if (v) stage[nvalid + (uint32_t)__popcll(vm & lt)] = ids[u];
+                            nvalid += (uint32_t)__popcll(vm);
+                        }
+                        __syncthreads();
+                        nch_c = (nvalid + 63u) / 64u;
+                        okbits = 0;
+#pragma unroll
+                        for (int u = 0; u < kClaimChunks; ++u) {
+                            const uint32_t i = (uint32_t)u * 64 + lane;
+                            const bool ok = i < nvalid;
+                            ids[u] = stage[ok ? i : 0u];
+                            okbits |= ok ? 1u << u : 0u;
+                        }
+                        __syncthreads();
+                    }
+                    nnew = claim_chunks(vis, ids, okbits, nch_c, newids, nnew, lt, nadj);
                 }
                 __syncthreads();
                 PANNS_PT(kPhClaim);
