@@ -331,6 +331,14 @@ def main():
                                 "learned_over_fixed": round(r["qps"] / fixed_other[t]["qps"], 3)}
                 log(f"target {t}: {lrn_other[t]}")
 
+    # the one-call host boundary (panns_search: ids up, plan, launch, results
+    # down), for reference beside the resident-batch rate that `value` is
+    t0 = time.perf_counter()
+    index.search(qids, K, fL, fM)
+    host_wall = time.perf_counter() - t0
+    host_inclusive = {"qps": round(nq / host_wall, 1), "ms": round(host_wall * 1e3, 3),
+                      "what": "fixed L, one panns_search call: host ids in, plan, launch, host results out"}
+    log(f"host-inclusive: {host_inclusive}")
     main_res, main_qps, main_recall = (lres, lrn["qps"], lrn["recall"]) if lrn else (fres, fixed["qps"],
                                                                                    fixed["recall"])
     main_ms = lrn["kernel_ms"] if lrn else fixed["kernel_ms"]
@@ -367,6 +375,7 @@ def main():
                    "fixed_L": fixed, "learned": lrn,
                    "learned_over_fixed": round(lrn["qps"] / fixed["qps"], 3) if lrn else None,
                    "per_query_stop_bound": oracle,
+                   "host_inclusive": host_inclusive,
                    "other_targets": lrn_other or ({str(t): {"fixed_L": v} for t, v in fixed_other.items()}
                                                   if fixed_other else None),
                    "workgroups": (ldesc or fdesc).get("workgroups"),
