@@ -165,7 +165,8 @@ def timed(index, qids, K, L, M, gt, steps, warmup, **kw):
     return res, desc, {"M": M, "L": L, "qps": round(steps * qids.shape[0] / wall, 1),
                        "recall": round(tools.recall(gt, res["ids"], L), 4),
                        "kernel_ms": round(float(np.mean(kms)), 4),
-                       "ndist": round(float(res["ndist"].mean()), 1), "table_eighths": int(opt.table_eighths)}
+                       "ndist": round(float(res["ndist"].mean()), 1), "table_eighths": int(opt.table_eighths),
+                       "waves": int(max(1, opt.waves))}
 
 
 def deficit_distribution(d):

@@ -57,7 +57,12 @@ typedef struct {
     float stop_threshold;  /* LEARNED: stop when sigmoid output >= threshold */
     uint32_t hash_slots;   /* 0 = automatic; else a multiple of 64 in [64, 32768]: 16-bit slots of
                               the per-query LDS visited table (raised when the id space needs) */
-    uint32_t reserved[4];
+    uint32_t waves;        /* wavefronts per query: 0 or 1, else 2 or 4 (TOP_M and SCALE_M with the
+                              fixed stop): the picks of an iteration are expanded by several waves
+                              and merged into the master queue, the deterministic form of the
+                              reference's multi-thread merge (PSS5LG.cpp:811-842, 1543-1673);
+                              results equal the single-wave search */
+    uint32_t reserved[3];
 } panns_params;
 
 typedef struct {
@@ -175,7 +180,8 @@ typedef struct {
     uint32_t use_graph;                        /* capture and replay as one hipGraph */
     uint32_t concurrent;                       /* buckets side by side */
     uint32_t want_queue;                       /* keep the final queues for panns_plan_fetch */
-    uint32_t reserved[4];
+    uint32_t waves;                            /* 0: panns_params.waves; else wavefronts per query */
+    uint32_t reserved[3];
 } panns_plan_options;
 
 int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *query_ids,
@@ -188,7 +194,8 @@ int panns_plan_describe_buckets(panns_plan *plan, uint32_t *n_buckets, uint32_t 
                                 uint32_t *workgroups, uint32_t *lds_bytes, uint32_t *hash_slots);
 /* The autotuner: times candidate bucketings (1, 2, 3, 4, 6 buckets of equal
  * summed L; side by side or not; visited tables at 8/8, 7/8, 6/8 or 5/8 of
- * the automatic size; always as a graph) on every `sample_stride`-th query of the batch (0: a stride that
+ * the automatic size; 1, 2 or 4 waves per query where the search allows
+ * it; always as a graph) on every `sample_stride`-th query of the batch (0: a stride that
  * leaves about 20,000 queries) and returns the fastest in *best, with its
  * bucket bounds filled in, and its device time on the sample in *best_ms
  * (nullable).  Results do not depend on the choice. */

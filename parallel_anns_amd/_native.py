@@ -28,7 +28,7 @@ class NativeLibraryMissing(RuntimeError):
 class PannsParams(ctypes.Structure):
     _fields_ = [("K", c_u32), ("L", c_u32), ("M", c_u32), ("mode", c_u32),
                 ("num_threads", c_u32), ("stop", c_u32), ("stop_threshold", c_float),
-                ("hash_slots", c_u32), ("reserved", c_u32 * 4)]
+                ("hash_slots", c_u32), ("waves", c_u32), ("reserved", c_u32 * 3)]
 
 
 class PannsCounters(ctypes.Structure):
@@ -46,7 +46,8 @@ MAX_BUCKETS = 8
 
 class PannsPlanOptions(ctypes.Structure):
     _fields_ = [("n_buckets", c_u32), ("bucket_max_L", c_u32 * MAX_BUCKETS), ("table_eighths", c_u32),
-                ("use_graph", c_u32), ("concurrent", c_u32), ("want_queue", c_u32), ("reserved", c_u32 * 4)]
+                ("use_graph", c_u32), ("concurrent", c_u32), ("want_queue", c_u32), ("waves", c_u32),
+                ("reserved", c_u32 * 3)]
 
 
 # name -> (restype, argtypes) for include/panns_tools.h

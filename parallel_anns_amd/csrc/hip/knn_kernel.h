@@ -22,58 +22,7 @@ constexpr int kKnnTPQ = 8;    // lanes per query
 constexpr int kKnnRT = 64;    // rows per tile
 constexpr int kKnnKCap = 128; // max k
 
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-}
 
-// merge_batch for a wave that shares its workgroup with waves doing other
-// work: the same algorithm, synchronised within the wave only.
-__device__ __forceinline__ void merge_batch_wave(uint64_t *Q, uint32_t &S, uint32_t L, uint64_t x,
-                                                 uint32_t nb, uint64_t *bb, int lane)
-{
-    uint32_t pos = kSent;
-    bool valid = false;
-    if ((uint32_t)lane < nb) {
-        pos = lb_queue(Q, S, x) + (uint32_t)lane;
-        valid = pos < L;
-    }
-    const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
-    const uint32_t p0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)pos);
-    if (nv == 0) return;
-    if (valid) bb[lane] = x;
-    wave_sync();
-    uint32_t hi = S;
-    while (hi > p0) {
-        const uint32_t lo = (hi - p0 > 64u) ? hi - 64u : p0;
-        const uint32_t j = lo + (uint32_t)lane;
-        uint64_t e = 0;
-        uint32_t np = kSent;
-        if (j < hi) {
-            e = Q[j];
-            uint32_t c = 0, cnt = nv;
-            while (cnt > 0) {
-                const uint32_t step = cnt >> 1;
-                const uint32_t mid = c + step;
-                if (bb[mid] < e) {
-                    c = mid + 1;
-                    cnt -= step + 1;
-                } else {
-                    cnt = step;
-                }
-            }
-            np = j + c;
-        }
-        wave_sync();
-        if (np < L) Q[np] = e;
-        wave_sync();
-        hi = lo;
-    }
-    if (valid) Q[pos] = x;
-    S = min(L, S + nv);
-    wave_sync();
-}
 
 template <int D>
 __global__ __launch_bounds__(256) void knn_kernel(const float *__restrict__ base, uint32_t n,
@@ -175,7 +124,7 @@ __global__ __launch_bounds__(256) void knn_kernel(const float *__restrict__ base
             uint64_t x = (uint32_t)lane < cnt ? cbuf[qq * 64 + lane] : kKeyMax;
             x = sort64(x, lane, xs);
             uint32_t S = lsz[qq];
-            merge_batch_wave(lists + qq * kKnnKCap, S, k, x, cnt, bbs + wave * 64, lane);
+            merge_batch<true>(lists + qq * kKnnKCap, S, k, x, cnt, bbs + wave * 64, lane);
             if (lane == 0) {
                 lsz[qq] = S;
                 thr[qq] = (S == k) ? lists[qq * kKnnKCap + k - 1] : kKeyMax;

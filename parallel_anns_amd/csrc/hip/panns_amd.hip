@@ -14,6 +14,7 @@
 #include "panns_amd.h"
 #include "panns_tools.h"
 #include "search_kernel.h"
+#include "search_kernel_mw.h"
 #include "knn_kernel.h"
 #include "mlp_kernels.h"
 
@@ -115,6 +116,7 @@ struct panns_plan {
     uint32_t *shared_bitmap = nullptr;
     uint32_t grid = 0, lds = 0, hash_slots = 0;
     uint32_t table_eighths = 8;
+    uint32_t waves = 1;  // wavefronts per query
     // scratch placement: bitmap/log slots [slot0, slot0 + grid) and the work
     // counter index, bound to the handle's buffers before every run
     uint32_t slot0 = 0, counter_idx = 0;
@@ -152,6 +154,13 @@ hipError_t launch_search(const panns::KArgs &a, uint32_t grid, uint32_t lds, hip
     return hipGetLastError();
 }
 
+template <int D, int W>
+hipError_t launch_search_mw(const panns::KArgs &a, uint32_t grid, uint32_t lds, hipStream_t s)
+{
+    hipLaunchKernelGGL((panns::search_kernel_mw<D, W>), dim3(grid), dim3(64 * W), lds, s, a);
+    return hipGetLastError();
+}
+
 using LaunchFn = hipError_t (*)(const panns::KArgs &, uint32_t, uint32_t, hipStream_t);
 using KernelPtr = const void *;
 
@@ -178,6 +187,33 @@ bool kernel_for_dim(uint32_t dim, bool learned, LaunchFn *fn, KernelPtr *kp)
     }
 }
 
+// The multi-wave kernel for a dimension and W waves per query (2 or 4).
+bool kernel_for_dim_mw(uint32_t dim, uint32_t waves, LaunchFn *fn, KernelPtr *kp)
+{
+    switch (dim) {
+#define PANNS_DIM(DD)                                                                          \
+    case DD:                                                                                   \
+        *fn = waves == 2 ? launch_search_mw<DD, 2> : launch_search_mw<DD, 4>;                   \
+        *kp = waves == 2 ? reinterpret_cast<const void *>(&panns::search_kernel_mw<DD, 2>)      \
+                         : reinterpret_cast<const void *>(&panns::search_kernel_mw<DD, 4>);     \
+        return true;
+        PANNS_DIM(16)
+        PANNS_DIM(32)
+        PANNS_DIM(40)
+        PANNS_DIM(64)
+        PANNS_DIM(96)
+        PANNS_DIM(128)
+#undef PANNS_DIM
+    default:
+        return false;
+    }
+}
+
+bool select_kernel(uint32_t dim, bool learned, uint32_t waves, LaunchFn *fn, KernelPtr *kp)
+{
+    return waves > 1 ? kernel_for_dim_mw(dim, waves, fn, kp) : kernel_for_dim(dim, learned, fn, kp);
+}
+
 int check_ready(panns_index *h)
 {
     if (!h) return fail(PANNS_ERR_ARG, "null handle");
@@ -202,6 +238,11 @@ int check_params(panns_index *h, const panns_params *p, uint32_t L, uint32_t K)
     if (p->stop == PANNS_STOP_LEARNED && !h->has_model) return fail(PANNS_ERR_STATE, "no stop model loaded");
     if (p->hash_slots && (p->hash_slots % 64 || p->hash_slots < 64 || p->hash_slots > 32768))
         return fail(PANNS_ERR_ARG, "hash_slots must be a multiple of 64 in [64, 32768]");
+    if (p->waves > 1) {
+        if (p->waves != 2 && p->waves != 4) return fail(PANNS_ERR_ARG, "waves must be 1, 2 or 4");
+        if (p->mode == PANNS_MODE_PSS_V5) return fail(PANNS_ERR_UNSUPPORTED, "PSS_V5 expands one candidate per step; use waves = 1");
+        if (p->stop != PANNS_STOP_FIXED) return fail(PANNS_ERR_UNSUPPORTED, "the learned stop runs with waves = 1");
+    }
     return PANNS_OK;
 }
 
@@ -212,6 +253,14 @@ int check_params(panns_index *h, const panns_params *p, uint32_t L, uint32_t K)
 uint32_t lds_bytes(uint32_t Lcap, uint32_t nb_cap, uint32_t tbl_slots, uint32_t Mcap, uint32_t dim)
 {
     return 8u * (Lcap + nb_cap + 64u) + 4u * dim + 2u * tbl_slots + 4u * (nb_cap + Mcap + 4u) + 4u * 64u;
+}
+
+// search_kernel_mw's carving: Q[Lcap] cand[W][nb_cap] bb[64] (8 B each),
+// query row, visited table, newids[W][nb_cap] picks[Mcap] ctrl (4 B each).
+uint32_t lds_bytes_mw(uint32_t Lcap, uint32_t nb_cap, uint32_t tbl_slots, uint32_t Mcap, uint32_t dim, uint32_t W)
+{
+    return 8u * (Lcap + W * nb_cap + 64u) + 4u * dim + 2u * tbl_slots + 4u * (W * nb_cap + Mcap) +
+           4u * panns::kCtrlWords;
 }
 
 int ensure_scratch(panns_index *h, uint32_t grid)
@@ -236,9 +285,12 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
 {
     panns_index *h = pl->h;
     const panns_params &p = pl->p;
+    pl->waves = std::max<uint32_t>(1u, p.waves);
+    if (pl->waves > 1 && (pl->init_queue.p || pl->trace_cap || pl->shared_bitmap))
+        return fail(PANNS_ERR_UNSUPPORTED, "traces and caller-held queues run with waves = 1");
     LaunchFn fn;
     KernelPtr kp;
-    if (!kernel_for_dim(h->dim, p.stop == PANNS_STOP_LEARNED, &fn, &kp))
+    if (!select_kernel(h->dim, p.stop == PANNS_STOP_LEARNED, pl->waves, &fn, &kp))
         return fail(PANNS_ERR_DIM, "dimension " + std::to_string(h->dim) + " has no kernel (16, 32, 40, 64, 96, 128)");
     pl->maxL = p.L;
     pl->maxK = p.K;
@@ -263,7 +315,10 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
             if (init_ids[i] >= h->n) return fail(PANNS_ERR_RANGE, "init id out of range");
 
     const uint32_t stride = 1u << h->stride_log2;
-    const uint32_t nb_cap = std::max<uint32_t>(panns::kGroupSlots, stride);
+    // multi-wave: per-wave capacity for its share of the picks' neighbours
+    const uint32_t W = pl->waves;
+    const uint32_t nb_cap = W > 1 ? std::max<uint32_t>(panns::kGroupSlots, (p.M + W - 1) / W * stride)
+                                  : std::max<uint32_t>(panns::kGroupSlots, stride);
     const uint32_t Mcap = std::max<uint32_t>(4u, (p.M + 3u) & ~3u);
     // visited table: hs 16-bit slots in nb buckets of 8; the tag of an id
     // is its scrambled value divided by nb and must stay below 2^15 (0xFFFF
@@ -282,12 +337,12 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     pl->hash_slots = hs;
     // Lcap rounded to 8 keys keeps every LDS region 16-byte aligned.
     const uint32_t Lcap = (pl->maxL + 7u) & ~7u;
-    pl->lds = lds_bytes(Lcap, nb_cap, hs, Mcap, h->dim);
+    pl->lds = W > 1 ? lds_bytes_mw(Lcap, nb_cap, hs, Mcap, h->dim, W) : lds_bytes(Lcap, nb_cap, hs, Mcap, h->dim);
     if (pl->lds > 65536) {
         HIP_TRY(hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->lds));
     }
     int per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, 64, pl->lds));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, 64 * W, pl->lds));
     if (per_cu <= 0) return fail(PANNS_ERR_UNSUPPORTED, "kernel does not fit on a CU with this L");
     pl->grid = std::min<uint32_t>(pl->nq, (uint32_t)per_cu * (uint32_t)h->num_cu);
     if (pl->grid == 0) pl->grid = 1;
@@ -404,7 +459,7 @@ hipError_t launch_one(const panns_plan *pl, hipStream_t st)
 {
     LaunchFn fn;
     KernelPtr kp;
-    kernel_for_dim(pl->h->dim, pl->args.stop_learned != 0, &fn, &kp);
+    select_kernel(pl->h->dim, pl->args.stop_learned != 0, pl->waves, &fn, &kp);
     return fn(pl->args, pl->grid, pl->lds, st);
 }
 
@@ -814,6 +869,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
     pl->h = h;
     pl->p = *p;
     pl->nq = nq;
+    if (o.waves) pl->p.waves = o.waves;
     pl->opt = o;
     pl->want_queue = o.want_queue != 0;
     pl->table_eighths = o.table_eighths;
@@ -830,7 +886,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
     for (uint32_t i = 0; i < nq; ++i) {
         Lall[i] = L_per_query ? L_per_query[i] : p->L;
         Kall[i] = K_per_query ? K_per_query[i] : p->K;
-        rc = check_params(h, p, Lall[i], Kall[i]);
+        rc = check_params(h, &pl->p, Lall[i], Kall[i]);
         if (rc) return rc;
         pl->maxL = std::max(pl->maxL, Lall[i]);
         pl->maxK = std::max(pl->maxK, Kall[i]);
@@ -877,7 +933,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
         if (m.empty()) continue;
         std::unique_ptr<panns_plan> part(new panns_plan());
         part->h = h;
-        part->p = *p;
+        part->p = pl->p;
         part->nq = (uint32_t)m.size();
         part->want_queue = pl->want_queue;
         part->table_eighths = o.table_eighths;
@@ -952,9 +1008,36 @@ int panns_plan_autotune(panns_index *h, const panns_params *p, const uint32_t *q
     double total = 0;
     for (uint32_t v : sorted) total += v;
     float best_t = std::numeric_limits<float>::infinity();
+    // one candidate: build on the sample, one warm run, best of three
+    auto trial = [&](const panns_plan_options &o, float *t) -> int {
+        panns_plan *pl = nullptr;
+        int rc = panns_plan_create_ex(h, p, sq.data(), (uint32_t)sq.size(), L_per_query ? sl.data() : nullptr,
+                                      K_per_query ? sk.data() : nullptr, init_ids, n_init, &o, &pl);
+        if (rc) return rc;
+        float ms = 0.f;
+        *t = std::numeric_limits<float>::infinity();
+        rc = run_plan(pl, 1, nullptr);
+        for (int r = 0; r < 3 && !rc; ++r) {
+            rc = run_plan(pl, 1, &ms);
+            *t = std::min(*t, ms);
+        }
+        panns_plan_destroy(pl);
+        return rc;
+    };
+    auto consider = [&](const panns_plan_options &o) -> int {
+        float t = 0.f;
+        int rc = trial(o, &t);
+        if (rc) return rc;
+        if (t < best_t) {
+            best_t = t;
+            *best = o;
+        }
+        return PANNS_OK;
+    };
+    // stage 1: buckets of equal summed L over the whole batch (explicit
+    // bounds, so they carry over from the sample), in sequence or side by side
     static const uint32_t kCounts[] = {1, 2, 3, 4, 6};
     for (uint32_t nb : kCounts) {
-        // equal summed L over the whole batch, as explicit bounds
         panns_plan_options o{};
         uint32_t k = 0;
         double acc = 0;
@@ -970,27 +1053,31 @@ int panns_plan_autotune(panns_index *h, const panns_params *p, const uint32_t *q
         if (nb > 1 && k < 2) continue;
         o.n_buckets = k;
         o.use_graph = 1;
-        for (uint32_t conc = 0; conc < (k > 1 ? 2u : 1u); ++conc)
-            for (uint32_t eighths : {8u, 7u, 6u, 5u}) {
-                o.concurrent = conc;
-                o.table_eighths = eighths;
-                panns_plan *pl = nullptr;
-                int rc = panns_plan_create_ex(h, p, sq.data(), (uint32_t)sq.size(), L_per_query ? sl.data() : nullptr,
-                                              K_per_query ? sk.data() : nullptr, init_ids, n_init, &o, &pl);
-                if (rc) return rc;
-                float t = std::numeric_limits<float>::infinity(), ms = 0.f;
-                rc = run_plan(pl, 1, nullptr);
-                for (int r = 0; r < 3 && !rc; ++r) {
-                    rc = run_plan(pl, 1, &ms);
-                    t = std::min(t, ms);
-                }
-                panns_plan_destroy(pl);
-                if (rc) return rc;
-                if (t < best_t) {
-                    best_t = t;
-                    *best = o;
-                }
-            }
+        o.table_eighths = 8;
+        o.waves = 1;
+        for (uint32_t conc = 0; conc < (k > 1 ? 2u : 1u); ++conc) {
+            o.concurrent = conc;
+            int rc = consider(o);
+            if (rc) return rc;
+        }
+    }
+    // stage 2: smaller visited tables (more workgroups per CU, more spills)
+    const panns_plan_options s1 = *best;
+    for (uint32_t eighths : {7u, 6u, 5u}) {
+        panns_plan_options o = s1;
+        o.table_eighths = eighths;
+        int rc = consider(o);
+        if (rc) return rc;
+    }
+    // stage 3: several waves per query, where the search allows it
+    if (p->mode != PANNS_MODE_PSS_V5 && p->stop == PANNS_STOP_FIXED) {
+        const panns_plan_options s2 = *best;
+        for (uint32_t waves : {2u, 4u}) {
+            panns_plan_options o = s2;
+            o.waves = waves;
+            int rc = consider(o);
+            if (rc) return rc;
+        }
     }
     if (best_ms) *best_ms = best_t;
     return PANNS_OK;

@@ -341,6 +341,25 @@ __device__ __forceinline__ uint32_t lb_queue(const uint64_t *Q, uint32_t S, uint
 // place, processed from the right end so nothing is overwritten before it is
 // read.  Returns the queue position of the smallest inserted key, or kSent;
 // adds the number of inserted keys to *ninserted when given.
+// Synchronises one wave's LDS accesses when it shares its workgroup with
+// waves doing other work (a workgroup barrier would wait for them).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <bool kWave>
+__device__ __forceinline__ void merge_sync()
+{
+    if constexpr (kWave) {
+        wave_sync();
+    } else {
+        __syncthreads();
+    }
+}
+
+template <bool kWave = false>
 __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32_t L, uint64_t x,
                                                 uint32_t nb, uint64_t *bb, int lane,
                                                 uint32_t *ninserted = nullptr)
@@ -356,7 +375,7 @@ __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32
     if (nv == 0) return kSent;
     if (ninserted) *ninserted += nv;
     if (valid) bb[lane] = x;
-    __syncthreads();
+    merge_sync<kWave>();
     uint32_t hi = S;
     while (hi > p0) {
         const uint32_t lo = (hi - p0 > 64u) ? hi - 64u : p0;
@@ -379,14 +398,14 @@ __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32
             }
             np = j + c;
         }
-        __syncthreads();
+        merge_sync<kWave>();
         if (np < L) Q[np] = e;
-        __syncthreads();
+        merge_sync<kWave>();
         hi = lo;
     }
     if (valid) Q[pos] = x;
     S = min(L, S + nv);
-    __syncthreads();
+    merge_sync<kWave>();
     return p0;
 }
 

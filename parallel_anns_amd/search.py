@@ -27,7 +27,7 @@ def _chk(rc):
 
 
 def make_params(K, L, M=1, mode="top_m", num_threads=1, stop="fixed", stop_threshold=0.5,
-                hash_slots=0):
+                hash_slots=0, waves=1):
     p = PannsParams()
     p.K, p.L, p.M = K, L, M
     p.mode = MODES[mode] if isinstance(mode, str) else int(mode)
@@ -35,6 +35,7 @@ def make_params(K, L, M=1, mode="top_m", num_threads=1, stop="fixed", stop_thres
     p.stop = STOPS[stop] if isinstance(stop, str) else int(stop)
     p.stop_threshold = stop_threshold
     p.hash_slots = hash_slots
+    p.waves = waves
     return p
 
 
@@ -164,7 +165,7 @@ class Index:
         return out
 
     def search(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, want_queue=False,
-               num_threads=1, stop="fixed", stop_threshold=0.5, hash_slots=0):
+               num_threads=1, stop="fixed", stop_threshold=0.5, hash_slots=0, waves=1):
         """Batch form of para_search_with_top_m_pure_large_graph /
         para_search_PSS_v5_large_graph_dist_thresh_profiling (num_threads_=1)."""
         qids = np.ascontiguousarray(np.atleast_1d(query_ids), dtype=np.uint32)
@@ -172,7 +173,7 @@ class Index:
         if init_ids is None:
             init_ids = self.prepare_init_ids(L)
         init_ids = np.ascontiguousarray(init_ids, dtype=np.uint32)
-        p = make_params(K, L, M, mode, num_threads, stop, stop_threshold, hash_slots)
+        p = make_params(K, L, M, mode, num_threads, stop, stop_threshold, hash_slots, waves)
         res = {"ids": np.empty((nq, K), np.uint32), "dists": np.empty((nq, K), np.float32),
                "ndist": np.empty(nq, np.uint32), "nexp": np.empty(nq, np.uint32),
                "niter": np.empty(nq, np.uint32)}
@@ -199,18 +200,19 @@ class Index:
 
     @staticmethod
     def plan_options(buckets=1, bucket_max_L=None, table_eighths=8, use_graph=False, concurrent=False,
-                     want_queue=False):
+                     want_queue=False, waves=0):
         o = _native.PannsPlanOptions()
         o.n_buckets = len(bucket_max_L) if bucket_max_L else buckets
         for i, v in enumerate(bucket_max_L or ()):
             o.bucket_max_L[i] = int(v)
         o.table_eighths, o.use_graph, o.concurrent = table_eighths, int(use_graph), int(concurrent)
         o.want_queue = int(want_queue)
+        o.waves = waves
         return o
 
     def plan(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, L_per_query=None,
              K_per_query=None, stop="fixed", stop_threshold=0.5, hash_slots=0, options=None,
-             want_queue=True):
+             want_queue=True, waves=1):
         """A resident batch (panns_plan_create_ex).  options: a
         PannsPlanOptions (plan_options() or autotune()) for queue-size
         buckets, graph replay and side-by-side buckets; want_queue keeps the
@@ -219,7 +221,7 @@ class Index:
         if init_ids is None:
             init_ids = self.prepare_init_ids(maxL)
         init_ids = np.ascontiguousarray(init_ids, dtype=np.uint32)
-        p = make_params(K, L, M, mode, 1, stop, stop_threshold, hash_slots)
+        p = make_params(K, L, M, mode, 1, stop, stop_threshold, hash_slots, waves)
         o = _native.PannsPlanOptions() if options is None else options
         o.want_queue = int(want_queue)
         h = _native.vp()
