@@ -193,9 +193,9 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
 int panns_plan_describe_buckets(panns_plan *plan, uint32_t *n_buckets, uint32_t *max_L, uint32_t *nq,
                                 uint32_t *workgroups, uint32_t *lds_bytes, uint32_t *hash_slots);
 /* The autotuner: times candidate bucketings (1, 2, 3, 4, 6 buckets of equal
- * summed L; side by side or not; visited tables at 8/8, 7/8, 6/8 or 5/8 of
- * the automatic size; 1, 2 or 4 waves per query where the search allows
- * it; always as a graph) on every `sample_stride`-th query of the batch (0: a stride that
+ * summed L; side by side or not; visited tables at 8/8 down to 4/8 of the
+ * automatic size; 1, 2 or 4 waves per query where the search allows it;
+ * always as a graph) on every `sample_stride`-th query of the batch (0: a stride that
  * leaves about 20,000 queries) and returns the fastest in *best, with its
  * bucket bounds filled in, and its device time on the sample in *best_ms
  * (nullable).  Results do not depend on the choice. */

@@ -324,7 +324,9 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     // is its scrambled value divided by nb and must stay below 2^15 (0xFFFF
     // marks an empty slot, bit 15 a second-bucket entry), so nb grows with N
     uint32_t hs = p.hash_slots;
-    if (!hs) hs = ((pl->maxL <= 512 ? 8192u : 16384u) * pl->table_eighths / 8u + 63u) & ~63u;
+    // automatic size: 8,192 slots up to L = 512, 10,240 above (measured:
+    // larger tables lose more to occupancy than they save in spills)
+    if (!hs) hs = ((pl->maxL <= 512 ? 8192u : 10240u) * pl->table_eighths / 8u + 63u) & ~63u;
     const uint32_t id_bits = std::max<uint32_t>(1u, log2u(h->n));
     const uint64_t min_nb = (((1ull << id_bits) - 1ull) / 0x7FFEull) + 1ull;
     uint32_t nb = std::max<uint32_t>(hs / 8u, (uint32_t)((min_nb + 7ull) & ~7ull));
@@ -1063,7 +1065,7 @@ int panns_plan_autotune(panns_index *h, const panns_params *p, const uint32_t *q
     }
     // stage 2: smaller visited tables (more workgroups per CU, more spills)
     const panns_plan_options s1 = *best;
-    for (uint32_t eighths : {7u, 6u, 5u}) {
+    for (uint32_t eighths : {7u, 6u, 5u, 4u}) {
         panns_plan_options o = s1;
         o.table_eighths = eighths;
         int rc = consider(o);
