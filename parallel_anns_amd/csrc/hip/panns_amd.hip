@@ -74,6 +74,9 @@ uint32_t log2u(uint32_t x)
 constexpr uint32_t kMaxL = 4096;
 constexpr uint32_t kMaxM = 1024;
 constexpr uint32_t kLogCap = 16384;
+constexpr uint32_t kRanges = 8;         // work ranges (one per XCD)
+constexpr uint32_t kPivots = 256;       // locality pivots
+constexpr uint32_t kLocalityMin = 512;  // smallest batch that is reordered
 
 }  // namespace
 
@@ -99,6 +102,11 @@ struct panns_index {
     // learned stop model (device copy), 8-64-64-1
     DevBuf<float> model;
     bool has_model = false;
+    // locality order of batches: pivot rows and their position on a
+    // nearest-neighbour tour (computed once per base set)
+    DevBuf<float> pivots;
+    std::vector<uint32_t> tourpos;
+    bool has_pivots = false;
 };
 
 struct panns_plan {
@@ -117,6 +125,9 @@ struct panns_plan {
     uint32_t grid = 0, lds = 0, hash_slots = 0;
     uint32_t table_eighths = 8;
     uint32_t waves = 1;  // wavefronts per query
+    // work item i is the caller's query perm[i] (empty: the caller's order)
+    std::vector<uint32_t> perm;
+    bool locality = true;
     // scratch placement: bitmap/log slots [slot0, slot0 + grid) and the work
     // counter index, bound to the handle's buffers before every run
     uint32_t slot0 = 0, counter_idx = 0;
@@ -276,7 +287,67 @@ int ensure_scratch(panns_index *h, uint32_t grid)
         h->bitmap_slots = grid;
         h->bitmap_words = words;
     }
-    if (!h->counter.p) HIP_TRY(h->counter.ensure(PANNS_MAX_BUCKETS + 8));
+    if (!h->counter.p) HIP_TRY(h->counter.ensure((PANNS_MAX_BUCKETS + 1) * kRanges));
+    return PANNS_OK;
+}
+
+// Locality order of a batch: each query's nearest of kPivots evenly spaced
+// base rows (computed on the GPU), queries sorted by that pivot's position
+// on a greedy nearest-neighbour tour of the pivots.  Queries that run
+// together then touch the same part of the graph, and the kernel's
+// per-XCD work ranges keep them on one L2.  Results do not depend on it.
+int locality_order(panns_index *h, const uint32_t *query_ids, uint32_t nq, std::vector<uint32_t> &order)
+{
+    const uint32_t D = h->dim;
+    const uint32_t P = std::min<uint32_t>(kPivots, h->n);
+    if (!h->has_pivots) {
+        std::vector<float> piv((size_t)P * D);
+        for (uint32_t i = 0; i < P; ++i) {
+            const uint64_t row = (uint64_t)i * h->n / P;
+            HIP_TRY(hipMemcpy(piv.data() + (size_t)i * D, h->base.p + row * D, D * 4ull, hipMemcpyDeviceToHost));
+        }
+        HIP_TRY(h->pivots.ensure((size_t)P * D));
+        HIP_TRY(hipMemcpy(h->pivots.p, piv.data(), (size_t)P * D * 4, hipMemcpyHostToDevice));
+        // greedy tour from pivot 0
+        std::vector<char> used(P, 0);
+        h->tourpos.assign(P, 0);
+        uint32_t cur = 0;
+        used[0] = 1;
+        for (uint32_t t = 1; t < P; ++t) {
+            double best = 1e300;
+            uint32_t bi = 0;
+            for (uint32_t j = 0; j < P; ++j) {
+                if (used[j]) continue;
+                double d = 0;
+                for (uint32_t k = 0; k < D; ++k) {
+                    const double x = (double)piv[(size_t)cur * D + k] - (double)piv[(size_t)j * D + k];
+                    d += x * x;
+                }
+                if (d < best) {
+                    best = d;
+                    bi = j;
+                }
+            }
+            used[bi] = 1;
+            h->tourpos[bi] = t;
+            cur = bi;
+        }
+        h->has_pivots = true;
+    }
+    DevBuf<uint32_t> dq, dn;
+    HIP_TRY(dq.ensure(nq));
+    HIP_TRY(dn.ensure(nq));
+    HIP_TRY(hipMemcpy(dq.p, query_ids, nq * 4ull, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(panns::nearest_pivot_kernel, dim3((nq + 255) / 256), dim3(256), 0, h->stream, h->queries.p,
+                       dq.p, nq, h->pivots.p, P, D, dn.p);
+    HIP_TRY(hipGetLastError());
+    std::vector<uint32_t> near(nq);
+    HIP_TRY(hipMemcpyAsync(near.data(), dn.p, nq * 4ull, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    order.resize(nq);
+    for (uint32_t i = 0; i < nq; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return h->tourpos[near[x]] < h->tourpos[near[y]]; });
     return PANNS_OK;
 }
 
@@ -351,6 +422,26 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     rc = ensure_scratch(h, pl->slot0 + pl->grid);
     if (rc) return rc;
 
+    // work items in locality order (plain searches of batches big enough)
+    std::vector<uint32_t> oq, oL, oK;
+    pl->perm.clear();
+    if (pl->locality && pl->nq >= kLocalityMin && !pl->init_queue.p && !pl->trace_cap) {
+        rc = locality_order(h, query_ids, pl->nq, pl->perm);
+        if (rc) return rc;
+        oq.resize(pl->nq);
+        for (uint32_t i = 0; i < pl->nq; ++i) oq[i] = query_ids[pl->perm[i]];
+        query_ids = oq.data();
+        if (Lq) {
+            oL.resize(pl->nq);
+            for (uint32_t i = 0; i < pl->nq; ++i) oL[i] = Lq[pl->perm[i]];
+            Lq = oL.data();
+        }
+        if (Kq) {
+            oK.resize(pl->nq);
+            for (uint32_t i = 0; i < pl->nq; ++i) oK[i] = Kq[pl->perm[i]];
+            Kq = oK.data();
+        }
+    }
     HIP_TRY(pl->qids.ensure(pl->nq));
     HIP_TRY(hipMemcpyAsync(pl->qids.p, query_ids, pl->nq * 4ull, hipMemcpyHostToDevice, h->stream));
     if (Lq) {
@@ -401,7 +492,8 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     a.Kstride = pl->maxK;
     a.Lstride = pl->maxL;
     a.init_ids = pl->init_ids.p;
-    a.work_counter = h->counter.p + pl->counter_idx;
+    a.work_counter = h->counter.p + (size_t)pl->counter_idx * kRanges;
+    a.n_ranges = kRanges;
     a.out_ids = pl->out_ids.p;
     a.out_dists = pl->out_dists.p;
     a.out_qids = pl->want_queue ? pl->out_qids.p : nullptr;
@@ -453,7 +545,7 @@ int bind_scratch(panns_plan *pl)
     panns::KArgs &a = pl->args;
     a.bitmaps = h->bitmaps.p + (size_t)pl->slot0 * h->bitmap_words;
     a.spill_log = h->spill_log.p + (size_t)pl->slot0 * kLogCap;
-    a.work_counter = h->counter.p + pl->counter_idx;
+    a.work_counter = h->counter.p + (size_t)pl->counter_idx * kRanges;
     return PANNS_OK;
 }
 
@@ -472,7 +564,7 @@ int launch_parts(panns_plan *pl)
 {
     panns_index *h = pl->h;
     const uint32_t nb = (uint32_t)pl->parts.size();
-    hipLaunchKernelGGL(panns::reset_counter_kernel, dim3(1), dim3(64), 0, h->stream, h->counter.p, nb);
+    hipLaunchKernelGGL(panns::reset_counter_kernel, dim3(1), dim3(64), 0, h->stream, h->counter.p, nb * kRanges);
     HIP_TRY(hipGetLastError());
     if (!pl->opt.concurrent) {
         for (auto &part : pl->parts) HIP_TRY(launch_one(part.get(), h->stream));
@@ -495,7 +587,7 @@ int run_plan(panns_plan *pl, int sync, float *ms)
         int rc = bind_scratch(pl);
         if (rc) return rc;
         hipLaunchKernelGGL(panns::reset_counter_kernel, dim3(1), dim3(64), 0, h->stream,
-                           h->counter.p + pl->counter_idx, 1u);
+                           h->counter.p + (size_t)pl->counter_idx * kRanges, kRanges);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(h->ev0, h->stream));
         HIP_TRY(launch_one(pl, h->stream));
@@ -549,22 +641,30 @@ int fetch_single(panns_plan *pl, panns_outputs *out, panns_counters *cnt, float 
     std::vector<uint32_t> st((size_t)pl->nq * 5);
     HIP_TRY(hipMemcpy(st.data(), pl->out_stats.p, st.size() * 4, hipMemcpyDeviceToHost));
     if (out) {
-        if (out->ids)
-            HIP_TRY(hipMemcpy(out->ids, pl->out_ids.p, (size_t)pl->nq * pl->maxK * 4, hipMemcpyDeviceToHost));
-        if (out->dists)
-            HIP_TRY(hipMemcpy(out->dists, pl->out_dists.p, (size_t)pl->nq * pl->maxK * 4, hipMemcpyDeviceToHost));
+        // rows come back in work-item order; perm maps them to the caller's
+        const std::vector<uint32_t> &pm = pl->perm;
+        auto row = [&](uint32_t i) -> size_t { return pm.empty() ? i : pm[i]; };
+        auto fetch_rows = [&](void *dst, const void *src, size_t row_bytes) -> hipError_t {
+            if (pm.empty()) return hipMemcpy(dst, src, (size_t)pl->nq * row_bytes, hipMemcpyDeviceToHost);
+            std::vector<char> tmp((size_t)pl->nq * row_bytes);
+            hipError_t e = hipMemcpy(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return e;
+            for (uint32_t i = 0; i < pl->nq; ++i)
+                std::memcpy(static_cast<char *>(dst) + row(i) * row_bytes, tmp.data() + (size_t)i * row_bytes,
+                            row_bytes);
+            return hipSuccess;
+        };
+        if (out->ids) HIP_TRY(fetch_rows(out->ids, pl->out_ids.p, (size_t)pl->maxK * 4));
+        if (out->dists) HIP_TRY(fetch_rows(out->dists, pl->out_dists.p, (size_t)pl->maxK * 4));
         if (pl->want_queue) {
-            if (out->queue_ids)
-                HIP_TRY(hipMemcpy(out->queue_ids, pl->out_qids.p, (size_t)pl->nq * pl->maxL * 4, hipMemcpyDeviceToHost));
-            if (out->queue_dists)
-                HIP_TRY(hipMemcpy(out->queue_dists, pl->out_qdists.p, (size_t)pl->nq * pl->maxL * 4, hipMemcpyDeviceToHost));
-            if (out->queue_checked)
-                HIP_TRY(hipMemcpy(out->queue_checked, pl->out_qchk.p, (size_t)pl->nq * pl->maxL, hipMemcpyDeviceToHost));
+            if (out->queue_ids) HIP_TRY(fetch_rows(out->queue_ids, pl->out_qids.p, (size_t)pl->maxL * 4));
+            if (out->queue_dists) HIP_TRY(fetch_rows(out->queue_dists, pl->out_qdists.p, (size_t)pl->maxL * 4));
+            if (out->queue_checked) HIP_TRY(fetch_rows(out->queue_checked, pl->out_qchk.p, (size_t)pl->maxL));
         }
         for (uint32_t i = 0; i < pl->nq; ++i) {
-            if (out->ndist) out->ndist[i] = st[(size_t)i * 5 + 0];
-            if (out->nexp) out->nexp[i] = st[(size_t)i * 5 + 1];
-            if (out->niter) out->niter[i] = st[(size_t)i * 5 + 2];
+            if (out->ndist) out->ndist[row(i)] = st[(size_t)i * 5 + 0];
+            if (out->nexp) out->nexp[row(i)] = st[(size_t)i * 5 + 1];
+            if (out->niter) out->niter[row(i)] = st[(size_t)i * 5 + 2];
         }
     }
     if (cnt) {
@@ -728,6 +828,7 @@ int panns_set_base(panns_index *h, const float *data, uint32_t n, uint32_t dim)
     if (h->n != n) h->has_graph = false;
     h->n = n;
     h->dim = dim;
+    h->has_pivots = false;
     return PANNS_OK;
 }
 

@@ -74,10 +74,11 @@ struct KArgs {
     const uint32_t *qL;    // [n_work] or null
     const uint32_t *qK;    // [n_work] or null
     uint32_t n_work;
+    uint32_t n_ranges;  // work split into this many contiguous ranges, one counter each
     uint32_t L, K, M, mode;
     uint32_t Kstride, Lstride;
     const uint32_t *init_ids;
-    uint32_t *work_counter;
+    uint32_t *work_counter;  // [n_ranges]
     uint32_t *out_ids;
     float *out_dists;
     uint32_t *out_qids;
@@ -682,6 +683,26 @@ __device__ __forceinline__ uint32_t distance_pass(const float *__restrict__ base
     return ncand;
 }
 
+// Next work item for a workgroup.  The items (queries in locality order)
+// are split into n_ranges contiguous ranges, each with its own counter; a
+// workgroup starts on range blockIdx.x % n_ranges -- with workgroups dealt
+// round-robin to the chip's 8 XCDs, each XCD works through its own range, so
+// queries running together on one L2 are neighbours -- and moves on to the
+// next range when its own is exhausted.  Returns n_work when every range is.
+// r and tried are the caller's state (lane 0 keeps them).
+__device__ __forceinline__ uint32_t next_work(const KArgs &a, uint32_t &r, uint32_t &tried)
+{
+    while (tried < a.n_ranges) {
+        const uint32_t lo = (uint32_t)((uint64_t)a.n_work * r / a.n_ranges);
+        const uint32_t hi = (uint32_t)((uint64_t)a.n_work * (r + 1) / a.n_ranges);
+        const uint32_t i = atomicAdd(a.work_counter + r, 1u);
+        if (i < hi - lo) return lo + i;
+        r = (r + 1 == a.n_ranges) ? 0u : r + 1u;
+        ++tried;
+    }
+    return a.n_work;
+}
+
 // kLearned: the variant that evaluates the stop model (a.stop_learned);
 // the other ignores it.
 template <int D, bool kLearned>
@@ -718,8 +739,9 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
     if constexpr (kLearned) lm = load_lane_model(a.model, lane);
     PANNS_PT_DECL
 
+    uint32_t wr = blockIdx.x % a.n_ranges, wtried = 0;
     for (;;) {
-        if (lane == 0) counts[2] = atomicAdd(a.work_counter, 1u);
+        if (lane == 0) counts[2] = next_work(a, wr, wtried);
         __syncthreads();
         const uint32_t w = counts[2];
         PANNS_PT(kPhIdle);
@@ -977,8 +999,34 @@ __global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ d
         dst[i] = src[i];
 }
 
+// Nearest of P pivot rows for each query (fp32, any summation order: used
+// only to order a batch's queries by locality, never for results).
+__global__ void nearest_pivot_kernel(const float *__restrict__ queries, const uint32_t *__restrict__ qids,
+                                     uint32_t nq, const float *__restrict__ piv, uint32_t P, uint32_t D,
+                                     uint32_t *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const float *q = queries + (size_t)qids[i] * D;
+    float best = __int_as_float(0x7f800000);
+    uint32_t bi = 0;
+    for (uint32_t p = 0; p < P; ++p) {
+        const float *v = piv + (size_t)p * D;
+        float d = 0.f;
+        for (uint32_t k = 0; k < D; ++k) {
+            const float t = v[k] - q[k];
+            d = __builtin_fmaf(t, t, d);
+        }
+        if (d < best) {
+            best = d;
+            bi = p;
+        }
+    }
+    out[i] = bi;
+}
+
 // Zeroes the n work counters of a launch sequence (a kernel, not a memset,
-// so the reset can live inside a captured graph).
+// so the reset can live inside a captured graph); n <= 64.
 __global__ void reset_counter_kernel(uint32_t *c, uint32_t n)
 {
     if (threadIdx.x < n) c[threadIdx.x] = 0;

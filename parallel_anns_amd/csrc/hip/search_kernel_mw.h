@@ -70,8 +70,9 @@ __global__ __launch_bounds__(64 * W, 2) void search_kernel_mw(KArgs a)
     vis.log_cap = a.log_cap / W;
     vis.log = a.spill_log + (size_t)blockIdx.x * a.log_cap + (size_t)wid * vis.log_cap;
 
+    uint32_t wr = blockIdx.x % a.n_ranges, wtried = 0;
     for (;;) {
-        if (threadIdx.x == 0) ctrl[kCtlWork] = atomicAdd(a.work_counter, 1u);
+        if (threadIdx.x == 0) ctrl[kCtlWork] = next_work(a, wr, wtried);
         __syncthreads();
         const uint32_t w = ctrl[kCtlWork];
         if (w >= a.n_work) break;
