@@ -194,8 +194,8 @@ def main():
                     help="further recall targets evaluated (3 timed launches each); empty for none")
     ap.add_argument("--ms", default="2,4,8", help="M values tried for the fixed-L operating point")
     ap.add_argument("--op", default=None, help="M,L: fixed-L operating point (skips its search)")
-    ap.add_argument("--learned-op", default=None, help="M,Lmax,threshold: skips training/calibration "
-                                                        "and reuses the model saved in the cache")
+    ap.add_argument("--learned-op", default=None, help="M,Lmax,threshold[,min_iter,every]: skips training and "
+                                                        "calibration and reuses the model saved in the cache")
     ap.add_argument("--train-queries", type=int, default=100_000)
     ap.add_argument("--val-queries", type=int, default=5_000)
     ap.add_argument("--train-L", type=int, default=400)
@@ -261,11 +261,12 @@ def main():
         model_path = os.path.join(ds["dir"] or args.cache_dir, "stop_model.npy")
         lM = fM
         if args.learned_op:
-            lM, lL, thr = args.learned_op.split(",")
-            lM, lL, thr = int(lM), int(lL), float(thr)
+            f = args.learned_op.split(",")
+            lM, lL, thr = int(f[0]), int(f[1]), float(f[2])
+            skw = dict(stop_min_iter=int(f[3]) if len(f) > 3 else 0, stop_every=int(f[4]) if len(f) > 4 else 1)
             model = np.load(model_path)
             train_info = {"reused": model_path}
-            picks = {target: (lL, thr)}
+            picks = {target: (lL, thr, skw)}
         else:
             t0 = time.time()
             ntr, nval = args.train_queries, args.val_queries
@@ -274,6 +275,7 @@ def main():
             feat, hsh, niter, _ = index.search_trace(np.arange(ntr, dtype=np.uint32), K, args.train_L, lM,
                                                      trace_cap=192)
             X, Y = learned.samples_from_trace(feat, hsh, niter)
+            min_iter = learned.first_final_iteration(hsh, niter, 0.01)
             del feat, hsh
             model, losses = learned.train(index, X, Y, epochs=4, batch=4096, lr=2e-3, seed=0)
             np.save(model_path, model)
@@ -281,7 +283,8 @@ def main():
                 f"loss {losses.round(4).tolist()} in {time.time() - t0:.1f}s")
             train_info = {"train_queries": ntr, "samples": int(X.shape[0]), "positive_frac": round(float(Y.mean()), 4),
                           "epoch_loss": [round(float(x), 5) for x in losses], "train_L": args.train_L, "M": lM,
-                          "label": "top-K set already final", "kernels": "BF16 MFMA fwd/bwd, FP32 accumulate, Adam"}
+                          "label": "top-K set already final", "kernels": "BF16 MFMA fwd/bwd, FP32 accumulate, Adam",
+                          "stop_min_iter": min_iter}
             del X, Y
             # calibrate on held-out training queries (their own exact truth)
             index.set_queries(tq[ntr:])
@@ -294,27 +297,35 @@ def main():
                 # target at the model's M; the plain fixed-L search at that L
                 # (a threshold above 1 never stops early) is the fallback
                 Lf = L_at[t]
-                _, nd_fixed, _ = learned.recall_work(index, vq, vgt, K, Lf, lM, 2.0)
-                best_l = (nd_fixed, Lf, 2.0)
+                # candidates are compared by validation device time (the
+                # model's own cost included); the model runs from the
+                # iteration by which 1 % of training queries had their final
+                # top-K, at every iteration or every second one
+                _, _, ms_fixed = learned.recall_work(index, vq, vgt, K, Lf, lM, None, runs=3)
+                best_l = (ms_fixed, Lf, 2.0, {})
                 for f in (float(x) for x in args.learned_L.split(",")):
                     lL = min(4096, int(round(Lf * f)))
-                    thr = learned.calibrate(index, vq, vgt, K, lL, lM, t + 0.002)
-                    if thr is None:
-                        continue
-                    rv, nd, _ = learned.recall_work(index, vq, vgt, K, lL, lM, thr)
-                    log(f"target {t}: calibrated L={lL} threshold {thr:.4g}: validation recall {rv:.4f}, ndist {nd:.0f}")
-                    if nd < best_l[0]:
-                        best_l = (nd, lL, thr)
+                    for every in (1, 2):
+                        kw = dict(stop_min_iter=min_iter, stop_every=every)
+                        thr = learned.calibrate(index, vq, vgt, K, lL, lM, t + 0.002, **kw)
+                        if thr is None:
+                            continue
+                        rv, nd, ms = learned.recall_work(index, vq, vgt, K, lL, lM, thr, runs=3, **kw)
+                        log(f"target {t}: calibrated L={lL} every {every} threshold {thr:.4g}: validation "
+                            f"recall {rv:.4f}, ndist {nd:.0f}, {ms:.3f} ms (fixed L={Lf}: {ms_fixed:.3f} ms)")
+                        if ms < best_l[0]:
+                            best_l = (ms, lL, thr, kw)
                 if best_l[2] >= 1.0:
                     log(f"target {t}: no threshold beats the fixed search on validation; learned = fixed L={Lf}")
                 picks[t] = best_l[1:]
             index.set_queries(ds["queries"])
         index.set_stop_model(model)
         if target in picks:
-            lL, thr = picks[target]
-            stop = dict(stop="learned", stop_threshold=thr) if thr < 1.0 else {}
+            lL, thr, skw = picks[target]
+            stop = dict(stop="learned", stop_threshold=thr, **skw) if thr < 1.0 else {}
             lres, ldesc, lrn = timed(index, qids, K, lL, lM, gt, args.steps, args.warmup, **stop)
             lrn = {"M": lM, "L_max": lL, "threshold": thr, **{k: v for k, v in lrn.items() if k not in ("M", "L")},
+                   "stop_every": skw.get("stop_every", 1), "stop_min_iter": skw.get("stop_min_iter", 0),
                    "fallback_to_fixed": thr >= 1.0, "training": train_info}
             fmax = index.search(qids, K, lL, lM)
             lrn["recall_deficit_vs_fixed_at_L_max"] = deficit_distribution(
@@ -322,8 +333,8 @@ def main():
             log(f"learned: {lrn}")
         for t in sorted(fixed_other):
             if t in picks:
-                lL, thr = picks[t]
-                stop = dict(stop="learned", stop_threshold=thr) if thr < 1.0 else {}
+                lL, thr, skw = picks[t]
+                stop = dict(stop="learned", stop_threshold=thr, **skw) if thr < 1.0 else {}
                 r = timed(index, qids, K, lL, lM, gt, 3, 1, **stop)[2]
                 lrn_other[t] = {"fixed_L": fixed_other[t],
                                 "learned": {"M": lM, "L_max": lL, "threshold": thr, "qps": r["qps"],

@@ -62,7 +62,9 @@ typedef struct {
                               and merged into the master queue, the deterministic form of the
                               reference's multi-thread merge (PSS5LG.cpp:811-842, 1543-1673);
                               results equal the single-wave search */
-    uint32_t reserved[3];
+    uint32_t stop_min_iter;  /* LEARNED: the model runs from this iteration on (0: from the first) */
+    uint32_t stop_every;     /* LEARNED: ... at every stop_every-th iteration (0 or 1: every one) */
+    uint32_t reserved[1];
 } panns_params;
 
 typedef struct {

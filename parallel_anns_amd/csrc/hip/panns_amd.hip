@@ -517,6 +517,8 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     a.shared_bitmap = pl->shared_bitmap;
     a.stop_learned = p.stop == PANNS_STOP_LEARNED ? 1u : 0u;
     a.stop_threshold = p.stop_threshold;
+    a.stop_min_iter = p.stop_min_iter;
+    a.stop_every = std::max<uint32_t>(1u, p.stop_every);
     if (h->has_model) {
         const float *m = h->model.p;
         a.model.w1 = m;

@@ -98,6 +98,7 @@ struct KArgs {
     // learned stop
     uint32_t stop_learned;
     float stop_threshold;
+    uint32_t stop_min_iter, stop_every;  // the model runs at iterations >= min_iter, every stop_every-th
     StopModel model;
     // traces for learned-stop training: per iteration 8 features + top-K hash
     float *trace_feat;      // [n_work][trace_cap][8] or null
@@ -906,7 +907,8 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
             ins_last = ins;
             if (a.mode == kModeScaleM && Mcur < a.M) Mcur <<= 1;
 
-            if (a.stop_learned || a.trace_feat) {
+            const bool eval_stop = kLearned && niter >= a.stop_min_iter && niter % a.stop_every == 0;
+            if (eval_stop || a.trace_feat) {
                 // features: step, top-1 dist, K-th dist, ratio, new keys below
                 // the queue's last key in the last step, first-init (entry) dist, query norm, unchecked count
                 uint32_t unc = 0;
@@ -936,8 +938,10 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                     if (lane == 0) a.trace_topk[(size_t)w * a.trace_cap + (niter - 1)] = hsh;
                 }
                 if constexpr (kLearned) {
-                    const float pstop = mlp_eval(lm, f, h1s, lane, xs);
-                    if (pstop >= a.stop_threshold) break;
+                    if (eval_stop) {
+                        const float pstop = mlp_eval(lm, f, h1s, lane, xs);
+                        if (pstop >= a.stop_threshold) break;
+                    }
                 }
                 PANNS_PT(kPhStop);
             }

@@ -65,27 +65,47 @@ def train(index, X, Y, epochs=4, batch=4096, lr=2e-3, seed=0):
     return P, losses
 
 
-def recall_work(index, qids, gt, K, L, M, threshold):
-    """(recall@K, mean distance computations, kernel ms) of one learned-stop run."""
-    plan = index.plan(qids, K, L, M, stop="learned", stop_threshold=threshold, want_queue=False)
-    ms = plan.run()
+def first_final_iteration(topk_hash, niter, quantile=0.01):
+    """The iteration (1-based) by which `quantile` of the traced queries have
+    reached their final top-K set: the stop model need not run earlier."""
+    firsts = []
+    for q in range(topk_hash.shape[0]):
+        n = int(min(niter[q], topk_hash.shape[1]))
+        if n == 0:
+            continue
+        same = topk_hash[q, :n] == topk_hash[q, n - 1]
+        # first step of the final run of equal hashes
+        j = n - 1
+        while j > 0 and same[j - 1]:
+            j -= 1
+        firsts.append(j + 1)
+    return int(np.quantile(np.asarray(firsts), quantile)) if firsts else 0
+
+
+def recall_work(index, qids, gt, K, L, M, threshold, runs=1, **stop_kw):
+    """(recall@K, mean distance computations, kernel ms -- the least of
+    `runs` launches) of a learned-stop search; stop_kw: stop_min_iter,
+    stop_every.  threshold None runs the fixed-L search."""
+    kw = dict(stop="learned", stop_threshold=threshold, **stop_kw) if threshold is not None else {}
+    plan = index.plan(qids, K, L, M, want_queue=False, **kw)
+    ms = min(plan.run() for _ in range(runs))
     res = plan.fetch()
     plan.close()
     return tools.recall(gt, res["ids"], L, R=min(K, gt.shape[1])), float(res["ndist"].mean()), ms
 
 
-def calibrate(index, qids, gt, K, L, M, target, lo=1e-4, hi=1.0, steps=14):
+def calibrate(index, qids, gt, K, L, M, target, lo=1e-4, hi=1.0, steps=14, **stop_kw):
     """Largest-saving threshold whose recall on (qids, gt) reaches target: the
     recall rises with the threshold, so bisect on it (log scale)."""
-    r_hi = recall_work(index, qids, gt, K, L, M, hi)[0]
+    r_hi = recall_work(index, qids, gt, K, L, M, hi, **stop_kw)[0]
     if r_hi < target:
         return None
     a, b = np.log(lo), np.log(hi)
-    if recall_work(index, qids, gt, K, L, M, lo)[0] >= target:
+    if recall_work(index, qids, gt, K, L, M, lo, **stop_kw)[0] >= target:
         return lo
     for _ in range(steps):
         m = 0.5 * (a + b)
-        if recall_work(index, qids, gt, K, L, M, float(np.exp(m)))[0] >= target:
+        if recall_work(index, qids, gt, K, L, M, float(np.exp(m)), **stop_kw)[0] >= target:
             b = m
         else:
             a = m

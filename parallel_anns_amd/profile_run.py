@@ -1,6 +1,7 @@
 """A fixed sequence of search launches for hardware profilers.
 
-  python -m parallel_anns_amd.profile_run --op M,L[,eighths] [--learned-op M,Lmax,threshold[,eighths]]
+  python -m parallel_anns_amd.profile_run --op M,L[,eighths]
+                                          [--learned-op M,Lmax,threshold[,eighths[,min_iter,every]]]
                                           [--config c1] [--steps 5]
 
 Loads the prepared workload from the cache (and the stop model bench.py
@@ -61,7 +62,8 @@ def main():
         model = np.load(os.path.join(ds["dir"] or args.cache_dir, "stop_model.npy"))
         idx.set_stop_model(model)
         out["learned"] = run(int(g[0]), int(g[1]), int(g[3]) if len(g) > 3 else 8, stop="learned",
-                             stop_threshold=float(g[2]))
+                             stop_threshold=float(g[2]), stop_min_iter=int(g[4]) if len(g) > 4 else 0,
+                             stop_every=int(g[5]) if len(g) > 5 else 1)
         out["learned"]["threshold"] = float(g[2])
     print(json.dumps(out), flush=True)
     idx.close()

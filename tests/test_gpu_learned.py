@@ -120,3 +120,26 @@ def test_in_loop_model_matches_oracle(small_set, gpu_index):
             judged += 1
             assert int(res["niter"][i]) == expect, f"query {i} threshold {thr}"
         assert judged >= 0.9 * q.shape[0]
+
+
+def test_stop_schedule(small_set, gpu_index):
+    """stop_min_iter and stop_every only restrict where the model may stop a
+    query: never before min_iter, only at multiples of stop_every."""
+    ds = small_set
+    K, L, M = 100, 160, 4
+    q = np.arange(ds["queries"].shape[0], dtype=np.uint32)
+    feat, hsh, niter, _ = gpu_index.search_trace(q, K, L, M, trace_cap=256)
+    X, Y = learned.samples_from_trace(feat, hsh, niter)
+    P, _ = learned.train(gpu_index, X, Y, epochs=2, batch=1024)
+    gpu_index.set_stop_model(P)
+    fixed = gpu_index.search(q, K, L, M)
+    late = gpu_index.search(q, K, L, M, stop="learned", stop_threshold=0.0, stop_min_iter=int(niter.max()) + 1)
+    assert (late["ids"] == fixed["ids"]).all() and (late["niter"] == fixed["niter"]).all()
+    five = gpu_index.search(q, K, L, M, stop="learned", stop_threshold=0.0, stop_min_iter=5)
+    assert ((five["niter"] == 5) | ((five["niter"] < 5) & (five["niter"] == fixed["niter"]))).all()
+    every = gpu_index.search(q, K, L, M, stop="learned", stop_threshold=0.4, stop_every=3)
+    stopped_early = every["niter"] < fixed["niter"]
+    assert stopped_early.any()
+    assert (every["niter"][stopped_early] % 3 == 0).all()
+    m1 = learned.first_final_iteration(hsh, niter, 0.01)
+    assert 1 <= m1 <= int(niter.max())
