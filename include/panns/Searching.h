@@ -1,12 +1,17 @@
-/* C++ host drop-in for PANNS::Searching (large-graph CSR form) over the C-ABI
- * of panns_amd.h.  Header-only: include it and link libpanns_amd.so.
+/* C++ host drop-in for PANNS::Searching over the C-ABI of panns_amd.h.
+ * Header-only: include it and link libpanns_amd.so.
  *
- * It keeps the names, argument order and meaning of the reference class
- * (core/Searching.202102031939.PSS_v5.large_graph.dist_thresh.profiling.h:30-268
- * and core/Searching.202011241615.top_m_pure_large_graph.h:30-352), so a
- * driver written against the reference, such as
+ * It keeps the names, argument order and meaning of the reference classes
+ * of both layouts: the large-graph CSR form
+ * (core/Searching.202102031939.PSS_v5.large_graph.dist_thresh.profiling.h:30-268,
+ * core/Searching.202011241615.top_m_pure_large_graph.h:30-352) and the
+ * opt-graph form (core/Searching.202101301609.PSS_v5.profiling.h:30-270,
+ * core/Searching.202011241533.top_m_pure.h:30-360), so a driver written
+ * against the reference, such as
  * app/profile_find_L_para_single_query_top_m_search_pure_large_graph.cpp,
- * compiles against this header once its include line points here.
+ * compiles against this header once its include line points here.  The
+ * graph loader picks the distance as the reference's class does:
+ * load_common_nsg_graph -> squared L2, load_nsg_graph -> norm - 2 v.q.
  *
  * What differs, and why:
  *  - The base set, queries and graph live in GPU memory owned by the handle;
@@ -179,6 +184,15 @@ public:
         refresh_info();
     }
 
+    /* PSS5.cpp:56-132 (opt-graph form): the same .nsg file; later searches
+     * use norm(v) - 2 v.q (PSS5.cpp:336-390) with the norms computed on the
+     * GPU (PSS5.cpp:293-334).  The base set must be loaded first. */
+    void load_nsg_graph(char *filename)
+    {
+        check(panns_load_nsg_graph(h_, filename), "load_nsg_graph");
+        refresh_info();
+    }
+
     /* PSS5LG.cpp:356-410: L ids; init_ids is resized to L when smaller. */
     void prepare_init_ids(std::vector<unsigned> &init_ids, const unsigned L) const
     {
@@ -260,6 +274,61 @@ public:
         (void)top_m_candidates;
         panns_params p = params(PANNS_MODE_TOP_M, K, L, M);
         one_query(p, query_id, set_L, init_ids, set_K, local_queues_starts, local_queues_sizes, is_visited);
+    }
+
+    /* Opt-graph form, PSS5.cpp:1211-1462 at num_threads_ = 1: the same
+     * search as the large-graph one, with the distance of the loaded graph. */
+    void para_search_PSS_v5_profiling(const idi query_id, const idi K, const idi L, std::vector<Candidate> &set_L,
+                                      const std::vector<idi> &init_ids, std::vector<idi> &set_K,
+                                      const idi local_queue_capacity, const std::vector<idi> &local_queues_starts,
+                                      std::vector<idi> &local_queues_sizes, dynamic_bitset &is_visited,
+                                      const idi subsearch_iterations)
+    {
+        para_search_PSS_v5_large_graph_dist_thresh_profiling(query_id, K, L, set_L, init_ids, set_K,
+                                                             local_queue_capacity, local_queues_starts,
+                                                             local_queues_sizes, is_visited, subsearch_iterations);
+    }
+
+    /* Opt-graph form, TOPM.cpp:1881-2070 (its num_threads_ = 1 branch, :1922). */
+    void para_search_with_top_m_pure(const idi M, const idi query_id, const idi K, const idi L,
+                                     std::vector<Candidate> &set_L, const std::vector<idi> &init_ids,
+                                     std::vector<idi> &set_K, const idi local_queue_capacity,
+                                     const std::vector<idi> &local_queues_starts,
+                                     std::vector<idi> &local_queues_sizes, std::vector<idi> &top_m_candidates,
+                                     dynamic_bitset &is_visited)
+    {
+        para_search_with_top_m_pure_large_graph(M, query_id, K, L, set_L, init_ids, set_K, local_queue_capacity,
+                                                local_queues_starts, local_queues_sizes, top_m_candidates,
+                                                is_visited);
+    }
+
+    /* TOPM.cpp:1382-1484 seq_search_with_top_m_pure: top-M with the queue
+     * at set_L[0, L); count_merge_ is not touched, as in the reference. */
+    void seq_search_with_top_m_pure(const idi M, const idi query_id, const idi K, const idi L,
+                                    std::vector<Candidate> &set_L, const std::vector<idi> &init_ids,
+                                    std::vector<idi> &set_K, std::vector<idi> &top_m_candidates,
+                                    dynamic_bitset &is_visited)
+    {
+        (void)top_m_candidates;
+        const std::vector<idi> starts;
+        std::vector<idi> sizes;
+        one_query(params(PANNS_MODE_TOP_M, K, L, M), query_id, set_L, init_ids, set_K, starts, sizes, is_visited,
+                  false);
+    }
+
+    /* TOPM.cpp:1273-1380 seq_search_with_top_m_scale_m: M = 1, 2, 4, ... up
+     * to M_max from the initial queue at set_L[0, L); count_merge_ is not
+     * touched, as in the reference. */
+    void seq_search_with_top_m_scale_m(const idi M_max, const idi query_id, const idi K, const idi L,
+                                       std::vector<Candidate> &set_L, const std::vector<idi> &init_ids,
+                                       std::vector<idi> &set_K, std::vector<idi> &top_m_candidates,
+                                       dynamic_bitset &is_visited)
+    {
+        (void)top_m_candidates;
+        const std::vector<idi> starts;
+        std::vector<idi> sizes;
+        one_query(params(PANNS_MODE_SCALE_M, K, L, M_max), query_id, set_L, init_ids, set_K, starts, sizes,
+                  is_visited, false);
     }
 
     /* TOPMLG.cpp:1023-1071: scale-M search on the initialised queue
@@ -363,7 +432,7 @@ private:
     void one_query(const panns_params &p, idi query_id, std::vector<Candidate> &set_L,
                    const std::vector<idi> &init_ids, std::vector<idi> &set_K,
                    const std::vector<idi> &local_queues_starts, std::vector<idi> &local_queues_sizes,
-                   dynamic_bitset &is_visited)
+                   dynamic_bitset &is_visited, bool count_merges = true)
     {
         const idi master = local_queues_starts.empty() ? 0 : local_queues_starts[num_threads_ - 1];
         if (set_L.size() < (size_t)master + p.L) fail("set_L is smaller than the master queue start + L");
@@ -385,7 +454,7 @@ private:
         for (idi i = 0; i < p.L; ++i) set_L[master + i] = Candidate(qids[i], qd[i], qc[i] != 0);
         if (local_queues_sizes.size() >= (size_t)num_threads_) local_queues_sizes[num_threads_ - 1] = p.L;
         count_distance_computation_ += cnt.distance_computations;
-        count_merge_ += cnt.merges;
+        if (count_merges) count_merge_ += cnt.merges;
         time_expand_ += cnt.kernel_ms * 1e-3;
         is_visited.reset();
     }

@@ -42,6 +42,15 @@ typedef enum {
     PANNS_MODE_SCALE_M = 2  /* M = 1, 2, 4, ... up to M (the subsearch_with_top_m schedule, TOPMLG.cpp:1023-1071) */
 } panns_mode;
 
+/* Distance family of a handle, chosen by the graph loader:
+ * load_common_nsg_graph (the large-graph searches, PSS5LG / TOPMLG) -> L2,
+ * load_nsg_graph (the opt-graph searches, PSS5.cpp / TOPM.cpp) -> NORM_DOT. */
+typedef enum {
+    PANNS_DISTANCE_L2 = 0,       /* squared L2, compute_distance (PSS5LG.cpp:525-560) */
+    PANNS_DISTANCE_NORM_DOT = 1  /* norm(v) - 2 v.q, compute_distance_with_norm (PSS5.cpp:336-390);
+                                    q.q is left out, so values may be negative */
+} panns_distance;
+
 typedef enum {
     PANNS_STOP_FIXED = 0,   /* run until no unchecked candidate (the reference) */
     PANNS_STOP_LEARNED = 1  /* stop when the loaded MLP predicts the top-K final */
@@ -109,11 +118,26 @@ int panns_set_base(panns_index *h, const float *data, uint32_t n, uint32_t dim);
 int panns_set_queries(panns_index *h, const float *data, uint32_t nq, uint32_t dim);
 
 /* PSS5LG.cpp:138-251 load_common_nsg_graph: .nsg file; errors when the
- * vertex count differs from the base set or a neighbour id is >= N. */
+ * vertex count differs from the base set or a neighbour id is >= N.  Sets
+ * the distance to PANNS_DISTANCE_L2. */
 int panns_load_common_nsg_graph(panns_index *h, const char *nsg_path);
-/* In-memory form: adj[v*stride + i] for i < deg[v]. */
+/* PSS5.cpp:56-132 load_nsg_graph (the opt-graph loader): the same file and
+ * checks, plus every vertex's norm (compute_norm, PSS5.cpp:293-334, computed
+ * on the device); sets the distance to PANNS_DISTANCE_NORM_DOT.  Like the
+ * reference, it needs the base set loaded first; unlike it, the base rows
+ * stay where they are (the reference copies them into per-vertex records
+ * and frees data_load_). */
+int panns_load_nsg_graph(panns_index *h, const char *nsg_path);
+/* In-memory form: adj[v*stride + i] for i < deg[v].  Keeps the distance. */
 int panns_set_graph(panns_index *h, uint32_t width, uint32_t ep, uint32_t n, uint32_t stride,
                     const uint32_t *adj, const uint32_t *deg);
+/* Switches the distance family of later searches (panns_distance); the
+ * norms are computed when first needed and again after panns_set_base.
+ * The learned stop and traces run with PANNS_DISTANCE_L2 only. */
+int panns_set_distance(panns_index *h, int distance);
+int panns_get_distance(panns_index *h, int *distance);
+/* The N per-vertex norms (PSS5.cpp:293-334 compute_norm), bit-exact. */
+int panns_get_norms(panns_index *h, float *norms);
 
 /* num_v_, dimension_, num_queries_, width_, ep_, num_e_ (PSS5LG.h:32-45). */
 int panns_get_info(panns_index *h, uint32_t *num_v, uint32_t *dim, uint32_t *num_queries,

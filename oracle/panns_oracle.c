@@ -16,35 +16,64 @@
 static inline int bit_get(const uint64_t *bits, uint32_t i) { return (int)((bits[i >> 6] >> (i & 63)) & 1u); }
 static inline void bit_set(uint64_t *bits, uint32_t i) { bits[i >> 6] |= (uint64_t)1 << (i & 63); }
 
-/* PSS5LG.cpp:525-560.  D8 = (D+7)&~7, DR = D8 % 16, DD = D8 - DR.  Eight
- * accumulators; the tail block [DD, DD+8) is added first when DR != 0, then
- * blocks of 16 as two 8-wide halves.  Each lane: acc = fma(v-q, v-q, acc).
- * Horizontal sum left to right. */
-float po_distance(const float *v, const float *q, uint32_t dim)
+/* The 8-accumulator chain shared by the reference's three AVX kernels
+ * (PSS5LG.cpp:525-560 compute_distance, PSS5.cpp:293-334 compute_norm,
+ * PSS5.cpp:336-390 compute_distance_with_norm).  D8 = (D+7)&~7,
+ * DR = D8 % 16, DD = D8 - DR.  The tail block [DD, DD+8) is added first when
+ * DR != 0, then blocks of 16 as two 8-wide halves; lane j accumulates with
+ * one fma per element:  L2: fma(v-q, v-q, acc)   DOT: fma(v, q, acc)
+ * NORM: fma(v, v, acc).  Horizontal sum left to right. */
+enum { CH_L2 = 0, CH_DOT = 1, CH_NORM = 2 };
+
+static inline float chain_term(float a, float b, float acc, int kind)
+{
+    if (kind == CH_L2) {
+        float t = a - b;
+        return fmaf(t, t, acc);
+    }
+    if (kind == CH_DOT) return fmaf(a, b, acc);
+    return fmaf(a, a, acc);
+}
+
+static float chain8(const float *v, const float *q, uint32_t dim, int kind)
 {
     const uint32_t D8 = (dim + 7u) & ~7u;
     const uint32_t DR = D8 % 16u;
     const uint32_t DD = D8 - DR;
     float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (DR) {
-        for (int j = 0; j < 8; ++j) {
-            float t = v[DD + j] - q[DD + j];
-            acc[j] = fmaf(t, t, acc[j]);
-        }
+        for (int j = 0; j < 8; ++j) acc[j] = chain_term(v[DD + j], q[DD + j], acc[j], kind);
     }
     for (uint32_t i = 0; i < DD; i += 16) {
-        for (int j = 0; j < 8; ++j) {
-            float t = v[i + j] - q[i + j];
-            acc[j] = fmaf(t, t, acc[j]);
-        }
-        for (int j = 0; j < 8; ++j) {
-            float t = v[i + 8 + j] - q[i + 8 + j];
-            acc[j] = fmaf(t, t, acc[j]);
-        }
+        for (int j = 0; j < 8; ++j) acc[j] = chain_term(v[i + j], q[i + j], acc[j], kind);
+        for (int j = 0; j < 8; ++j) acc[j] = chain_term(v[i + 8 + j], q[i + 8 + j], acc[j], kind);
     }
     float r = acc[0];
     for (int j = 1; j < 8; ++j) r = r + acc[j];
     return r;
+}
+
+/* PSS5LG.cpp:525-560 */
+float po_distance(const float *v, const float *q, uint32_t dim) { return chain8(v, q, dim, CH_L2); }
+
+/* PSS5.cpp:293-334 */
+float po_norm(const float *v, uint32_t dim) { return chain8(v, v, dim, CH_NORM); }
+
+/* PSS5.cpp:336-390: result = -2 * dot + vertex_norm (one rounding whether
+ * or not it is fused, -2 * dot being exact). */
+float po_distance_with_norm(const float *v, const float *q, uint32_t dim, float norm)
+{
+    return fmaf(-2.0f, chain8(v, q, dim, CH_DOT), norm);
+}
+
+/* Distance of base row v to the query under the graph's layout: the CSR
+ * (large-graph) family uses squared L2, the opt-graph family, whose vertex
+ * records carry the norm (PSS5.cpp:86-91), uses norm - 2 * dot. */
+static inline float vdist(const float *norms, const float *base, uint32_t dim, uint32_t v,
+                          const float *query)
+{
+    const float *row = base + (size_t)v * dim;
+    return norms ? po_distance_with_norm(row, query, dim, norms[v]) : po_distance(row, query, dim);
 }
 
 int po_cand_less(const po_cand *a, const po_cand *b)
@@ -136,10 +165,11 @@ static int cmp_cand(const void *a, const void *b)
     return 0;
 }
 
-/* PSS5LG.cpp:1055-1096 */
+/* PSS5LG.cpp:1055-1096 (PSS5.cpp:890-927 with norms) */
 void po_initialize_set_L(const float *base, uint32_t dim, const float *query, uint32_t L,
                          po_cand *set_L, uint32_t set_L_start, uint32_t *set_L_size,
-                         const uint32_t *init_ids, uint64_t *visited, uint64_t *count_distance)
+                         const uint32_t *init_ids, uint64_t *visited, uint64_t *count_distance,
+                         const float *norms)
 {
     for (uint32_t c = 0; c < L; ++c) bit_set(visited, init_ids[c]);
     for (uint32_t i = 0; i < L; ++i) {
@@ -147,7 +177,7 @@ void po_initialize_set_L(const float *base, uint32_t dim, const float *query, ui
         po_cand c;
         memset(&c, 0, sizeof(c));
         c.id = v;
-        c.dist = po_distance(base + (size_t)v * dim, query, dim);
+        c.dist = vdist(norms, base, dim, v, query);
         c.checked = 0;
         set_L[set_L_start + i] = c;
     }
@@ -176,7 +206,7 @@ static uint32_t expand_one_candidate(const po_graph *g, const float *base, uint3
         if (bit_get(visited, nb)) continue;
         bit_set(visited, nb);
         ++cnt;
-        float d = po_distance(base + (size_t)nb * dim, query, dim);
+        float d = vdist(g->norms, base, dim, nb, query);
         if (d > *dist_bound) continue;
         po_cand c;
         memset(&c, 0, sizeof(c));
@@ -215,7 +245,7 @@ void po_search_top_m(const po_graph *g, const float *base, uint32_t dim, const f
 {
     uint32_t size = 0;
     uint64_t ndist = 0;
-    po_initialize_set_L(base, dim, query, L, set_L, 0, &size, init_ids, visited, &ndist);
+    po_initialize_set_L(base, dim, query, L, set_L, 0, &size, init_ids, visited, &ndist, g->norms);
     const float *last_dist = &set_L[L - 1].dist;
     uint32_t *top_m = (uint32_t *)malloc(sizeof(uint32_t) * (M ? M : 1));
     uint32_t k = 0, last_k = 0, n_top = 0;
@@ -255,7 +285,7 @@ void po_search_pss_v5_t1(const po_graph *g, const float *base, uint32_t dim, con
 {
     uint32_t size = 0;
     uint64_t ndist = 0, merges = 0, exps = 0;
-    po_initialize_set_L(base, dim, query, L, set_L, 0, &size, init_ids, visited, &ndist);
+    po_initialize_set_L(base, dim, query, L, set_L, 0, &size, init_ids, visited, &ndist, g->norms);
     const float *last_dist = &set_L[size - 1].dist;
     uint32_t k_master = 0;
     int no_need = 0;

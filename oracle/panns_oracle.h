@@ -9,10 +9,22 @@
  * Every function names the reference file:line it follows.  Tags:
  *   PSS5LG = core/Searching.202102031939.PSS_v5.large_graph.dist_thresh.profiling.{cpp,h}
  *   TOPMLG = core/Searching.202011241615.top_m_pure_large_graph.{cpp,h}
+ *   PSS5   = core/Searching.202101301609.PSS_v5.profiling.{cpp,h}   (opt-graph family)
+ *   TOPM   = core/Searching.202011241533.top_m_pure.{cpp,h}         (opt-graph family)
+ *
+ * The opt-graph family runs the same searches with the distance
+ * norm(v) - 2 v.q (PSS5.cpp:336-390) instead of squared L2; a po_graph with
+ * non-NULL norms selects it.
  *
  * Pinning (what checks each function against something other than itself):
  *   po_distance            AVX2 intrinsics build with GCC contraction (tests),
  *                          fp64 closeness, v==q -> +0, one-hot exact values.
+ *   po_norm, po_distance_with_norm   the same AVX2 route for the reference's
+ *                          norm and dot kernels; exact values on integer
+ *                          vectors (norm - 2 v.q == |v-q|^2 - |q|^2); fp64
+ *                          closeness.  Searches with norms: on integer data
+ *                          equal the squared-L2 searches (same ids, counts;
+ *                          distances shifted by |q|^2 exactly).
  *   po_add_into_queue      brute force over tiny queues (sorted insert model).
  *   po_prepare_init_ids    hand-made graphs with known answers.
  *   po_search_top_m        L == N reduces to exact brute force; complete graph;
@@ -55,6 +67,10 @@ typedef struct {
     uint64_t num_e;
     const uint64_t *vertex_base;
     const uint32_t *deg_ngbrs;
+    /* NULL: squared-L2 distances (large-graph family).  Else norms[v] =
+     * po_norm of base row v, and distances are norm - 2 v.q (opt-graph
+     * family, load_nsg_graph PSS5.cpp:56-132 stores it in each record). */
+    const float *norms;
 } po_graph;
 
 typedef struct {
@@ -68,6 +84,13 @@ typedef struct {
  * horizontal sum.  dim must be a multiple of 8. */
 float po_distance(const float *v, const float *q, uint32_t dim);
 
+/* PSS5.cpp:293-334 compute_norm: the same chain with fma(v, v, acc). */
+float po_norm(const float *v, uint32_t dim);
+
+/* PSS5.cpp:336-390 compute_distance_with_norm: the same chain with
+ * fma(v, q, acc), then -2 * dot + norm. */
+float po_distance_with_norm(const float *v, const float *q, uint32_t dim, float norm);
+
 /* include/Candidate.h:19-27 operator< */
 int po_cand_less(const po_cand *a, const po_cand *b);
 
@@ -80,7 +103,8 @@ uint32_t po_add_into_queue(po_cand *queue, uint32_t queue_start, uint32_t *queue
 int po_prepare_init_ids(const po_graph *g, uint32_t L, uint32_t *init_ids);
 
 /* TOPMLG.cpp:2022-2207 para_search_with_top_m_pure_large_graph with
- * num_threads_ == 1.  visited must hold num_v bits, all clear; it is cleared
+ * num_threads_ == 1 (TOPM.cpp:1881-2070 para_search_with_top_m_pure, its
+ * num_threads_ == 1 branch at :1922, when g->norms != NULL).  visited must hold num_v bits, all clear; it is cleared
  * again on return (PSS5LG.cpp:1691).  set_L receives the final queue (L
  * entries), set_K the first K ids. */
 void po_search_top_m(const po_graph *g, const float *base, uint32_t dim, const float *query,
@@ -88,7 +112,8 @@ void po_search_top_m(const po_graph *g, const float *base, uint32_t dim, const f
                      po_cand *set_L, uint32_t *set_K, uint64_t *visited, po_counters *cnt);
 
 /* PSS5LG.cpp:1376-1724 para_search_PSS_v5_large_graph_dist_thresh_profiling
- * with num_threads_ == 1 (X = subsearch_iterations). */
+ * with num_threads_ == 1 (X = subsearch_iterations); PSS5.cpp:1211-1462
+ * para_search_PSS_v5_profiling when g->norms != NULL. */
 void po_search_pss_v5_t1(const po_graph *g, const float *base, uint32_t dim, const float *query,
                          uint32_t K, uint32_t L, uint32_t X, const uint32_t *init_ids,
                          po_cand *set_L, uint32_t *set_K, uint64_t *visited, po_counters *cnt);
@@ -103,10 +128,12 @@ void po_subsearch_with_top_m(const po_graph *g, const float *base, uint32_t dim,
                              uint64_t *visited, uint64_t *count_distance);
 
 /* PSS5LG.cpp:1055-1096 initialize_set_L_para (marks init ids visited,
- * computes L distances, sorts).  Exposed for subsearch tests. */
+ * computes L distances, sorts); PSS5.cpp:890-927 when norms != NULL.
+ * Exposed for subsearch tests. */
 void po_initialize_set_L(const float *base, uint32_t dim, const float *query, uint32_t L,
                          po_cand *set_L, uint32_t set_L_start, uint32_t *set_L_size,
-                         const uint32_t *init_ids, uint64_t *visited, uint64_t *count_distance);
+                         const uint32_t *init_ids, uint64_t *visited, uint64_t *count_distance,
+                         const float *norms);
 
 /* Batch driver used for the CPU baseline: queries [q0, q0+nq) searched
  * independently, one per OpenMP thread (inter-query parallel, num_t = 1 per

@@ -26,7 +26,7 @@ CAND_DT = np.dtype([("id", "<u4"), ("dist", "<f4"), ("checked", "u1"), ("pad", "
 
 class Graph(ctypes.Structure):
     _fields_ = [("num_v", u32), ("width", u32), ("ep", u32), ("num_e", u64),
-                ("vertex_base", vp), ("deg_ngbrs", vp)]
+                ("vertex_base", vp), ("deg_ngbrs", vp), ("norms", vp)]
 
 
 class Counters(ctypes.Structure):
@@ -35,6 +35,8 @@ class Counters(ctypes.Structure):
 
 _API = {
     "po_distance": (cf, [vp, vp, u32]),
+    "po_norm": (cf, [vp, u32]),
+    "po_distance_with_norm": (cf, [vp, vp, u32, cf]),
     "po_add_into_queue": (u32, [vp, u32, ctypes.POINTER(u32), u32, Cand]),
     "po_prepare_init_ids": (cint, [ctypes.POINTER(Graph), u32, vp]),
     "po_search_top_m": (None, [ctypes.POINTER(Graph), vp, u32, vp, u32, u32, u32, vp, vp, vp, vp,
@@ -44,7 +46,7 @@ _API = {
     "po_subsearch_with_top_m": (None, [ctypes.POINTER(Graph), vp, u32, vp, u32, u32, vp, u32,
                                        ctypes.POINTER(u32), vp, ctypes.POINTER(u64)]),
     "po_initialize_set_L": (None, [vp, u32, vp, u32, vp, u32, ctypes.POINTER(u32), vp, vp,
-                                   ctypes.POINTER(u64)]),
+                                   ctypes.POINTER(u64), vp]),
     "po_search_batch": (u64, [ctypes.POINTER(Graph), vp, u32, vp, u32, u32, u32, u32, cint, vp, vp,
                               vp, vp, cint]),
     "po_brute_force_knn": (None, [vp, u32, u32, vp, u32, u32, vp, vp, cint]),
@@ -76,9 +78,13 @@ def _p(a):
 
 
 class CsrGraph:
-    """The reference's CSR (vertex_base + [deg, ids...]) built from padded rows."""
+    """The reference's CSR (vertex_base + [deg, ids...]) built from padded rows.
 
-    def __init__(self, adj, deg, ep, width=None):
+    With `base` given, the graph also carries every vertex's norm (po_norm)
+    and the searches use the opt-graph distance norm - 2 v.q (the PSS5 / TOPM
+    family, load_nsg_graph PSS5.cpp:56-132); without, squared L2."""
+
+    def __init__(self, adj, deg, ep, width=None, base=None):
         adj = np.asarray(adj, dtype=np.uint32)
         deg = np.asarray(deg, dtype=np.uint32)
         n = adj.shape[0]
@@ -92,8 +98,9 @@ class CsrGraph:
             self.deg_ngbrs[off] = d
             self.deg_ngbrs[off + 1:off + 1 + d] = adj[v, :d]
             off += 1 + d
+        self.norms = None if base is None else norms(base)
         self.g = Graph(n, int(width if width is not None else (deg.max() if n else 0)), int(ep),
-                       int(deg.sum()), _p(self.vertex_base), _p(self.deg_ngbrs))
+                       int(deg.sum()), _p(self.vertex_base), _p(self.deg_ngbrs), _p(self.norms))
         self.n = n
 
 
@@ -101,6 +108,22 @@ def distance(v, q):
     v = np.ascontiguousarray(v, np.float32)
     q = np.ascontiguousarray(q, np.float32)
     return float(lib().po_distance(_p(v), _p(q), v.shape[-1]))
+
+
+def norm(v):
+    v = np.ascontiguousarray(v, np.float32)
+    return float(lib().po_norm(_p(v), v.shape[-1]))
+
+
+def norms(base):
+    base = np.ascontiguousarray(base, np.float32)
+    return np.array([lib().po_norm(_p(base[i]), base.shape[1]) for i in range(base.shape[0])], np.float32)
+
+
+def distance_with_norm(v, q, nrm):
+    v = np.ascontiguousarray(v, np.float32)
+    q = np.ascontiguousarray(q, np.float32)
+    return float(lib().po_distance_with_norm(_p(v), _p(q), v.shape[-1], float(np.float32(nrm))))
 
 
 def prepare_init_ids(graph, L):
@@ -126,7 +149,7 @@ def search_one(graph, base, query, K, L, M, init_ids, mode="top_m", X=1):
             "nexp": c.expansions, "visited_clear": not visited.any()}
 
 
-def initialize_set_L(base, query, L, init_ids, n):
+def initialize_set_L(base, query, L, init_ids, n, norms=None):
     base = np.ascontiguousarray(base, np.float32)
     query = np.ascontiguousarray(query, np.float32)
     init_ids = np.ascontiguousarray(init_ids, np.uint32)
@@ -135,7 +158,7 @@ def initialize_set_L(base, query, L, init_ids, n):
     visited = np.zeros((n + 63) // 64, np.uint64)
     cnt = u64(0)
     lib().po_initialize_set_L(_p(base), base.shape[1], _p(query), L, _p(set_L), 0, ctypes.byref(size),
-                              _p(init_ids), _p(visited), ctypes.byref(cnt))
+                              _p(init_ids), _p(visited), ctypes.byref(cnt), _p(norms))
     return set_L, visited, cnt.value
 
 

@@ -107,6 +107,11 @@ struct panns_index {
     DevBuf<float> pivots;
     std::vector<uint32_t> tourpos;
     bool has_pivots = false;
+    // distance family: squared L2 (large-graph searches) or norm - 2 v.q
+    // (opt-graph searches), with the per-vertex norms the latter reads
+    uint32_t distance = PANNS_DISTANCE_L2;
+    DevBuf<float> norms;
+    bool norms_valid = false;
 };
 
 struct panns_plan {
@@ -158,17 +163,17 @@ struct panns_plan {
 
 namespace {
 
-template <int D, bool kLearned>
+template <int D, bool kLearned, bool kNorm>
 hipError_t launch_search(const panns::KArgs &a, uint32_t grid, uint32_t lds, hipStream_t s)
 {
-    hipLaunchKernelGGL((panns::search_kernel<D, kLearned>), dim3(grid), dim3(64), lds, s, a);
+    hipLaunchKernelGGL((panns::search_kernel<D, kLearned, kNorm>), dim3(grid), dim3(64), lds, s, a);
     return hipGetLastError();
 }
 
-template <int D, int W>
+template <int D, int W, bool kNorm>
 hipError_t launch_search_mw(const panns::KArgs &a, uint32_t grid, uint32_t lds, hipStream_t s)
 {
-    hipLaunchKernelGGL((panns::search_kernel_mw<D, W>), dim3(grid), dim3(64 * W), lds, s, a);
+    hipLaunchKernelGGL((panns::search_kernel_mw<D, W, kNorm>), dim3(grid), dim3(64 * W), lds, s, a);
     return hipGetLastError();
 }
 
@@ -176,15 +181,17 @@ using LaunchFn = hipError_t (*)(const panns::KArgs &, uint32_t, uint32_t, hipStr
 using KernelPtr = const void *;
 
 // The search kernel for a dimension; `learned` selects the variant that
-// evaluates the stop model.
-bool kernel_for_dim(uint32_t dim, bool learned, LaunchFn *fn, KernelPtr *kp)
+// evaluates the stop model (squared L2 only), `norm` the opt-graph distance.
+bool kernel_for_dim(uint32_t dim, bool learned, bool norm, LaunchFn *fn, KernelPtr *kp)
 {
     switch (dim) {
-#define PANNS_DIM(DD)                                                                          \
-    case DD:                                                                                   \
-        *fn = learned ? launch_search<DD, true> : launch_search<DD, false>;                     \
-        *kp = learned ? reinterpret_cast<const void *>(&panns::search_kernel<DD, true>)        \
-                      : reinterpret_cast<const void *>(&panns::search_kernel<DD, false>);      \
+#define PANNS_DIM(DD)                                                                                   \
+    case DD:                                                                                            \
+        *fn = norm ? launch_search<DD, false, true>                                                     \
+                   : (learned ? launch_search<DD, true, false> : launch_search<DD, false, false>);       \
+        *kp = norm ? reinterpret_cast<const void *>(&panns::search_kernel<DD, false, true>)             \
+                   : (learned ? reinterpret_cast<const void *>(&panns::search_kernel<DD, true, false>)  \
+                              : reinterpret_cast<const void *>(&panns::search_kernel<DD, false, false>)); \
         return true;
         PANNS_DIM(16)
         PANNS_DIM(32)
@@ -199,14 +206,20 @@ bool kernel_for_dim(uint32_t dim, bool learned, LaunchFn *fn, KernelPtr *kp)
 }
 
 // The multi-wave kernel for a dimension and W waves per query (2 or 4).
-bool kernel_for_dim_mw(uint32_t dim, uint32_t waves, LaunchFn *fn, KernelPtr *kp)
+bool kernel_for_dim_mw(uint32_t dim, uint32_t waves, bool norm, LaunchFn *fn, KernelPtr *kp)
 {
     switch (dim) {
-#define PANNS_DIM(DD)                                                                          \
-    case DD:                                                                                   \
-        *fn = waves == 2 ? launch_search_mw<DD, 2> : launch_search_mw<DD, 4>;                   \
-        *kp = waves == 2 ? reinterpret_cast<const void *>(&panns::search_kernel_mw<DD, 2>)      \
-                         : reinterpret_cast<const void *>(&panns::search_kernel_mw<DD, 4>);     \
+#define PANNS_DIM(DD)                                                                                     \
+    case DD:                                                                                              \
+        if (norm) {                                                                                       \
+            *fn = waves == 2 ? launch_search_mw<DD, 2, true> : launch_search_mw<DD, 4, true>;              \
+            *kp = waves == 2 ? reinterpret_cast<const void *>(&panns::search_kernel_mw<DD, 2, true>)       \
+                             : reinterpret_cast<const void *>(&panns::search_kernel_mw<DD, 4, true>);      \
+        } else {                                                                                          \
+            *fn = waves == 2 ? launch_search_mw<DD, 2, false> : launch_search_mw<DD, 4, false>;            \
+            *kp = waves == 2 ? reinterpret_cast<const void *>(&panns::search_kernel_mw<DD, 2, false>)      \
+                             : reinterpret_cast<const void *>(&panns::search_kernel_mw<DD, 4, false>);     \
+        }                                                                                                 \
         return true;
         PANNS_DIM(16)
         PANNS_DIM(32)
@@ -220,9 +233,36 @@ bool kernel_for_dim_mw(uint32_t dim, uint32_t waves, LaunchFn *fn, KernelPtr *kp
     }
 }
 
-bool select_kernel(uint32_t dim, bool learned, uint32_t waves, LaunchFn *fn, KernelPtr *kp)
+bool select_kernel(uint32_t dim, bool learned, bool norm, uint32_t waves, LaunchFn *fn, KernelPtr *kp)
 {
-    return waves > 1 ? kernel_for_dim_mw(dim, waves, fn, kp) : kernel_for_dim(dim, learned, fn, kp);
+    return waves > 1 ? kernel_for_dim_mw(dim, waves, norm, fn, kp) : kernel_for_dim(dim, learned, norm, fn, kp);
+}
+
+// PSS5.cpp:293-334 compute_norm for every base row, once per base set.
+int ensure_norms(panns_index *h)
+{
+    if (h->norms_valid) return PANNS_OK;
+    HIP_TRY(h->norms.ensure(h->n));
+    const dim3 grid((h->n + 255) / 256), block(256);
+    switch (h->dim) {
+#define PANNS_DIM(DD)                                                                                  \
+    case DD:                                                                                           \
+        hipLaunchKernelGGL(panns::norm_kernel<DD>, grid, block, 0, h->stream, h->base.p, h->n, h->norms.p); \
+        break;
+        PANNS_DIM(16)
+        PANNS_DIM(32)
+        PANNS_DIM(40)
+        PANNS_DIM(64)
+        PANNS_DIM(96)
+        PANNS_DIM(128)
+#undef PANNS_DIM
+    default:
+        return fail(PANNS_ERR_DIM, "dimension " + std::to_string(h->dim) + " has no kernel (16, 32, 40, 64, 96, 128)");
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->norms_valid = true;
+    return PANNS_OK;
 }
 
 int check_ready(panns_index *h)
@@ -247,6 +287,8 @@ int check_params(panns_index *h, const panns_params *p, uint32_t L, uint32_t K)
     if (p->M == 0) return fail(PANNS_ERR_ARG, "M must be positive");
     if (p->M > kMaxM) return fail(PANNS_ERR_UNSUPPORTED, "M above 1024 is not offered");
     if (p->stop == PANNS_STOP_LEARNED && !h->has_model) return fail(PANNS_ERR_STATE, "no stop model loaded");
+    if (p->stop == PANNS_STOP_LEARNED && h->distance != PANNS_DISTANCE_L2)
+        return fail(PANNS_ERR_UNSUPPORTED, "the learned stop's features are squared-L2 distances; use the fixed stop");
     if (p->hash_slots && (p->hash_slots % 64 || p->hash_slots < 64 || p->hash_slots > 32768))
         return fail(PANNS_ERR_ARG, "hash_slots must be a multiple of 64 in [64, 32768]");
     if (p->waves > 1) {
@@ -359,9 +401,11 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     pl->waves = std::max<uint32_t>(1u, p.waves);
     if (pl->waves > 1 && (pl->init_queue.p || pl->trace_cap || pl->shared_bitmap))
         return fail(PANNS_ERR_UNSUPPORTED, "traces and caller-held queues run with waves = 1");
+    const bool norm = h->distance == PANNS_DISTANCE_NORM_DOT;
+    if (norm && pl->trace_cap) return fail(PANNS_ERR_UNSUPPORTED, "traces are taken with squared-L2 distances");
     LaunchFn fn;
     KernelPtr kp;
-    if (!select_kernel(h->dim, p.stop == PANNS_STOP_LEARNED, pl->waves, &fn, &kp))
+    if (!select_kernel(h->dim, p.stop == PANNS_STOP_LEARNED, norm, pl->waves, &fn, &kp))
         return fail(PANNS_ERR_DIM, "dimension " + std::to_string(h->dim) + " has no kernel (16, 32, 40, 64, 96, 128)");
     pl->maxL = p.L;
     pl->maxK = p.K;
@@ -477,6 +521,12 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     panns::KArgs &a = pl->args;
     a = panns::KArgs{};
     a.base = h->base.p;
+    a.norms = nullptr;
+    if (norm) {
+        int nrc = ensure_norms(h);
+        if (nrc) return nrc;
+        a.norms = h->norms.p;
+    }
     a.adj = h->adj.p;
     a.n = h->n;
     a.stride_log2 = h->stride_log2;
@@ -555,7 +605,7 @@ hipError_t launch_one(const panns_plan *pl, hipStream_t st)
 {
     LaunchFn fn;
     KernelPtr kp;
-    select_kernel(pl->h->dim, pl->args.stop_learned != 0, pl->waves, &fn, &kp);
+    select_kernel(pl->h->dim, pl->args.stop_learned != 0, pl->args.norms != nullptr, pl->waves, &fn, &kp);
     return fn(pl->args, pl->grid, pl->lds, st);
 }
 
@@ -831,6 +881,7 @@ int panns_set_base(panns_index *h, const float *data, uint32_t n, uint32_t dim)
     h->n = n;
     h->dim = dim;
     h->has_pivots = false;
+    h->norms_valid = false;
     return PANNS_OK;
 }
 
@@ -881,8 +932,51 @@ int panns_load_common_nsg_graph(panns_index *h, const char *path)
     const uint32_t stride = std::max<uint32_t>(md, 1u);
     std::vector<uint32_t> adj((size_t)n * stride), deg(n);
     if (pt_nsg_read(path, n, stride, adj.data(), deg.data())) return fail(PANNS_ERR_FORMAT, pt_last_error());
+    int rc2;
     HIP_TRY(hipSetDevice(h->device));
-    return upload_graph(h, width, ep, n, stride, adj.data(), deg.data());
+    rc2 = upload_graph(h, width, ep, n, stride, adj.data(), deg.data());
+    if (rc2 == PANNS_OK) h->distance = PANNS_DISTANCE_L2;
+    return rc2;
+}
+
+int panns_load_nsg_graph(panns_index *h, const char *path)
+{
+    int rc = panns_load_common_nsg_graph(h, path);
+    if (rc) return rc;
+    return panns_set_distance(h, PANNS_DISTANCE_NORM_DOT);
+}
+
+int panns_set_distance(panns_index *h, int distance)
+{
+    if (!h) return fail(PANNS_ERR_ARG, "null handle");
+    if (distance != PANNS_DISTANCE_L2 && distance != PANNS_DISTANCE_NORM_DOT)
+        return fail(PANNS_ERR_ARG, "unknown distance");
+    if (distance == PANNS_DISTANCE_NORM_DOT) {
+        if (!h->n) return fail(PANNS_ERR_STATE, "load the base set before choosing the norm distance");
+        HIP_TRY(hipSetDevice(h->device));
+        int rc = ensure_norms(h);
+        if (rc) return rc;
+    }
+    h->distance = (uint32_t)distance;
+    return PANNS_OK;
+}
+
+int panns_get_distance(panns_index *h, int *distance)
+{
+    if (!h || !distance) return fail(PANNS_ERR_ARG, "null argument");
+    *distance = (int)h->distance;
+    return PANNS_OK;
+}
+
+int panns_get_norms(panns_index *h, float *out)
+{
+    if (!h || !out) return fail(PANNS_ERR_ARG, "null argument");
+    if (!h->n) return fail(PANNS_ERR_STATE, "base set not loaded");
+    HIP_TRY(hipSetDevice(h->device));
+    int rc = ensure_norms(h);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(out, h->norms.p, (size_t)h->n * 4, hipMemcpyDeviceToHost));
+    return PANNS_OK;
 }
 
 int panns_set_graph(panns_index *h, uint32_t width, uint32_t ep, uint32_t n, uint32_t stride,
@@ -1270,16 +1364,23 @@ int panns_subsearch_with_top_m(panns_index *h, uint32_t M_max, uint32_t query_id
     p.num_threads = 1;
     rc = check_params(h, &p, local_L, 1);
     if (rc) return rc;
+    const bool norm = h->distance == PANNS_DISTANCE_NORM_DOT;
     std::vector<uint64_t> keys(local_L);
     for (uint32_t i = 0; i < local_L; ++i) {
         if (queue_ids[i] >= h->n) return fail(PANNS_ERR_RANGE, "queue id out of range");
         uint32_t db;
         std::memcpy(&db, &queue_dists[i], 4);
+        if (norm) {
+            // the kernel's key order for distances of either sign (-0 as +0)
+            if (db == 0x80000000u) db = 0;
+            db ^= (db & 0x80000000u) ? 0xFFFFFFFFu : 0x80000000u;
+        }
         keys[i] = ((uint64_t)db << 32) | queue_ids[i] | (queue_checked[i] ? panns::kChk : 0ull);
         if (i && (keys[i] & ~panns::kChk) < (keys[i - 1] & ~panns::kChk))
             return fail(PANNS_ERR_ARG, "queue is not sorted by (distance, id)");
-        if (queue_dists[i] < 0.f || queue_dists[i] != queue_dists[i])
-            return fail(PANNS_ERR_ARG, "queue distances must be non-negative numbers");
+        if (queue_dists[i] != queue_dists[i] || (!norm && queue_dists[i] < 0.f))
+            return fail(PANNS_ERR_ARG, norm ? "queue distances must be numbers"
+                                            : "queue distances must be non-negative numbers");
     }
     HIP_TRY(hipSetDevice(h->device));
     const size_t words32 = ((size_t)h->n + 31) / 32;

@@ -66,6 +66,7 @@ struct StopModel {
 
 struct KArgs {
     const float *base;
+    const float *norms;   // [n] per-vertex norms (opt-graph distance), or null
     const uint32_t *adj;  // [n][stride], unused slots = kSent
     uint32_t n;
     uint32_t stride_log2;
@@ -141,6 +142,28 @@ enum : int { kPhInit = 0, kPhPick, kPhClaim, kPhDist, kPhMerge, kPhStop, kPhOut,
 __device__ __forceinline__ uint64_t mk_key(float d, uint32_t id)
 {
     return ((uint64_t)__float_as_uint(d) << 32) | (uint64_t)id;
+}
+
+// Distances as ordered 32-bit key words.  Squared L2 is never negative
+// (an fma chain of squares from +0), so its bit pattern already orders like
+// its value.  The opt-graph distance norm - 2 v.q is negative whenever
+// v.q > norm / 2, so kNorm maps the sign-magnitude bits onto an unsigned
+// order: positive values get the top bit set, negative ones are inverted.
+// -0 cannot occur (the dot chain starts at +0 and -2 * dot + norm rounds an
+// exact zero to +0), so the key order equals the reference's float order.
+template <bool kNorm>
+__device__ __forceinline__ uint32_t dkey(float d)
+{
+    const uint32_t u = __float_as_uint(d);
+    if constexpr (kNorm) return u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
+    return u;
+}
+template <bool kNorm>
+__device__ __forceinline__ float kdist(uint64_t e)
+{
+    const uint32_t k = (uint32_t)(e >> 32);
+    if constexpr (kNorm) return __uint_as_float(k ^ ((k & 0x80000000u) ? 0x80000000u : 0xFFFFFFFFu));
+    return __uint_as_float(k);
 }
 __device__ __forceinline__ uint64_t km(uint64_t e) { return e & ~kChk; }
 
@@ -278,26 +301,30 @@ __device__ __forceinline__ constexpr int blk_of(int i)
     return (nb & 1) ? (i == 0 ? nb - 1 : i - 1) : i;
 }
 
-// Squared L2 of NV base rows against the query, each computed by a lane
-// pair: the even lane owns accumulators 0-3, the odd lane 4-7 of every
-// 8-wide block; each accumulator is an fma chain in the reference's block
-// order and the horizontal sum is ((((((a0+a1)+a2)+a3)+a4)+a5)+a6)+a7.  All
-// rows' loads are issued before any is reduced.  The exact values are
-// returned in the odd lane.  qs is the query row in LDS.  Loads are
-// unconditional: callers pass a valid row (id 0) for inactive lanes and
-// discard those results, because exec-masked loads make the compiler wait
-// for each one where the branches join.
-template <int D, int NV>
-__device__ __forceinline__ void pair_dist(const float *__restrict__ base, const uint32_t (&id)[NV], int half,
-                                          const float4 *qs, float (&d)[NV])
+// Distances of NV base rows to the query, each computed by a lane pair:
+// the even lane owns accumulators 0-3, the odd lane 4-7 of every 8-wide
+// block; each accumulator is an fma chain in the reference's block order and
+// the horizontal sum is ((((((a0+a1)+a2)+a3)+a4)+a5)+a6)+a7.  kNorm = false:
+// squared L2, acc = fma(v-q, v-q, acc) (PSS5LG.cpp:525-560); kNorm = true:
+// the opt-graph distance, acc = fma(v, q, acc) then -2 * dot + norms[id]
+// (PSS5.cpp:336-390).  All rows' loads are issued before any is reduced.
+// The exact values are returned in the odd lane.  qs is the query row in
+// LDS.  Loads are unconditional: callers pass a valid row (id 0) for
+// inactive lanes and discard those results, because exec-masked loads make
+// the compiler wait for each one where the branches join.
+template <int D, int NV, bool kNorm>
+__device__ __forceinline__ void pair_dist(const float *__restrict__ base, const float *__restrict__ norms,
+                                          const uint32_t (&id)[NV], int half, const float4 *qs, float (&d)[NV])
 {
     constexpr int NB = D / 8;
     float4 v[NV][NB];
+    float nrm[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
         const float4 *r = reinterpret_cast<const float4 *>(base + (size_t)id[j] * D);
 #pragma unroll
         for (int i = 0; i < NB; ++i) v[j][i] = r[2 * blk_of<D>(i) + half];
+        if constexpr (kNorm) nrm[j] = norms[id[j]];
     }
     float acc[NV][4];
 #pragma unroll
@@ -307,11 +334,19 @@ __device__ __forceinline__ void pair_dist(const float *__restrict__ base, const 
         const float4 q = qs[2 * blk_of<D>(i) + half];
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
-            const float tx = v[j][i].x - q.x, ty = v[j][i].y - q.y, tz = v[j][i].z - q.z, tw = v[j][i].w - q.w;
-            acc[j][0] = __builtin_fmaf(tx, tx, acc[j][0]);
-            acc[j][1] = __builtin_fmaf(ty, ty, acc[j][1]);
-            acc[j][2] = __builtin_fmaf(tz, tz, acc[j][2]);
-            acc[j][3] = __builtin_fmaf(tw, tw, acc[j][3]);
+            if constexpr (kNorm) {
+                acc[j][0] = __builtin_fmaf(v[j][i].x, q.x, acc[j][0]);
+                acc[j][1] = __builtin_fmaf(v[j][i].y, q.y, acc[j][1]);
+                acc[j][2] = __builtin_fmaf(v[j][i].z, q.z, acc[j][2]);
+                acc[j][3] = __builtin_fmaf(v[j][i].w, q.w, acc[j][3]);
+            } else {
+                const float tx = v[j][i].x - q.x, ty = v[j][i].y - q.y, tz = v[j][i].z - q.z,
+                            tw = v[j][i].w - q.w;
+                acc[j][0] = __builtin_fmaf(tx, tx, acc[j][0]);
+                acc[j][1] = __builtin_fmaf(ty, ty, acc[j][1]);
+                acc[j][2] = __builtin_fmaf(tz, tz, acc[j][2]);
+                acc[j][3] = __builtin_fmaf(tw, tw, acc[j][3]);
+            }
         }
     }
 #pragma unroll
@@ -319,7 +354,36 @@ __device__ __forceinline__ void pair_dist(const float *__restrict__ base, const 
         const float sj = ((acc[j][0] + acc[j][1]) + acc[j][2]) + acc[j][3];
         const float o = __uint_as_float(dpp_mov<kDppXor1>(__float_as_uint(sj)));
         d[j] = (((o + acc[j][0]) + acc[j][1]) + acc[j][2]) + acc[j][3];
+        if constexpr (kNorm) d[j] = __builtin_fmaf(-2.0f, d[j], nrm[j]);
     }
+}
+
+// PSS5.cpp:293-334 compute_norm: one thread per base row, the same chain
+// with acc = fma(v, v, acc) over all 8 accumulators.
+template <int D>
+__global__ void norm_kernel(const float *__restrict__ base, uint32_t n, float *__restrict__ out)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    constexpr int NB = D / 8;
+    const float4 *r = reinterpret_cast<const float4 *>(base + (size_t)v * D);
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const float4 lo = r[2 * blk_of<D>(i)], hi = r[2 * blk_of<D>(i) + 1];
+        acc[0] = __builtin_fmaf(lo.x, lo.x, acc[0]);
+        acc[1] = __builtin_fmaf(lo.y, lo.y, acc[1]);
+        acc[2] = __builtin_fmaf(lo.z, lo.z, acc[2]);
+        acc[3] = __builtin_fmaf(lo.w, lo.w, acc[3]);
+        acc[4] = __builtin_fmaf(hi.x, hi.x, acc[4]);
+        acc[5] = __builtin_fmaf(hi.y, hi.y, acc[5]);
+        acc[6] = __builtin_fmaf(hi.z, hi.z, acc[6]);
+        acc[7] = __builtin_fmaf(hi.w, hi.w, acc[7]);
+    }
+    float s = acc[0];
+#pragma unroll
+    for (int j = 1; j < 8; ++j) s = s + acc[j];
+    out[v] = s;
 }
 
 __device__ __forceinline__ uint32_t lb_queue(const uint64_t *Q, uint32_t S, uint64_t x)
@@ -635,10 +699,11 @@ __device__ __forceinline__ float mlp_eval(const LaneModel &lm, const float (&f)[
 // Distances for ids[cnt0, cnt0 + 32 * NV) of ids[0, cnt) (in LDS), one
 // lane pair per id with NV rows per pair in flight; keys below `bound` are
 // appended to cand[ncand...].  Returns the new ncand.
-template <int D, int NV>
-__device__ __forceinline__ uint32_t distance_group(const float *__restrict__ base, const uint32_t *ids,
-                                                   uint32_t cnt0, uint32_t cnt, const float4 *qs, uint64_t bound,
-                                                   uint64_t *cand, uint32_t ncand, int lane, uint64_t lt)
+template <int D, int NV, bool kNorm>
+__device__ __forceinline__ uint32_t distance_group(const float *__restrict__ base, const float *__restrict__ norms,
+                                                   const uint32_t *ids, uint32_t cnt0, uint32_t cnt,
+                                                   const float4 *qs, uint64_t bound, uint64_t *cand,
+                                                   uint32_t ncand, int lane, uint64_t lt)
 {
     const int half = lane & 1;
     uint32_t id[NV];
@@ -651,10 +716,10 @@ __device__ __forceinline__ uint32_t distance_group(const float *__restrict__ bas
         id[j] = ok[j] ? r : 0u;
     }
     float d[NV];
-    pair_dist<D, NV>(base, id, half, qs, d);
+    pair_dist<D, NV, kNorm>(base, norms, id, half, qs, d);
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
-        const uint64_t kj = mk_key(d[j], id[j]);
+        const uint64_t kj = ((uint64_t)dkey<kNorm>(d[j]) << 32) | (uint64_t)id[j];
         const bool sj = ok[j] && half && kj < bound;
         const uint64_t mj = __ballot(sj);
         if (sj) cand[ncand + (uint32_t)__popcll(mj & lt)] = kj;
@@ -666,18 +731,19 @@ __device__ __forceinline__ uint32_t distance_group(const float *__restrict__ bas
 // Distances for ids[0, cnt): groups of 96 ids (three rows per lane pair)
 // while more than 64 remain, then groups of 64.  kWide = false keeps to
 // groups of 64 (fewer registers; the learned-stop kernel holds its model).
-template <int D, bool kWide>
-__device__ __forceinline__ uint32_t distance_pass(const float *__restrict__ base, const uint32_t *ids,
-                                                  uint32_t cnt, const float4 *qs, uint64_t bound,
-                                                  uint64_t *cand, uint32_t ncand, int lane, uint64_t lt)
+template <int D, bool kWide, bool kNorm>
+__device__ __forceinline__ uint32_t distance_pass(const float *__restrict__ base, const float *__restrict__ norms,
+                                                  const uint32_t *ids, uint32_t cnt, const float4 *qs,
+                                                  uint64_t bound, uint64_t *cand, uint32_t ncand, int lane,
+                                                  uint64_t lt)
 {
     uint32_t p = 0;
     while (p < cnt) {
         if (kWide && cnt - p > 64u) {
-            ncand = distance_group<D, 3>(base, ids, p, cnt, qs, bound, cand, ncand, lane, lt);
+            ncand = distance_group<D, 3, kNorm>(base, norms, ids, p, cnt, qs, bound, cand, ncand, lane, lt);
             p += 96u;
         } else {
-            ncand = distance_group<D, 2>(base, ids, p, cnt, qs, bound, cand, ncand, lane, lt);
+            ncand = distance_group<D, 2, kNorm>(base, norms, ids, p, cnt, qs, bound, cand, ncand, lane, lt);
             p += 64u;
         }
     }
@@ -705,8 +771,9 @@ __device__ __forceinline__ uint32_t next_work(const KArgs &a, uint32_t &r, uint3
 }
 
 // kLearned: the variant that evaluates the stop model (a.stop_learned);
-// the other ignores it.
-template <int D, bool kLearned>
+// the other ignores it.  kNorm: the opt-graph distance norm - 2 v.q
+// (a.norms) instead of squared L2.
+template <int D, bool kLearned, bool kNorm>
 __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
 {
     extern __shared__ uint64_t smem[];
@@ -778,8 +845,8 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
             }
             for (uint32_t c0 = 0; c0 < L; c0 += 64) {
                 const uint32_t cnt = min(64u, L - c0);
-                const uint32_t n0 = distance_pass<D, !kLearned>(a.base, a.init_ids + c0, cnt, qs, kKeyMax, cand, 0,
-                                                     lane, lt);
+                const uint32_t n0 = distance_pass<D, !kLearned, kNorm>(a.base, a.norms, a.init_ids + c0, cnt, qs,
+                                                                       kKeyMax, cand, 0, lane, lt);
                 (void)n0;
                 __syncthreads();
                 uint64_t x = (uint32_t)lane < cnt ? cand[lane] : kKeyMax;
@@ -793,7 +860,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
             S = L;
             __syncthreads();
         }
-        ep_dist = __uint_as_float((uint32_t)(Q[0] >> 32));
+        ep_dist = kdist<kNorm>(Q[0]);
         float qnorm = 0.f;
         if (a.stop_learned || a.trace_feat) {
             if (lane < D / 4) {
@@ -887,7 +954,8 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                 __syncthreads();
                 PANNS_PT(kPhClaim);
                 ndist += nnew;
-                const uint32_t ncand = distance_pass<D, !kLearned>(a.base, newids, nnew, qs, km(Q[S - 1]), cand, 0, lane, lt);
+                const uint32_t ncand = distance_pass<D, !kLearned, kNorm>(a.base, a.norms, newids, nnew, qs,
+                                                                          km(Q[S - 1]), cand, 0, lane, lt);
                 // stop feature: new keys that beat the queue's last key (a set
                 // fixed by the group, unlike the keys a batch-wise merge
                 // inserts and a later batch evicts)
@@ -915,8 +983,8 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                 for (uint32_t j = lane; j < S; j += 64) unc += (Q[j] & kChk) ? 0u : 1u;
                 unc = wave_sum(unc, xs);
                 float f[kFeat];
-                const float d1 = __uint_as_float((uint32_t)(Q[0] >> 32));
-                const float dk = __uint_as_float((uint32_t)(Q[min(K, S) - 1] >> 32));
+                const float d1 = kdist<kNorm>(Q[0]);
+                const float dk = kdist<kNorm>(Q[min(K, S) - 1]);
                 f[0] = (float)niter;
                 f[1] = d1;
                 f[2] = dk;
@@ -952,7 +1020,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
             const bool have = i < K && i < S;
             const uint64_t e = have ? Q[i] : kKeyMax;
             a.out_ids[(size_t)w * a.Kstride + i] = have ? ((uint32_t)e & kIdMask) : kSent;
-            if (a.out_dists) a.out_dists[(size_t)w * a.Kstride + i] = have ? __uint_as_float((uint32_t)(e >> 32)) : __int_as_float(0x7f800000);
+            if (a.out_dists) a.out_dists[(size_t)w * a.Kstride + i] = have ? kdist<kNorm>(e) : __int_as_float(0x7f800000);
         }
         if (a.out_qids) {
             // rows of queries with L below the batch's largest are padded
@@ -960,8 +1028,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                 const bool have = i < L;
                 const uint64_t e = have ? Q[i] : 0ull;
                 a.out_qids[(size_t)w * a.Lstride + i] = have ? ((uint32_t)e & kIdMask) : kSent;
-                a.out_qdists[(size_t)w * a.Lstride + i] = have ? __uint_as_float((uint32_t)(e >> 32))
-                                                              : __int_as_float(0x7f800000);
+                a.out_qdists[(size_t)w * a.Lstride + i] = have ? kdist<kNorm>(e) : __int_as_float(0x7f800000);
                 a.out_qchk[(size_t)w * a.Lstride + i] = (have && (e & kChk)) ? 1 : 0;
             }
         }

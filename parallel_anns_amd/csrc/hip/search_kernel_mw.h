@@ -35,7 +35,7 @@ enum : int {
     kCtlSpill = 32,   // [W] spills per wave
 };
 
-template <int D, int W>
+template <int D, int W, bool kNorm>
 __global__ __launch_bounds__(64 * W, 2) void search_kernel_mw(KArgs a)
 {
     extern __shared__ uint64_t smem[];
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(64 * W, 2) void search_kernel_mw(KArgs a)
             }
             for (uint32_t c0 = 0; c0 < L; c0 += 64) {
                 const uint32_t cnt = min(64u, L - c0);
-                distance_pass<D, true>(a.base, a.init_ids + c0, cnt, qs, kKeyMax, cand, 0, lane, lt);
+                distance_pass<D, true, kNorm>(a.base, a.norms, a.init_ids + c0, cnt, qs, kKeyMax, cand, 0, lane, lt);
                 wave_sync();
                 uint64_t x = (uint32_t)lane < cnt ? cand[lane] : kKeyMax;
                 x = sort64(x, lane, xs);
@@ -174,7 +174,8 @@ __global__ __launch_bounds__(64 * W, 2) void search_kernel_mw(KArgs a)
             ndist += nnew;
             // every wave tests against the queue's last key before this
             // iteration's merge (one group: top-L of the union)
-            const uint32_t ncand = distance_pass<D, true>(a.base, newids, nnew, qs, km(Q[S - 1]), cand, 0, lane, lt);
+            const uint32_t ncand =
+                distance_pass<D, true, kNorm>(a.base, a.norms, newids, nnew, qs, km(Q[S - 1]), cand, 0, lane, lt);
             if (lane == 0) ctrl[kCtlN + wid] = ncand;
             __syncthreads();
 
@@ -225,16 +226,14 @@ __global__ __launch_bounds__(64 * W, 2) void search_kernel_mw(KArgs a)
                 const uint64_t e = have ? Q[i] : kKeyMax;
                 a.out_ids[(size_t)w * a.Kstride + i] = have ? ((uint32_t)e & kIdMask) : kSent;
                 if (a.out_dists)
-                    a.out_dists[(size_t)w * a.Kstride + i] = have ? __uint_as_float((uint32_t)(e >> 32))
-                                                                  : __int_as_float(0x7f800000);
+                    a.out_dists[(size_t)w * a.Kstride + i] = have ? kdist<kNorm>(e) : __int_as_float(0x7f800000);
             }
             if (a.out_qids) {
                 for (uint32_t i = lane; i < a.Lstride; i += 64) {
                     const bool have = i < L;
                     const uint64_t e = have ? Q[i] : 0ull;
                     a.out_qids[(size_t)w * a.Lstride + i] = have ? ((uint32_t)e & kIdMask) : kSent;
-                    a.out_qdists[(size_t)w * a.Lstride + i] = have ? __uint_as_float((uint32_t)(e >> 32))
-                                                                  : __int_as_float(0x7f800000);
+                    a.out_qdists[(size_t)w * a.Lstride + i] = have ? kdist<kNorm>(e) : __int_as_float(0x7f800000);
                     a.out_qchk[(size_t)w * a.Lstride + i] = (have && (e & kChk)) ? 1 : 0;
                 }
             }

@@ -138,6 +138,32 @@ class Index:
     def load_common_nsg_graph(self, path):
         _chk(self.lib.panns_load_common_nsg_graph(self.h, str(path).encode()))
 
+    # the opt-graph family (PSS5.cpp:56-132): same file, norm - 2 v.q distances
+    def load_nsg_graph(self, path):
+        _chk(self.lib.panns_load_nsg_graph(self.h, str(path).encode()))
+
+    DISTANCES = {"l2": 0, "norm_dot": 1}
+
+    def set_distance(self, name):
+        """"l2" (the large-graph searches) or "norm_dot" (the opt-graph
+        searches: norm(v) - 2 v.q, PSS5.cpp:336-390)."""
+        if name not in self.DISTANCES:
+            raise SearchError(-1, f"unknown distance {name!r}")
+        _chk(self.lib.panns_set_distance(self.h, self.DISTANCES[name]))
+
+    @property
+    def distance(self):
+        v = _native.c_int()
+        _chk(self.lib.panns_get_distance(self.h, v))
+        return {0: "l2", 1: "norm_dot"}[v.value]
+
+    def norms(self):
+        """Every base row's norm, compute_norm (PSS5.cpp:293-334)."""
+        n = self.info()["num_v"]
+        out = np.empty(n, dtype=np.float32)
+        _chk(self.lib.panns_get_norms(self.h, ptr(out)))
+        return out
+
     def set_base(self, data):
         data = np.ascontiguousarray(data, dtype=np.float32)
         _chk(self.lib.panns_set_base(self.h, ptr(data), data.shape[0], data.shape[1]))

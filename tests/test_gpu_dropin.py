@@ -45,7 +45,8 @@ def _read_result(path):
     return np.stack(rows)
 
 
-@pytest.mark.parametrize("how,M", [("per_query", 4), ("per_query", 1), ("batch", 8), ("pss", 1)])
+@pytest.mark.parametrize("how,M", [("per_query", 4), ("per_query", 1), ("batch", 8), ("pss", 1), ("opt", 4),
+                                   ("opt_pss", 1)])
 def test_find_L_driver(tiny_set, tmp_path, how, M):
     assert os.path.exists(DRIVER), "build the drop-in driver first (python -m parallel_anns_amd.build)"
     ds = tiny_set
@@ -55,7 +56,9 @@ def test_find_L_driver(tiny_set, tmp_path, how, M):
     res_path = os.path.join(str(tmp_path), "result.bin")
     env = dict(os.environ)
     env["PANNS_BATCH"] = "1" if how == "batch" else "0"
-    env["PANNS_PSS"] = "1" if how == "pss" else "0"
+    env["PANNS_PSS"] = "1" if how in ("pss", "opt_pss") else "0"
+    # the opt-graph family: load_nsg_graph, norm - 2 v.q distances
+    env["PANNS_OPT_GRAPH"] = "1" if how.startswith("opt") else "0"
     # L_lower = L_upper = L: exactly one search at L
     cmd = [DRIVER, p["base"], p["query"], p["graph"], str(L), str(K), res_path, str(M), p["truth"], "1",
            str(L), "0.5"]
@@ -63,7 +66,7 @@ def test_find_L_driver(tiny_set, tmp_path, how, M):
     assert r.returncode == 0, r.stderr
     got = _read_result(res_path)
     assert got.shape == (nq, K)
-    g = po.CsrGraph(ds["adj"], ds["deg"], ds["ep"])
+    g = po.CsrGraph(ds["adj"], ds["deg"], ds["ep"], base=ds["base"] if how.startswith("opt") else None)
     init = po.prepare_init_ids(g, L)
     for q in range(nq):
         ref = po.search_one(g, ds["base"], ds["queries"][q], K, L, M, init)

@@ -61,9 +61,13 @@ def small_graph(n=300, dim=16, R=8, seed=0, comps=6):
     return base, queries, adj, deg, ep
 
 
-def set_model(adj, deg, base, q, L, M, init_ids, scale_M=None, queue=None, visited=None):
-    """Appendix-A semantics with sets; returns (queue [(d, id, checked)], ndist, iters, visited)."""
-    dist = lambda v: np.float32(po.distance(base[v], q))
+def set_model(adj, deg, base, q, L, M, init_ids, scale_M=None, queue=None, visited=None, norms=None):
+    """The search's semantics with sets; returns (queue [(d, id, checked)],
+    ndist, iters, visited).  norms: the opt-graph distance norm - 2 v.q."""
+    if norms is None:
+        dist = lambda v: np.float32(po.distance(base[v], q))
+    else:
+        dist = lambda v: np.float32(po.distance_with_norm(base[v], q, norms[v]))
     if queue is None:
         visited = set(int(v) for v in init_ids)
         Q = sorted([(dist(v), int(v), False) for v in init_ids[:L]], key=lambda e: (e[0], e[1]))
@@ -148,6 +152,63 @@ def test_scale_m_subsearch_equals_set_semantics(mmax):
         assert nd == nd_m
         bits = np.unpackbits(visited.view(np.uint8), bitorder="little")[:n]
         assert set(np.nonzero(bits)[0].tolist()) == vis_m
+
+
+# ----------------------------------------------------------------------------
+# opt-graph family (PSS5 / TOPM): the same searches with norm - 2 v.q
+@pytest.mark.parametrize("M", [1, 4])
+def test_norm_dot_top_m_equals_set_semantics(M):
+    base, queries, adj, deg, ep = small_graph(seed=6)
+    g = po.CsrGraph(adj, deg, ep, base=base)
+    L, K = 40, 20
+    init = po.prepare_init_ids(g, L)
+    negative = 0
+    for qi in range(queries.shape[0]):
+        r = po.search_one(g, base, queries[qi], K, L, M, init)
+        Q, nd, it, _ = set_model(adj, deg, base, queries[qi], L, M, init, norms=g.norms)
+        assert [e[1] for e in Q] == list(r["queue"]["id"])
+        assert [e[0] for e in Q] == list(r["queue"]["dist"])
+        assert nd == r["ndist"] and it == r["merges"]
+        negative += int((r["queue"]["dist"] < 0).sum())
+    assert negative > 0  # the ordering of negative distances is exercised
+
+
+def test_norm_dot_scale_m_equals_set_semantics():
+    base, queries, adj, deg, ep = small_graph(seed=7)
+    g = po.CsrGraph(adj, deg, ep, base=base)
+    L, n = 36, base.shape[0]
+    init = po.prepare_init_ids(g, L)
+    for qi in range(6):
+        set_L, visited, _ = po.initialize_set_L(base, queries[qi], L, init, n, norms=g.norms)
+        queue = [(np.float32(d), int(i), bool(c)) for i, d, c in zip(set_L["id"], set_L["dist"], set_L["checked"])]
+        nd = po.subsearch_with_top_m(g, base, queries[qi], 8, L, set_L, visited)
+        Q, nd_m, _, _ = set_model(adj, deg, base, queries[qi], L, None, None, scale_M=8, queue=queue,
+                                  visited=set(int(x) for x in init), norms=g.norms)
+        assert [e[1] for e in Q] == list(set_L["id"]) and nd == nd_m
+
+
+@pytest.mark.parametrize("mode,M", [("top_m", 1), ("top_m", 4), ("pss_v5", 1)])
+def test_norm_dot_search_on_integer_data_equals_l2_search(mode, M):
+    """On small-integer vectors every distance is exact, and norm - 2 v.q is
+    |v - q|^2 - |q|^2: the same order (ties included) as squared L2, so both
+    families must return the same ids, counts and iterations, with distances
+    shifted by |q|^2 exactly."""
+    base, queries, adj, deg, ep = small_graph(seed=8)
+    base = np.round(base * 12).astype(np.float32)
+    queries = np.round(queries * 12).astype(np.float32)
+    g2 = po.CsrGraph(adj, deg, ep)
+    gn = po.CsrGraph(adj, deg, ep, base=base)
+    L, K = 48, 16
+    init = po.prepare_init_ids(g2, L)
+    for qi in range(queries.shape[0]):
+        a = po.search_one(g2, base, queries[qi], K, L, M, init, mode=mode, X=3)
+        b = po.search_one(gn, base, queries[qi], K, L, M, init, mode=mode, X=3)
+        assert (a["queue"]["id"] == b["queue"]["id"]).all()
+        assert (a["ids"] == b["ids"]).all()
+        assert (a["ndist"], a["merges"], a["nexp"]) == (b["ndist"], b["merges"], b["nexp"])
+        q2 = float((queries[qi].astype(np.int64) ** 2).sum())
+        assert (b["queue"]["dist"] == a["queue"]["dist"] - np.float32(q2)).all()
+        assert (b["queue"]["dist"] < 0).any()
 
 
 def test_L_equals_N_is_exact_brute_force():
