@@ -1,0 +1,194 @@
+"""The combined workload at scale: 50M base, uneven L, learned stop vs fixed.
+
+  python -m parallel_anns_amd.at_scale [--config c4] [--M 4] [--steps 3]
+                                       [--train-queries 20000] [--val-queries 5000]
+                                       [--target-drop 0.01]
+
+Workload c4: 50,000,000 x 96 float32 (19.2 GB) from the seeded normalised
+1,024-component mixture (sigma 0.3); graph out-degree <= 32 from exact 64-NN
+lists computed per mixture component on the GPU (checked against global
+brute force on sampled rows) and the RNG prune with DFS stitch; 100,000
+queries, each with L drawn log-uniformly from [100, 2000], K = 100.  Ground
+truth is exact brute force over the whole base set.
+
+Two searches of the same batch, inputs resident in HBM, bucketed by the
+autotuner:
+
+  fixed     the plain search, every query at its own L
+  learned   the same with the in-loop stop model, each query's L as its cap
+
+The stop model is trained on traces of seed-disjoint training queries
+(grouped into L bins, each traced at its bin's median L), and its threshold
+is calibrated on held-out training queries so that their recall@100 stays
+within --target-drop of the fixed search's.  Prints one JSON line with both
+searches' QPS, recall@100 and algorithmic HBM rate against the 8 TB/s peak.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[at_scale {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(res, dim, K, nq):
+    """Bytes a search must move: one row per distance, one adjacency row
+    (its read slots) per expansion, the query rows and the results."""
+    c = res["counters"]
+    return (4.0 * dim * c["distance_computations"] + 4.0 * (c["expansions"] + c["adjacency_slots"])
+            + nq * (4.0 * dim + 8.0 * K))
+
+
+def recall_rows(gt, ids, R=100):
+    """Per-query |truth[:R] & ids[:R]| / R."""
+    out = np.empty(ids.shape[0])
+    for q in range(ids.shape[0]):
+        out[q] = np.intersect1d(gt[q, :R], ids[q, :R]).shape[0] / R
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="c4")
+    ap.add_argument("--cache-dir", default=None)
+    ap.add_argument("--M", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--train-queries", type=int, default=20_000)
+    ap.add_argument("--val-queries", type=int, default=5_000)
+    ap.add_argument("--train-bins", type=int, default=6)
+    ap.add_argument("--target-drop", type=float, default=0.01)
+    args = ap.parse_args()
+
+    from . import datasets, learned
+    from .search import Index
+
+    t_all = time.time()
+    ds = datasets.prepare(args.config, cache_dir=args.cache_dir, gpu_knn=datasets.GpuKnn(0))
+    prep_s = time.time() - t_all
+    cfg = ds["cfg"]
+    K, M = cfg["K"], args.M
+    nq, dim = ds["queries"].shape[0], ds["base"].shape[1]
+    Ls, _ = datasets.uneven_L_K(cfg, nq)
+    maxL = int(Ls.max())
+    qids = np.arange(nq, dtype=np.uint32)
+    idx = Index(0)
+    idx.set_base(ds["base"])
+    idx.set_queries(ds["queries"])
+    idx.set_graph(ds["adj"], ds["deg"], ds["ep"])
+    gt = ds["gt_ids"]
+    base_gb = ds["base"].nbytes / 1e9
+    graph_gb = ds["adj"].nbytes / 1e9
+    del ds["base"]
+    log(f"resident: base {base_gb:.1f} GB, adjacency {graph_gb:.1f} GB; prepared in {prep_s:.0f}s")
+    init = idx.prepare_init_ids(maxL)
+
+    def run(name, **kw):
+        t0 = time.time()
+        opt, _ = idx.autotune(qids, K, maxL, M, init_ids=init, L_per_query=Ls, **kw)
+        pl = idx.plan(qids, K, maxL, M, init_ids=init, L_per_query=Ls, options=opt, want_queue=False, **kw)
+        pl.run()
+        ms = [pl.run() for _ in range(args.steps)]
+        res = pl.fetch()
+        nb = pl.buckets()
+        pl.close()
+        kms = float(np.mean(ms))
+        B = algorithmic_bytes(res, dim, K, nq)
+        rec = recall_rows(gt, res["ids"])
+        info = {"kernel_ms": round(kms, 3), "qps": round(nq / (kms * 1e-3), 1), "recall@100": round(float(rec.mean()), 4),
+                "ndist_mean": round(float(res["ndist"].mean()), 1),
+                "algorithmic_bytes_per_launch": int(B), "achieved_GBps": round(B / (kms * 1e-3) / 1e9, 1),
+                "hbm_frac": round(B / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "launches": len(nb),
+                "table_eighths": opt.table_eighths, "concurrent": bool(opt.concurrent),
+                "tune_and_time_s": round(time.time() - t0, 1)}
+        log(f"{name}: {info}")
+        return res, rec, info
+
+    fres, frec, fixed = run("fixed")
+
+    # ---- stop model: traces of seed-disjoint training queries, by L bin
+    t0 = time.time()
+    ntr, nval = args.train_queries, args.val_queries
+    tq = datasets.generate(cfg, "train", n=ntr + nval)
+    tL, _ = datasets.uneven_L_K(cfg, ntr + nval, seed=12)
+    idx.set_queries(tq[:ntr])
+    edges = np.geomspace(cfg["L_range"][0], cfg["L_range"][1] + 1, args.train_bins + 1)
+    Xs, Ys, firsts = [], [], []
+    for b in range(args.train_bins):
+        sel = np.nonzero((tL[:ntr] >= edges[b]) & (tL[:ntr] < edges[b + 1]))[0].astype(np.uint32)
+        if sel.shape[0] == 0:
+            continue
+        Lb = int(np.median(tL[sel]))
+        cap = min(4096, 2 * Lb // M + 64)
+        feat, hsh, niter, _ = idx.search_trace(sel, K, Lb, M, trace_cap=cap)
+        X, Y = learned.samples_from_trace(feat, hsh, niter)
+        Xs.append(X)
+        Ys.append(Y)
+        firsts.append(learned.first_final_iteration(hsh, niter, 0.01))
+        log(f"traced {sel.shape[0]} training queries at L={Lb}: {X.shape[0]:,} steps, "
+            f"max iterations {int(niter.max())} (cap {cap})")
+        del feat, hsh
+    X, Y = np.concatenate(Xs), np.concatenate(Ys)
+    min_iter = int(min(firsts))
+    model, losses = learned.train(idx, X, Y, epochs=4, batch=4096, lr=2e-3, seed=0)
+    train_info = {"train_queries": ntr, "samples": int(X.shape[0]), "positive_frac": round(float(Y.mean()), 4),
+                  "epoch_loss": [round(float(x), 5) for x in losses], "L_bins": args.train_bins,
+                  "stop_min_iter": min_iter, "seconds": round(time.time() - t0, 1)}
+    log(f"trained: {train_info}")
+    del X, Y, Xs, Ys
+
+    # ---- threshold on held-out training queries (their own exact truth)
+    idx.set_queries(tq[ntr:])
+    vgt, _ = idx.exact_knn(K, source="queries")
+    vq = np.arange(nval, dtype=np.uint32)
+    vL = tL[ntr:]
+    vmax = int(vL.max())
+    idx.set_stop_model(model)
+    rf, _, ms_f = learned.recall_work(idx, vq, vgt, K, vmax, M, None, runs=3, L_per_query=vL)
+    target = rf - args.target_drop
+    best = (ms_f, None, {})
+    for every in (1, 2):
+        kw = dict(stop_min_iter=min_iter, stop_every=every)
+        thr = learned.calibrate(idx, vq, vgt, K, vmax, M, target, L_per_query=vL, **kw)
+        if thr is None:
+            continue
+        rv, nd, ms = learned.recall_work(idx, vq, vgt, K, vmax, M, thr, runs=3, L_per_query=vL, **kw)
+        log(f"validation: every {every} threshold {thr:.4g}: recall {rv:.4f} (fixed {rf:.4f}), "
+            f"{ms:.3f} ms (fixed {ms_f:.3f} ms)")
+        if ms < best[0]:
+            best = (ms, thr, kw)
+    idx.set_queries(ds["queries"])
+
+    out_l = None
+    if best[1] is not None:
+        thr, kw = best[1], best[2]
+        lres, lrec, out_l = run("learned", stop="learned", stop_threshold=thr, **kw)
+        d = frec - lrec
+        out_l.update({"threshold": thr, "stop_every": kw["stop_every"], "stop_min_iter": kw["stop_min_iter"],
+                      "recall_deficit_vs_fixed": {"mean": round(float(d.mean()), 5),
+                                                  "frac_queries_with_loss": round(float((d > 0).mean()), 4),
+                                                  "p99": round(float(np.quantile(d, 0.99)), 3)}})
+    else:
+        log("no threshold beats the fixed search on validation; learned = fixed")
+
+    out = {"workload": f"{args.config}: {cfg['n']:,} x {dim} fp32 ({base_gb:.1f} GB), {cfg['comps']} components, "
+                       f"R<={cfg['R']} ({graph_gb:.1f} GB adjacency), {nq:,} queries, L log-uniform "
+                       f"{cfg['L_range']}, K={K}, M={M}",
+           "mean_L": round(float(Ls.mean()), 1),
+           "fixed": fixed, "learned": out_l,
+           "learned_over_fixed": round(out_l["qps"] / fixed["qps"], 3) if out_l else 1.0,
+           "validation": {"queries": nval, "fixed_recall": round(rf, 4), "target": round(target, 4)},
+           "training": train_info,
+           "hbm_peak_GBps": HBM_PEAK_GBS,
+           "prepare_s": round(prep_s, 1), "total_s": round(time.time() - t_all, 1)}
+    print(json.dumps(out), flush=True)
+    idx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -28,6 +28,13 @@ CONFIGS = {
     # global brute force on a sample of rows)
     "c2": dict(n=10_000_000, nq=100_000, dim=96, comps=256, sigma=0.3, knn=64, R=32, K=100, L=100,
                knn_by_component=True, L_range=(100, 2000), K_choices=(10, 100)),
+    # C4: 50M base (19.2 GB) from 1,024 components, 100k queries with L
+    # log-uniform in [100, 2000] and K = 100; exact 64-NN per component
+    "c4": dict(n=50_000_000, nq=100_000, dim=96, comps=1024, sigma=0.3, knn=64, R=32, K=100, L=100,
+               knn_by_component=True, L_range=(100, 2000), K_choices=(100,)),
+    # a 1M-row rehearsal of C4's pipeline (per-component k-NN, uneven L)
+    "c4s": dict(n=1_000_000, nq=10_000, dim=96, comps=64, sigma=0.3, knn=64, R=32, K=100, L=100,
+                knn_by_component=True, L_range=(100, 2000), K_choices=(100,)),
     # small cases for tests
     "t20k": dict(n=20_000, nq=200, dim=96, comps=64, sigma=0.3, knn=64, R=32, K=100, L=100),
     "t5k": dict(n=5_000, nq=64, dim=96, comps=16, sigma=0.3, knn=32, R=24, K=50, L=64),

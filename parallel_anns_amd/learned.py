@@ -82,21 +82,25 @@ def first_final_iteration(topk_hash, niter, quantile=0.01):
     return int(np.quantile(np.asarray(firsts), quantile)) if firsts else 0
 
 
-def recall_work(index, qids, gt, K, L, M, threshold, runs=1, **stop_kw):
+def recall_work(index, qids, gt, K, L, M, threshold, runs=1, L_per_query=None, options=None, **stop_kw):
     """(recall@K, mean distance computations, kernel ms -- the least of
     `runs` launches) of a learned-stop search; stop_kw: stop_min_iter,
-    stop_every.  threshold None runs the fixed-L search."""
+    stop_every.  threshold None runs the fixed-L search.  L_per_query: each
+    query's own L (then L is their maximum and recall is counted over the
+    first min(K, smallest L) ids); options: plan options (buckets)."""
     kw = dict(stop="learned", stop_threshold=threshold, **stop_kw) if threshold is not None else {}
-    plan = index.plan(qids, K, L, M, want_queue=False, **kw)
+    plan = index.plan(qids, K, L, M, want_queue=False, L_per_query=L_per_query, options=options, **kw)
     ms = min(plan.run() for _ in range(runs))
     res = plan.fetch()
     plan.close()
-    return tools.recall(gt, res["ids"], L, R=min(K, gt.shape[1])), float(res["ndist"].mean()), ms
+    Lr = L if L_per_query is None else int(np.min(L_per_query))
+    return tools.recall(gt, res["ids"], Lr, R=min(K, gt.shape[1])), float(res["ndist"].mean()), ms
 
 
 def calibrate(index, qids, gt, K, L, M, target, lo=1e-4, hi=1.0, steps=14, **stop_kw):
     """Largest-saving threshold whose recall on (qids, gt) reaches target: the
-    recall rises with the threshold, so bisect on it (log scale)."""
+    recall rises with the threshold, so bisect on it (log scale).  stop_kw
+    also passes L_per_query and options through to recall_work."""
     r_hi = recall_work(index, qids, gt, K, L, M, hi, **stop_kw)[0]
     if r_hi < target:
         return None
