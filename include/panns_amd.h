@@ -73,7 +73,8 @@ typedef struct {
                               results equal the single-wave search */
     uint32_t stop_min_iter;  /* LEARNED: the model runs from this iteration on (0: from the first) */
     uint32_t stop_every;     /* LEARNED: ... at every stop_every-th iteration (0 or 1: every one) */
-    uint32_t reserved[1];
+    uint32_t bucket_slots;   /* visited-table buckets of 8 or 4 slots; 0 = automatic (8, or 4 when the
+                                id space needs more buckets than an 8-slot table of that size has) */
 } panns_params;
 
 typedef struct {

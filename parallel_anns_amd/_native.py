@@ -29,7 +29,7 @@ class PannsParams(ctypes.Structure):
     _fields_ = [("K", c_u32), ("L", c_u32), ("M", c_u32), ("mode", c_u32),
                 ("num_threads", c_u32), ("stop", c_u32), ("stop_threshold", c_float),
                 ("hash_slots", c_u32), ("waves", c_u32), ("stop_min_iter", c_u32), ("stop_every", c_u32),
-                ("reserved", c_u32 * 1)]
+                ("bucket_slots", c_u32)]
 
 
 class PannsCounters(ctypes.Structure):

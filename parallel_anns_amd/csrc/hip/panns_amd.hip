@@ -127,7 +127,7 @@ struct panns_plan {
     uint32_t trace_cap = 0;
     bool want_queue = false;
     uint32_t *shared_bitmap = nullptr;
-    uint32_t grid = 0, lds = 0, hash_slots = 0;
+    uint32_t grid = 0, lds = 0, hash_slots = 0, bucket_slots = 8;
     uint32_t table_eighths = 8;
     uint32_t waves = 1;  // wavefronts per query
     // work item i is the caller's query perm[i] (empty: the caller's order)
@@ -291,6 +291,8 @@ int check_params(panns_index *h, const panns_params *p, uint32_t L, uint32_t K)
         return fail(PANNS_ERR_UNSUPPORTED, "the learned stop's features are squared-L2 distances; use the fixed stop");
     if (p->hash_slots && (p->hash_slots % 64 || p->hash_slots < 64 || p->hash_slots > 32768))
         return fail(PANNS_ERR_ARG, "hash_slots must be a multiple of 64 in [64, 32768]");
+    if (p->bucket_slots && p->bucket_slots != 4 && p->bucket_slots != 8)
+        return fail(PANNS_ERR_ARG, "bucket_slots must be 0 (automatic), 4 or 8");
     if (p->waves > 1) {
         if (p->waves != 2 && p->waves != 4) return fail(PANNS_ERR_ARG, "waves must be 1, 2 or 4");
         if (p->mode == PANNS_MODE_PSS_V5) return fail(PANNS_ERR_UNSUPPORTED, "PSS_V5 expands one candidate per step; use waves = 1");
@@ -444,13 +446,19 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     if (!hs) hs = ((pl->maxL <= 512 ? 8192u : 10240u) * pl->table_eighths / 8u + 63u) & ~63u;
     const uint32_t id_bits = std::max<uint32_t>(1u, log2u(h->n));
     const uint64_t min_nb = (((1ull << id_bits) - 1ull) / 0x7FFEull) + 1ull;
-    uint32_t nb = std::max<uint32_t>(hs / 8u, (uint32_t)((min_nb + 7ull) & ~7ull));
+    // buckets of 8 slots, or of 4 when an 8-slot table of this size has
+    // fewer buckets than the id space needs (ids above 2^25 at 8,192 slots):
+    // the table then keeps its size instead of growing with N
+    uint32_t bslots = p.bucket_slots ? p.bucket_slots : ((hs / 8u) >= min_nb ? 8u : 4u);
+    uint32_t nb = std::max<uint32_t>(hs / bslots, (uint32_t)((min_nb + 7ull) & ~7ull));
     nb = std::max<uint32_t>(nb, 8u);
-    hs = nb * 8u;
+    hs = nb * bslots;
     if (pl->shared_bitmap) {
         hs = 64;
         nb = 8;
+        bslots = 8;
     }
+    pl->bucket_slots = bslots;
     pl->hash_slots = hs;
     // Lcap rounded to 8 keys keeps every LDS region 16-byte aligned.
     const uint32_t Lcap = (pl->maxL + 7u) & ~7u;
@@ -557,6 +565,7 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     a.Lcap = Lcap;
     a.tbl_words = hs / 2u;
     a.tbl_buckets = nb;
+    a.tbl_bshift = bslots == 8u ? 2u : 1u;
     a.tbl_magic = (uint32_t)((1ull << 32) / nb);
     a.id_bits = id_bits;
     a.use_table = pl->shared_bitmap ? 0u : 1u;

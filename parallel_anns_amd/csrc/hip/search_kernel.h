@@ -92,6 +92,7 @@ struct KArgs {
     uint32_t log_cap;
     uint32_t Lcap;
     uint32_t tbl_words, tbl_buckets, tbl_magic, id_bits, use_table;  // LDS visited table
+    uint32_t tbl_bshift;  // log2 of 32-bit words per bucket: 2 (8 slots) or 1 (4 slots)
     uint32_t Mcap, nb_cap, group_picks;
     // subsearch on a caller-held queue and visited bitmap
     const uint64_t *init_queue;  // [n_work][L] keys, or null
@@ -489,8 +490,9 @@ __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32
 // bitmap, and an id is never both in the table and in the bitmap.  spills is
 // a wave-uniform count kept in registers.
 struct Visited {
-    uint32_t *tbl;      // LDS: buckets x 4 words
+    uint32_t *tbl;      // LDS: buckets x (1 << bshift) words
     uint32_t nb, magic, idbits, idmask, xshift, use;  // buckets, floor(2^32 / nb)
+    uint32_t bshift;    // 2: buckets of 8 slots, 1: of 4 slots
     uint32_t *bitmap;   // global
     uint32_t *log;      // global: words touched in the bitmap, may be null
     uint32_t log_cap;
@@ -502,20 +504,31 @@ struct Visited {
         return h ^ (h >> xshift);
     }
 
-    // One bucket: 0 if `tag` is there, 1 if it was inserted, 2 if the
-    // bucket is full without it.  Every failed CAS means another slot of the
-    // bucket was filled, so at most 8 retries precede a full bucket or a
-    // success.
-    __device__ __forceinline__ int probe_bucket(uint32_t *w, uint32_t tag) const
+    // One bucket of NW words (2 slots each): 0 if `tag` is there, 1 if it
+    // was inserted, 2 if the bucket is full without it.  Every failed CAS
+    // means another slot of the bucket was filled, so at most 2 * NW retries
+    // precede a full bucket or a success.
+    template <int NW>
+    __device__ __forceinline__ int probe_words(uint32_t *w, uint32_t tag) const
     {
 #pragma unroll 1
-        for (int attempt = 0; attempt < 9; ++attempt) {
-            const uint4 v = *reinterpret_cast<const uint4 *>(w);
-            const uint32_t words[4] = {v.x, v.y, v.z, v.w};
+        for (int attempt = 0; attempt <= 2 * NW; ++attempt) {
+            uint32_t words[NW];
+            if constexpr (NW == 4) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(w);
+                words[0] = v.x;
+                words[1] = v.y;
+                words[2] = v.z;
+                words[3] = v.w;
+            } else {
+                const uint2 v = *reinterpret_cast<const uint2 *>(w);
+                words[0] = v.x;
+                words[1] = v.y;
+            }
             int ew = -1;
             uint32_t eshift = 0, eold = 0;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < NW; ++i) {
                 const uint32_t lo = words[i] & 0xFFFFu, hi = words[i] >> 16;
                 if (lo == tag || hi == tag) return 0;
                 if (ew < 0 && (lo == 0xFFFFu || hi == 0xFFFFu)) {
@@ -530,9 +543,15 @@ struct Visited {
         }
         return 2;
     }
+    __device__ __forceinline__ int probe_bucket(uint32_t b, uint32_t tag) const
+    {
+        return bshift == 2 ? probe_words<4>(tbl + (b << 2), tag) : probe_words<2>(tbl + (b << 1), tag);
+    }
 
     // 0: visited before; 1: new and now in the table; 2: in neither of its
-    // two buckets, both full -- the bitmap decides.
+    // two buckets, both full -- the bitmap decides.  Buckets hold 8 slots,
+    // or 4 where the id space needs more buckets than an 8-slot table of the
+    // chosen size has (the tag must stay below 2^15).
     // With h = scramble(id) = t * nb + b1, the id's primary bucket is b1 and
     // its tag t (< 2^15: the host keeps nb large enough).  Only when b1 is
     // full does the id go to its second bucket b2 = (b1 + alt(t)) mod nb,
@@ -552,11 +571,11 @@ struct Visited {
             b1 -= nb;
             ++tag;
         }
-        const int r = probe_bucket(tbl + (b1 << 2), tag);
+        const int r = probe_bucket(b1, tag);
         if (r != 2 || tag == 0x7FFFu) return r;
         uint32_t b2 = b1 + 1u + __umulhi(tag * 0x85EBCA77u, nb - 1u);
         if (b2 >= nb) b2 -= nb;
-        return probe_bucket(tbl + (b2 << 2), tag | 0x8000u);
+        return probe_bucket(b2, tag | 0x8000u);
     }
 };
 
@@ -796,6 +815,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
     vis.tbl = tbl;
     vis.nb = a.tbl_buckets;
     vis.magic = a.tbl_magic;
+    vis.bshift = a.tbl_bshift;
     vis.idbits = a.id_bits;
     vis.idmask = a.id_bits >= 32 ? ~0u : (1u << a.id_bits) - 1u;
     vis.xshift = a.id_bits / 2 + 1;

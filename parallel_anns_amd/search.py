@@ -27,7 +27,7 @@ def _chk(rc):
 
 
 def make_params(K, L, M=1, mode="top_m", num_threads=1, stop="fixed", stop_threshold=0.5,
-                hash_slots=0, waves=1, stop_min_iter=0, stop_every=1):
+                hash_slots=0, waves=1, stop_min_iter=0, stop_every=1, bucket_slots=0):
     p = PannsParams()
     p.K, p.L, p.M = K, L, M
     p.mode = MODES[mode] if isinstance(mode, str) else int(mode)
@@ -37,6 +37,7 @@ def make_params(K, L, M=1, mode="top_m", num_threads=1, stop="fixed", stop_thres
     p.hash_slots = hash_slots
     p.waves = waves
     p.stop_min_iter, p.stop_every = stop_min_iter, stop_every
+    p.bucket_slots = bucket_slots
     return p
 
 
@@ -193,7 +194,7 @@ class Index:
 
     def search(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, want_queue=False,
                num_threads=1, stop="fixed", stop_threshold=0.5, hash_slots=0, waves=1, stop_min_iter=0,
-               stop_every=1):
+               stop_every=1, bucket_slots=0):
         """Batch form of para_search_with_top_m_pure_large_graph /
         para_search_PSS_v5_large_graph_dist_thresh_profiling (num_threads_=1)."""
         qids = np.ascontiguousarray(np.atleast_1d(query_ids), dtype=np.uint32)
@@ -202,7 +203,7 @@ class Index:
             init_ids = self.prepare_init_ids(L)
         init_ids = np.ascontiguousarray(init_ids, dtype=np.uint32)
         p = make_params(K, L, M, mode, num_threads, stop, stop_threshold, hash_slots, waves, stop_min_iter,
-                        stop_every)
+                        stop_every, bucket_slots)
         res = {"ids": np.empty((nq, K), np.uint32), "dists": np.empty((nq, K), np.float32),
                "ndist": np.empty(nq, np.uint32), "nexp": np.empty(nq, np.uint32),
                "niter": np.empty(nq, np.uint32)}
@@ -241,7 +242,7 @@ class Index:
 
     def plan(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, L_per_query=None,
              K_per_query=None, stop="fixed", stop_threshold=0.5, hash_slots=0, options=None,
-             want_queue=True, waves=1, stop_min_iter=0, stop_every=1):
+             want_queue=True, waves=1, stop_min_iter=0, stop_every=1, bucket_slots=0):
         """A resident batch (panns_plan_create_ex).  options: a
         PannsPlanOptions (plan_options() or autotune()) for queue-size
         buckets, graph replay and side-by-side buckets; want_queue keeps the
@@ -250,7 +251,8 @@ class Index:
         if init_ids is None:
             init_ids = self.prepare_init_ids(maxL)
         init_ids = np.ascontiguousarray(init_ids, dtype=np.uint32)
-        p = make_params(K, L, M, mode, 1, stop, stop_threshold, hash_slots, waves, stop_min_iter, stop_every)
+        p = make_params(K, L, M, mode, 1, stop, stop_threshold, hash_slots, waves, stop_min_iter, stop_every,
+                        bucket_slots)
         o = _native.PannsPlanOptions() if options is None else options
         o.want_queue = int(want_queue)
         h = _native.vp()

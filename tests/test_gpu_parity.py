@@ -100,6 +100,31 @@ def test_small_tables_spill_and_keep_results(small_set, gpu_index, slots):
     assert (again["ids"] == res["ids"]).all() and (again["ndist"] == res["ndist"]).all()
 
 
+@pytest.mark.parametrize("slots", [64, 192, 1536, 8192])
+def test_four_slot_buckets_keep_results(small_set, gpu_index, slots):
+    """Visited tables of 4-slot buckets (the automatic choice for id spaces
+    above 2^25 at the default size, forced here): second buckets and
+    spills, results those of the oracle."""
+    ds = small_set
+    K, L, M = 100, 160, 4
+    qids = np.arange(48, dtype=np.uint32)
+    _print_case(test="four_slot_buckets", hash_slots=slots)
+    init = gpu_index.prepare_init_ids(L)
+    res = gpu_index.search(qids, K, L, M, init_ids=init, hash_slots=slots, bucket_slots=4, want_queue=True)
+    if slots < 1536:
+        assert res["counters"]["visited_spills"] > 0
+    ref = oracle_run(ds, qids, K, L, M, init)
+    for i in range(qids.shape[0]):
+        assert (res["ids"][i] == ref[i]["ids"]).all(), f"query {i}"
+        assert (res["queue_ids"][i] == ref[i]["queue"]["id"]).all()
+        assert res["ndist"][i] == ref[i]["ndist"]
+    for w in (2, 4):
+        mw = gpu_index.search(qids, K, L, M, init_ids=init, hash_slots=slots, bucket_slots=4, waves=w)
+        assert (mw["ids"] == res["ids"]).all() and (mw["ndist"] == res["ndist"]).all()
+    with pytest.raises(SearchError):
+        gpu_index.search(qids[:2], K, L, M, init_ids=init, bucket_slots=6)
+
+
 def test_uneven_plan_matches_uniform(small_set, gpu_index):
     ds = small_set
     rng = np.random.default_rng(5)

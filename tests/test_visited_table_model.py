@@ -12,15 +12,17 @@ EMPTY = 0xFFFF
 
 
 class TableModel:
-    def __init__(self, slots, n):
+    def __init__(self, slots, n, bucket_slots=0):
         self.id_bits = max(1, int(np.ceil(np.log2(n)))) if n > 1 else 1
         min_nb = ((1 << self.id_bits) - 1) // 0x7FFE + 1
-        nb = max(slots // 8, (min_nb + 7) // 8 * 8, 8)
-        self.nb = nb
+        # 8-slot buckets, or 4 when an 8-slot table this size has too few
+        bs = bucket_slots or (8 if slots // 8 >= min_nb else 4)
+        nb = max(slots // bs, (min_nb + 7) // 8 * 8, 8)
+        self.nb, self.bucket_slots = nb, bs
         self.magic = (1 << 32) // nb
         self.idmask = (1 << self.id_bits) - 1
         self.xshift = self.id_bits // 2 + 1
-        self.buckets = [[EMPTY] * 8 for _ in range(nb)]
+        self.buckets = [[EMPTY] * bs for _ in range(nb)]
         self.bitmap = set()
 
     def scramble(self, i):
@@ -92,3 +94,29 @@ def test_second_buckets_take_the_overflow():
     for i in ids:
         t.claim(i)
     assert len(t.bitmap) < 0.01 * len(ids)
+
+
+@pytest.mark.parametrize("slots,n,seed", [(64, 20000, 6), (1536, 1_000_000, 7), (8192, 50_000_000, 8),
+                                          (4096, 10_000_000, 9)])
+def test_four_slot_buckets_membership_equals_a_set(slots, n, seed):
+    rng = np.random.default_rng(seed)
+    t = TableModel(slots, n, bucket_slots=4)
+    assert t.bucket_slots == 4
+    seen = set()
+    ids = np.concatenate([rng.integers(0, n, 6000), rng.integers(0, n, 200).repeat(4)])
+    rng.shuffle(ids)
+    for i in ids.tolist():
+        assert t.claim(i) == (i not in seen), i
+        seen.add(i)
+
+
+def test_automatic_bucket_width_keeps_the_table_size():
+    """Above 2^25 ids an 8-slot table of 8,192 slots has too few buckets for
+    15-bit tags; 4-slot buckets keep it at about 8,192 slots instead of
+    doubling it."""
+    assert TableModel(8192, 1_000_000).bucket_slots == 8
+    assert TableModel(8192, 10_000_000).bucket_slots == 8
+    t = TableModel(8192, 50_000_000)
+    assert t.bucket_slots == 4 and t.nb * 4 <= 8192 + 64
+    t8 = TableModel(8192, 50_000_000, bucket_slots=8)
+    assert t8.nb * 8 > 16000
