@@ -560,22 +560,30 @@ struct Visited {
     // still determines the id (b1 = b2 - alt(t) mod nb, then h, then the id,
     // since the scramble is a bijection).  t = 0x7FFF has no second bucket
     // (its secondary tag would read as an empty slot).
-    __device__ __forceinline__ int probe(uint32_t id) const
+    __device__ __forceinline__ void locate(uint32_t id, uint32_t &b1, uint32_t &tag) const
     {
         const uint32_t h = scramble(id);
         // h / nb by multiply-high with magic = floor(2^32 / nb): the
         // estimate is low by at most one
-        uint32_t tag = __umulhi(h, magic);
-        uint32_t b1 = h - tag * nb;
+        tag = __umulhi(h, magic);
+        b1 = h - tag * nb;
         if (b1 >= nb) {
             b1 -= nb;
             ++tag;
         }
+    }
+    __device__ __forceinline__ uint32_t second(uint32_t b1, uint32_t tag) const
+    {
+        uint32_t b2 = b1 + 1u + __umulhi(tag * 0x85EBCA77u, nb - 1u);
+        return b2 >= nb ? b2 - nb : b2;
+    }
+    __device__ __forceinline__ int probe(uint32_t id) const
+    {
+        uint32_t b1, tag;
+        locate(id, b1, tag);
         const int r = probe_bucket(b1, tag);
         if (r != 2 || tag == 0x7FFFu) return r;
-        uint32_t b2 = b1 + 1u + __umulhi(tag * 0x85EBCA77u, nb - 1u);
-        if (b2 >= nb) b2 -= nb;
-        return probe_bucket(b2, tag | 0x8000u);
+        return probe_bucket(second(b1, tag), tag | 0x8000u);
     }
 };
 
