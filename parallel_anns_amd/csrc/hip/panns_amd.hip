@@ -1474,6 +1474,8 @@ int panns_set_stop_model(panns_index *h, const float *params, uint32_t n_floats)
     return PANNS_OK;
 }
 
+}  // extern "C"
+
 namespace {
 
 // (x - mu) / sd for n rows of 8 features.
@@ -1496,6 +1498,8 @@ int check_model_scales(const float *params)
 }
 
 }  // namespace
+
+extern "C" {
 
 int panns_mlp_grad(panns_index *h, const float *params, const float *X, const float *Y, uint32_t n, float *grad,
                    float *loss)
