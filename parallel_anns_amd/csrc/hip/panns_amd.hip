@@ -129,6 +129,7 @@ struct panns_plan {
     uint32_t *shared_bitmap = nullptr;
     uint32_t grid = 0, lds = 0, hash_slots = 0, bucket_slots = 8;
     uint32_t table_eighths = 8;
+    uint32_t rows_per_wave = 0;  // neighbour rows per distance step (0: 96, as registers allow)
     uint32_t waves = 1;  // wavefronts per query
     // work item i is the caller's query perm[i] (empty: the caller's order)
     std::vector<uint32_t> perm;
@@ -572,6 +573,7 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     a.Mcap = Mcap;
     a.nb_cap = nb_cap;
     a.group_picks = std::max<uint32_t>(1u, nb_cap / stride);
+    a.group_rows = pl->rows_per_wave ? pl->rows_per_wave : 96u;
     a.init_queue = pl->init_queue.p;
     a.shared_bitmap = pl->shared_bitmap;
     a.stop_learned = p.stop == PANNS_STOP_LEARNED ? 1u : 0u;
@@ -1072,6 +1074,8 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
     if (o.n_buckets > PANNS_MAX_BUCKETS) return fail(PANNS_ERR_ARG, "at most 8 buckets");
     if (o.table_eighths == 0) o.table_eighths = 8;
     if (o.table_eighths > 8) return fail(PANNS_ERR_ARG, "table_eighths must be 1..8");
+    if (o.rows_per_wave && o.rows_per_wave != 32 && o.rows_per_wave != 64 && o.rows_per_wave != 96)
+        return fail(PANNS_ERR_ARG, "rows_per_wave must be 0, 32, 64 or 96");
     HIP_TRY(hipSetDevice(h->device));
     std::unique_ptr<panns_plan> pl(new panns_plan());
     pl->h = h;
@@ -1081,6 +1085,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
     pl->opt = o;
     pl->want_queue = o.want_queue != 0;
     pl->table_eighths = o.table_eighths;
+    pl->rows_per_wave = o.rows_per_wave;
     if (o.n_buckets == 1 && !o.use_graph && !o.concurrent) {
         rc = build_plan(pl.get(), query_ids, L_per_query, K_per_query, init_ids, n_init);
         if (rc) return rc;
@@ -1145,6 +1150,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
         part->nq = (uint32_t)m.size();
         part->want_queue = pl->want_queue;
         part->table_eighths = o.table_eighths;
+        part->rows_per_wave = o.rows_per_wave;
         part->slot0 = o.concurrent ? slot : 0u;
         part->counter_idx = counter++;
         std::vector<uint32_t> q(m.size()), lq(m.size()), kq(m.size());
@@ -1283,6 +1289,18 @@ int panns_plan_autotune(panns_index *h, const panns_params *p, const uint32_t *q
         for (uint32_t waves : {2u, 4u}) {
             panns_plan_options o = s2;
             o.waves = waves;
+            int rc = consider(o);
+            if (rc) return rc;
+        }
+    }
+    // stage 4: neighbour rows gathered per wave and distance step (96 is
+    // the default where the kernel's registers allow it; fewer rows in
+    // flight mean shorter groups at the tail of an iteration)
+    {
+        const panns_plan_options s3 = *best;
+        for (uint32_t rows : {64u, 32u}) {
+            panns_plan_options o = s3;
+            o.rows_per_wave = rows;
             int rc = consider(o);
             if (rc) return rc;
         }

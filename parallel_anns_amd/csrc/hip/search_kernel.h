@@ -94,6 +94,7 @@ struct KArgs {
     uint32_t tbl_words, tbl_buckets, tbl_magic, id_bits, use_table;  // LDS visited table
     uint32_t tbl_bshift;  // log2 of 32-bit words per bucket: 2 (8 slots) or 1 (4 slots)
     uint32_t Mcap, nb_cap, group_picks;
+    uint32_t group_rows;  // neighbour rows a wave gathers per distance step: 32, 64 or 96
     // subsearch on a caller-held queue and visited bitmap
     const uint64_t *init_queue;  // [n_work][L] keys, or null
     uint32_t *shared_bitmap;     // used instead of bitmaps when non-null; never cleared
@@ -755,23 +756,28 @@ __device__ __forceinline__ uint32_t distance_group(const float *__restrict__ bas
     return ncand;
 }
 
-// Distances for ids[0, cnt): groups of 96 ids (three rows per lane pair)
-// while more than 64 remain, then groups of 64.  kWide = false keeps to
-// groups of 64 (fewer registers; the learned-stop kernel holds its model).
+// Distances for ids[0, cnt) in groups of `rows` neighbour rows per wave
+// (one, two or three rows per lane pair in flight): with rows = 96, groups
+// of 96 while more than 64 remain, then groups of 64; with 64, groups of 64;
+// with 32, groups of 32.  kWide = false caps groups at 64 (fewer registers;
+// the learned-stop kernel holds its model).
 template <int D, bool kWide, bool kNorm>
 __device__ __forceinline__ uint32_t distance_pass(const float *__restrict__ base, const float *__restrict__ norms,
                                                   const uint32_t *ids, uint32_t cnt, const float4 *qs,
                                                   uint64_t bound, uint64_t *cand, uint32_t ncand, int lane,
-                                                  uint64_t lt)
+                                                  uint64_t lt, uint32_t rows)
 {
     uint32_t p = 0;
     while (p < cnt) {
-        if (kWide && cnt - p > 64u) {
+        if (kWide && rows >= 96u && cnt - p > 64u) {
             ncand = distance_group<D, 3, kNorm>(base, norms, ids, p, cnt, qs, bound, cand, ncand, lane, lt);
             p += 96u;
-        } else {
+        } else if (rows >= 64u) {
             ncand = distance_group<D, 2, kNorm>(base, norms, ids, p, cnt, qs, bound, cand, ncand, lane, lt);
             p += 64u;
+        } else {
+            ncand = distance_group<D, 1, kNorm>(base, norms, ids, p, cnt, qs, bound, cand, ncand, lane, lt);
+            p += 32u;
         }
     }
     return ncand;
@@ -874,7 +880,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
             for (uint32_t c0 = 0; c0 < L; c0 += 64) {
                 const uint32_t cnt = min(64u, L - c0);
                 const uint32_t n0 = distance_pass<D, !kLearned, kNorm>(a.base, a.norms, a.init_ids + c0, cnt, qs,
-                                                                       kKeyMax, cand, 0, lane, lt);
+                                                                       kKeyMax, cand, 0, lane, lt, a.group_rows);
                 (void)n0;
                 __syncthreads();
                 uint64_t x = (uint32_t)lane < cnt ? cand[lane] : kKeyMax;
@@ -983,7 +989,8 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                 PANNS_PT(kPhClaim);
                 ndist += nnew;
                 const uint32_t ncand = distance_pass<D, !kLearned, kNorm>(a.base, a.norms, newids, nnew, qs,
-                                                                          km(Q[S - 1]), cand, 0, lane, lt);
+                                                                          km(Q[S - 1]), cand, 0, lane, lt,
+                                                                          a.group_rows);
                 // stop feature: new keys that beat the queue's last key (a set
                 // fixed by the group, unlike the keys a batch-wise merge
                 // inserts and a later batch evicts)

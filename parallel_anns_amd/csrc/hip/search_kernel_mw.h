@@ -105,7 +105,8 @@ __global__ __launch_bounds__(64 * W, 2) void search_kernel_mw(KArgs a)
             }
             for (uint32_t c0 = 0; c0 < L; c0 += 64) {
                 const uint32_t cnt = min(64u, L - c0);
-                distance_pass<D, true, kNorm>(a.base, a.norms, a.init_ids + c0, cnt, qs, kKeyMax, cand, 0, lane, lt);
+                distance_pass<D, true, kNorm>(a.base, a.norms, a.init_ids + c0, cnt, qs, kKeyMax, cand, 0, lane, lt,
+                                              a.group_rows);
                 wave_sync();
                 uint64_t x = (uint32_t)lane < cnt ? cand[lane] : kKeyMax;
                 x = sort64(x, lane, xs);
@@ -176,7 +177,8 @@ __global__ __launch_bounds__(64 * W, 2) void search_kernel_mw(KArgs a)
             // every wave tests against the queue's last key before this
             // iteration's merge (one group: top-L of the union)
             const uint32_t ncand =
-                distance_pass<D, true, kNorm>(a.base, a.norms, newids, nnew, qs, km(Q[S - 1]), cand, 0, lane, lt);
+                distance_pass<D, true, kNorm>(a.base, a.norms, newids, nnew, qs, km(Q[S - 1]), cand, 0, lane, lt,
+                                              a.group_rows);
             if (lane == 0) ctrl[kCtlN + wid] = ncand;
             __syncthreads();
 

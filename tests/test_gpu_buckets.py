@@ -117,3 +117,25 @@ def test_bucket_options_are_checked(gpu_index, uneven):
     with pytest.raises(ValueError):
         pl.fetch(want_queue=True)
     pl.close()
+
+
+@pytest.mark.parametrize("rows", [32, 64, 96])
+def test_rows_per_wave_keep_results(small_set, gpu_index, uneven, rows):
+    """Neighbour rows gathered per wave and distance step (an autotuner
+    dimension) change the launch shape only: buckets with one or two waves
+    per query give the default's results."""
+    qids, Ls, Ks = uneven
+    print("case rows_per_wave", rows, flush=True)
+    init = gpu_index.prepare_init_ids(int(Ls.max()))
+    ref = gpu_index.plan(qids, 100, int(Ls.max()), 4, init_ids=init, L_per_query=Ls, K_per_query=Ks)
+    ref.run()
+    r0 = ref.fetch(want_queue=True)
+    ref.close()
+    for waves in (0, 2):
+        pl = gpu_index.plan(qids, 100, int(Ls.max()), 4, init_ids=init, L_per_query=Ls, K_per_query=Ks,
+                            options=Index.plan_options(buckets=2, rows_per_wave=rows, waves=waves))
+        pl.run()
+        _same(pl.fetch(want_queue=True), r0, ("ids", "dists", "ndist", "nexp", "niter", "queue_ids", "queue_dists"))
+        pl.close()
+    with pytest.raises(SearchError):
+        gpu_index.plan(qids[:4], 100, 200, 4, init_ids=init, options=Index.plan_options(rows_per_wave=48))
