@@ -120,7 +120,15 @@ def cpu_baseline(ds, K, L, M, init_ids, budget_s=12.0):
     t0 = time.perf_counter()
     ids, _, _, _ = po.search_batch(g, ds["base"], q[:n_s], K, L, M, init_ids, nthreads=cores)
     el = time.perf_counter() - t0
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(n_s / el, 1), "unit": "QPS", "cores": cores, "kind": "port",
+            "cpu_model": model, "compiler": "gcc -O3 -march=x86-64-v3 -fopenmp (oracle/Makefile)",
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""), "omp_places": os.environ.get("OMP_PLACES", ""),
             "sample": f"first {n_s} of {q.shape[0]} test queries, fixed-L top-M M={M} L={L} K={K}, "
                       f"OpenMP inter-query over {cores} threads (oracle restatement; ids identical "
                       f"to the GPU fixed-L search)"}, ids
@@ -205,6 +213,10 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
 
+    # the CPU baseline's threads pinned one per core, close together (set
+    # before any OpenMP runtime starts)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; queries shard across GPUs with no "
                                    "exchange step, so N>1 is not offered", "n_gpus": args.gpus}))
