@@ -70,7 +70,8 @@ def report(a):
                      f"{fk:,.0f} | {wk:,.0f} | {traffic / 1e9:.3f} | {alg / 1e9:.3f} | {traffic / alg:.3f} | "
                      f"{alg / (mean_us * 1e-6) / 1e9:,.1f} |")
         rec = {"config": run["config"], "kernel": f"search_kernel<{dim}, {variant}>", "M": r["M"], "L": r["L"],
-               "traffic_bytes": int(traffic), "fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
+               "traffic_bytes": int(traffic), "algorithmic_bytes": int(alg), "fetch_kib": round(fk, 1),
+               "write_kib": round(wk, 1),
                "fetch_factor": round(ff, 4), "write_factor": round(wf, 4), "trace_mean_us": round(mean_us, 1),
                "command": a.command}
         if "threshold" in r:
