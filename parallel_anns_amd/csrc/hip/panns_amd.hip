@@ -383,8 +383,23 @@ int locality_order(panns_index *h, const uint32_t *query_ids, uint32_t nq, std::
     HIP_TRY(dq.ensure(nq));
     HIP_TRY(dn.ensure(nq));
     HIP_TRY(hipMemcpy(dq.p, query_ids, nq * 4ull, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(panns::nearest_pivot_kernel, dim3((nq + 255) / 256), dim3(256), 0, h->stream, h->queries.p,
-                       dq.p, nq, h->pivots.p, P, D, dn.p);
+    const dim3 pg((nq + 31) / 32), pb(256);
+    switch (D) {
+#define PANNS_DIM(DD)                                                                                        \
+    case DD:                                                                                                 \
+        hipLaunchKernelGGL(panns::nearest_pivot_kernel<DD>, pg, pb, 0, h->stream, h->queries.p, dq.p, nq,     \
+                           h->pivots.p, P, dn.p);                                                            \
+        break;
+        PANNS_DIM(16)
+        PANNS_DIM(32)
+        PANNS_DIM(40)
+        PANNS_DIM(64)
+        PANNS_DIM(96)
+        PANNS_DIM(128)
+#undef PANNS_DIM
+    default:
+        return fail(PANNS_ERR_DIM, "dimension " + std::to_string(D) + " has no kernel");
+    }
     HIP_TRY(hipGetLastError());
     std::vector<uint32_t> near(nq);
     HIP_TRY(hipMemcpyAsync(near.data(), dn.p, nq * 4ull, hipMemcpyDeviceToHost, h->stream));

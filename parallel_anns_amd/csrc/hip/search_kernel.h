@@ -1106,29 +1106,54 @@ __global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ d
 }
 
 // Nearest of P pivot rows for each query (fp32, any summation order: used
-// only to order a batch's queries by locality, never for results).
+// only to order a batch's queries by locality, never for results).  Eight
+// lanes per query, each holding the query row in registers and scanning
+// every eighth pivot (the pivot reads of a lane group are shared); the
+// eight partial minima are reduced with lane exchanges, ties to the lower
+// pivot.  Blocks of 256 threads serve 32 queries.
+template <int D>
 __global__ void nearest_pivot_kernel(const float *__restrict__ queries, const uint32_t *__restrict__ qids,
-                                     uint32_t nq, const float *__restrict__ piv, uint32_t P, uint32_t D,
+                                     uint32_t nq, const float *__restrict__ piv, uint32_t P,
                                      uint32_t *__restrict__ out)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    const float *q = queries + (size_t)qids[i] * D;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = t >> 3, slice = t & 7u;
+    const bool live = i < nq;
+    float q[D];
+    const float4 *qr = reinterpret_cast<const float4 *>(queries + (size_t)qids[live ? i : 0u] * D);
+#pragma unroll
+    for (int k = 0; k < D / 4; ++k) {
+        const float4 v = qr[k];
+        q[4 * k] = v.x;
+        q[4 * k + 1] = v.y;
+        q[4 * k + 2] = v.z;
+        q[4 * k + 3] = v.w;
+    }
     float best = __int_as_float(0x7f800000);
-    uint32_t bi = 0;
-    for (uint32_t p = 0; p < P; ++p) {
+    uint32_t bi = 0xFFFFFFFFu;
+    for (uint32_t p = slice; p < P; p += 8) {
         const float *v = piv + (size_t)p * D;
         float d = 0.f;
-        for (uint32_t k = 0; k < D; ++k) {
-            const float t = v[k] - q[k];
-            d = __builtin_fmaf(t, t, d);
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const float x = v[k] - q[k];
+            d = __builtin_fmaf(x, x, d);
         }
         if (d < best) {
             best = d;
             bi = p;
         }
     }
-    out[i] = bi;
+#pragma unroll
+    for (int m = 1; m < 8; m <<= 1) {
+        const float ob = __shfl_xor(best, m);
+        const uint32_t oi = (uint32_t)__shfl_xor((int)bi, m);
+        if (ob < best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
+        }
+    }
+    if (live && slice == 0) out[i] = bi;
 }
 
 // Zeroes the n work counters of a launch sequence (a kernel, not a memset,
