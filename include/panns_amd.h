@@ -242,6 +242,24 @@ int panns_plan_autotune(panns_index *h, const panns_params *p, const uint32_t *q
  * queries (ground truth).  k <= 128.  out_* hold (number of queries) * k. */
 int panns_exact_knn(panns_index *h, int source, uint32_t k, uint32_t *out_ids, float *out_dists);
 
+typedef enum {
+    PANNS_KNN_AUTO = 0,        /* the faster of the two on this hardware: EXACT_VALU */
+    PANNS_KNN_EXACT_VALU = 1,  /* the exact distance chain for every pair */
+    PANNS_KNN_MFMA = 2         /* FP32 MFMA candidates, exact re-rank, exact recompute where unproven */
+} panns_knn_method;
+
+/* panns_exact_knn with a choice of method; all give the same ids and
+ * distances.  MFMA (for k <= 120) keeps k + 28 candidates per query by
+ * |v|^2 - 2 q.v from FP32 matrix instructions, recomputes their distances
+ * with the reference's chain, and proves the result with an fp32 error bound
+ * on the candidate margin; queries the bound does not cover are recomputed
+ * with the exact chain over every row (their count in *n_recomputed,
+ * nullable).  Measured on MI355X it is about 1.25x slower than EXACT_VALU
+ * at 1M x 96: the kernel time goes to tile staging, synchronisation and
+ * selection, not to the arithmetic the matrix cores take over. */
+int panns_exact_knn_ex(panns_index *h, int source, uint32_t k, int method, uint32_t *out_ids, float *out_dists,
+                       uint32_t *n_recomputed);
+
 /* Learned early termination (no reference counterpart; the reference's
  * studies of when the top-K stops changing are
  * core/Searching.202012041559.top_m_pure_large_graph.last_update_step.* and
@@ -282,6 +300,9 @@ int panns_mlp_grad(panns_index *h, const float *params, const float *X, const fl
 /* One 16x16x16 BF16 MFMA tile, C = A * B (row-major fp32 in and out), to
  * check the fragment layout the training kernels assume. */
 int panns_mfma_probe(panns_index *h, const float *A, const float *B, float *C);
+/* One 16x16x4 FP32 MFMA tile, C (16x16) = A (16x4) * B (4x16), row-major,
+ * to check the fragment layout panns_exact_knn_ex assumes. */
+int panns_mfma_f32_probe(panns_index *h, const float *A, const float *B, float *C);
 
 /* Measurement helpers (no reference counterpart).
  * panns_plan_phase_cycles: shader-clock cycles the last run spent per search

@@ -239,6 +239,27 @@ bool select_kernel(uint32_t dim, bool learned, bool norm, uint32_t waves, Launch
     return waves > 1 ? kernel_for_dim_mw(dim, waves, norm, fn, kp) : kernel_for_dim(dim, learned, norm, fn, kp);
 }
 
+using KnnLaunch = void (*)(const float *, uint32_t, const float *, uint32_t, uint32_t, int, uint32_t,
+                          const uint32_t *, uint32_t *, float *, uint32_t, uint32_t, hipStream_t);
+using KnnMfmaLaunch = void (*)(const float *, const float *, uint32_t, const float *, uint32_t, uint32_t, uint32_t,
+                              int, uint32_t, float, uint32_t *, float *, uint32_t *, uint32_t, uint32_t, hipStream_t);
+
+template <int D>
+void launch_knn(const float *b, uint32_t n, const float *q, uint32_t nq, uint32_t k, int ex, uint32_t qoff,
+                const uint32_t *self, uint32_t *oi, float *od, uint32_t grid, uint32_t lds, hipStream_t s)
+{
+    hipLaunchKernelGGL(panns::knn_kernel<D>, dim3(grid), dim3(256), lds, s, b, n, q, nq, k, ex, qoff, self, oi, od);
+}
+
+template <int D>
+void launch_knn_mfma(const float *b, const float *norms, uint32_t n, const float *q, uint32_t nq, uint32_t kc,
+                     uint32_t k, int ex, uint32_t qoff, float vmax2, uint32_t *oi, float *od, uint32_t *fl,
+                     uint32_t grid, uint32_t lds, hipStream_t s)
+{
+    hipLaunchKernelGGL(panns::knn_mfma_kernel<D>, dim3(grid), dim3(256), lds, s, b, norms, n, q, nq, kc, k, ex, qoff,
+                       vmax2, oi, od, fl);
+}
+
 // PSS5.cpp:293-334 compute_norm for every base row, once per base set.
 int ensure_norms(panns_index *h)
 {
@@ -1694,27 +1715,35 @@ int panns_device_copy(panns_index *h, uint64_t bytes, float *ms)
 
 int panns_exact_knn(panns_index *h, int source, uint32_t k, uint32_t *out_ids, float *out_dists)
 {
+    return panns_exact_knn_ex(h, source, k, PANNS_KNN_AUTO, out_ids, out_dists, nullptr);
+}
+
+int panns_exact_knn_ex(panns_index *h, int source, uint32_t k, int method, uint32_t *out_ids, float *out_dists,
+                       uint32_t *n_recomputed)
+{
     if (!h || !out_ids) return fail(PANNS_ERR_ARG, "null argument");
     if (!h->n) return fail(PANNS_ERR_STATE, "base set not loaded");
     if (source != 0 && source != 1) return fail(PANNS_ERR_ARG, "source must be 0 (base rows) or 1 (queries)");
+    if (method != PANNS_KNN_AUTO && method != PANNS_KNN_EXACT_VALU && method != PANNS_KNN_MFMA)
+        return fail(PANNS_ERR_ARG, "unknown k-NN method");
     if (source == 1 && !h->nq) return fail(PANNS_ERR_STATE, "queries not loaded");
     if (k == 0 || k > (uint32_t)panns::kKnnKCap) return fail(PANNS_ERR_ARG, "k must be in [1, 128]");
     if (k > h->n - (source == 0 ? 1u : 0u)) return fail(PANNS_ERR_ARG, "k exceeds the number of candidates");
     HIP_TRY(hipSetDevice(h->device));
     const float *qsrc = source == 0 ? h->base.p : h->queries.p;
     const uint32_t nq = source == 0 ? h->n : h->nq;
-    const void *kp = nullptr;
-    void (*launch)(const float *, uint32_t, const float *, uint32_t, uint32_t, int, uint32_t, uint32_t *,
-                   float *, uint32_t, uint32_t, hipStream_t) = nullptr;
+    const int ex = source == 0 ? 1 : 0;
+    if (n_recomputed) *n_recomputed = 0;
+    const void *kp = nullptr, *kpm = nullptr;
+    KnnLaunch launch = nullptr;
+    KnnMfmaLaunch launch_m = nullptr;
     switch (h->dim) {
-#define PANNS_KNN_DIM(DD)                                                                             \
-    case DD:                                                                                          \
-        kp = reinterpret_cast<const void *>(&panns::knn_kernel<DD>);                                  \
-        launch = [](const float *b, uint32_t n, const float *q, uint32_t nq_, uint32_t kk, int ex,    \
-                    uint32_t qo, uint32_t *oi, float *od, uint32_t grid, uint32_t lds, hipStream_t s) { \
-            hipLaunchKernelGGL(panns::knn_kernel<DD>, dim3(grid), dim3(256), lds, s, b, n, q, nq_, kk, \
-                               ex, qo, oi, od);                                                       \
-        };                                                                                            \
+#define PANNS_KNN_DIM(DD)                                                                                 \
+    case DD:                                                                                              \
+        kp = reinterpret_cast<const void *>(&panns::knn_kernel<DD>);                                      \
+        kpm = reinterpret_cast<const void *>(&panns::knn_mfma_kernel<DD>);                                \
+        launch = launch_knn<DD>;                                                                          \
+        launch_m = launch_knn_mfma<DD>;                                                                   \
         break;
         PANNS_KNN_DIM(16)
         PANNS_KNN_DIM(32)
@@ -1726,29 +1755,101 @@ int panns_exact_knn(panns_index *h, int source, uint32_t k, uint32_t *out_ids, f
     default:
         return fail(PANNS_ERR_DIM, "dimension has no k-NN kernel (16, 32, 40, 64, 96, 128)");
     }
+    const uint32_t D = h->dim, RS = D + 4u;
     const uint32_t lds = 8u * (panns::kKnnQT * panns::kKnnKCap + panns::kKnnQT * 64u + 4u * 64u) +
-                         4u * panns::kKnnRT * (h->dim + 4u) + 4u * 2u * panns::kKnnQT + 8u * panns::kKnnQT;
+                         4u * panns::kKnnRT * RS + 4u * 2u * panns::kKnnQT + 8u * panns::kKnnQT;
+    const uint32_t lds_m = 8u * (panns::kKnnQT * panns::kKnnKCap + panns::kKnnQT * 64u + 4u * 64u + panns::kKnnQT) +
+                           4u * (panns::kKnnRT * RS + panns::kKnnQT * RS + (panns::kKnnQT + 1u) * panns::kKnnRT +
+                                 panns::kKnnRT) +
+                           4u * 2u * panns::kKnnQT;
     HIP_TRY(hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    HIP_TRY(hipFuncSetAttribute(kpm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_m));
+    // candidates: k plus a margin, at most the list capacity
+    const uint32_t kc = std::min<uint32_t>((uint32_t)panns::kKnnKCap, k + 28u);
+    const bool use_mfma = method == PANNS_KNN_MFMA && kc >= k + 8u;
+    float vmax2 = 0.f;
+    if (use_mfma) {
+        int rc = ensure_norms(h);
+        if (rc) return rc;
+        std::vector<float> nv(h->n);
+        HIP_TRY(hipMemcpy(nv.data(), h->norms.p, (size_t)h->n * 4, hipMemcpyDeviceToHost));
+        for (float x : nv) vmax2 = std::max(vmax2, x);
+    }
     // launches of at most 64k queries keep each kernel short
     const uint32_t chunk = 65536;
-    DevBuf<uint32_t> di;
-    DevBuf<float> dd;
+    DevBuf<uint32_t> di, df, dsel;
+    DevBuf<float> dd, dq;
     HIP_TRY(di.ensure((size_t)std::min(nq, chunk) * k));
     HIP_TRY(dd.ensure((size_t)std::min(nq, chunk) * k));
+    HIP_TRY(df.ensure(std::min(nq, chunk)));
+    std::vector<uint32_t> flags, redo;
     for (uint32_t q0 = 0; q0 < nq; q0 += chunk) {
         const uint32_t cn = std::min(chunk, nq - q0);
         const uint32_t grid = (cn + panns::kKnnQT - 1) / panns::kKnnQT;
         // q0 offsets the chunk's query index so that exclude_self compares
         // base-row ids with global query ids.
-        launch(h->base.p, h->n, qsrc + (size_t)q0 * h->dim, cn, k, source == 0 ? 1 : 0, q0, di.p, dd.p, grid,
-               lds, h->stream);
+        if (use_mfma)
+            launch_m(h->base.p, h->norms.p, h->n, qsrc + (size_t)q0 * D, cn, kc, k, ex, q0, vmax2, di.p, dd.p, df.p,
+                     grid, lds_m, h->stream);
+        else
+            launch(h->base.p, h->n, qsrc + (size_t)q0 * D, cn, k, ex, q0, nullptr, di.p, dd.p, grid, lds, h->stream);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(out_ids + (size_t)q0 * k, di.p, (size_t)cn * k * 4, hipMemcpyDeviceToHost, h->stream));
         if (out_dists)
             HIP_TRY(hipMemcpyAsync(out_dists + (size_t)q0 * k, dd.p, (size_t)cn * k * 4, hipMemcpyDeviceToHost,
                                    h->stream));
+        if (use_mfma) {
+            flags.resize(cn);
+            HIP_TRY(hipMemcpyAsync(flags.data(), df.p, (size_t)cn * 4, hipMemcpyDeviceToHost, h->stream));
+        }
         HIP_TRY(hipStreamSynchronize(h->stream));
+        for (uint32_t i = 0; use_mfma && i < cn; ++i)
+            if (flags[i]) redo.push_back(q0 + i);
     }
+    if (!redo.empty()) {
+        // the queries whose candidate margin did not prove the result: the
+        // exact kernel on their rows, own ids excluded as before
+        if (n_recomputed) *n_recomputed = (uint32_t)redo.size();
+        for (size_t r0 = 0; r0 < redo.size(); r0 += chunk) {
+            const uint32_t cn = (uint32_t)std::min<size_t>(chunk, redo.size() - r0);
+            HIP_TRY(dq.ensure((size_t)cn * D));
+            HIP_TRY(dsel.ensure(cn));
+            HIP_TRY(hipMemcpy(dsel.p, redo.data() + r0, (size_t)cn * 4, hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(panns::gather_rows_kernel, dim3((uint32_t)(((uint64_t)cn * (D / 4) + 255) / 256)),
+                               dim3(256), 0, h->stream, qsrc, dsel.p, cn, D, dq.p);
+            HIP_TRY(hipGetLastError());
+            const uint32_t grid = (cn + panns::kKnnQT - 1) / panns::kKnnQT;
+            launch(h->base.p, h->n, dq.p, cn, k, ex, 0, dsel.p, di.p, dd.p, grid, lds, h->stream);
+            HIP_TRY(hipGetLastError());
+            std::vector<uint32_t> ids((size_t)cn * k);
+            std::vector<float> ds((size_t)cn * k);
+            HIP_TRY(hipMemcpyAsync(ids.data(), di.p, ids.size() * 4, hipMemcpyDeviceToHost, h->stream));
+            HIP_TRY(hipMemcpyAsync(ds.data(), dd.p, ds.size() * 4, hipMemcpyDeviceToHost, h->stream));
+            HIP_TRY(hipStreamSynchronize(h->stream));
+            for (uint32_t i = 0; i < cn; ++i) {
+                const size_t dst = (size_t)redo[r0 + i] * k;
+                std::memcpy(out_ids + dst, ids.data() + (size_t)i * k, k * 4);
+                if (out_dists) std::memcpy(out_dists + dst, ds.data() + (size_t)i * k, k * 4);
+            }
+        }
+    }
+    return PANNS_OK;
+}
+
+int panns_mfma_f32_probe(panns_index *h, const float *A, const float *B, float *C)
+{
+    if (!h || !A || !B || !C) return fail(PANNS_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(h->device));
+    DevBuf<float> dA, dB, dC;
+    HIP_TRY(dA.ensure(64));
+    HIP_TRY(dB.ensure(64));
+    HIP_TRY(dC.ensure(256));
+    HIP_TRY(hipMemcpy(dA.p, A, 256, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dB.p, B, 256, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(panns::mfma_f32_probe_kernel, dim3(1), dim3(64), 0, h->stream, dA.p, dB.p, dC.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(C, dC.p, 1024, hipMemcpyDeviceToHost));
     return PANNS_OK;
 }
 

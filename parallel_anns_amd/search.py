@@ -332,16 +332,32 @@ class Index:
         _chk(self.lib.panns_device_copy(self.h, nbytes, ms))
         return ms.value
 
-    def exact_knn(self, k, source="queries"):
+    def exact_knn(self, k, source="queries", method="auto", stats=None):
         """Exact k-NN on the GPU (reference distance, ties to the lower id).
         source="queries": of the loaded queries (ground truth); "base": of
-        every base row with itself excluded (graph k-NN lists)."""
+        every base row with itself excluded (graph k-NN lists).  method
+        "valu": the exact chain for every pair ("auto" picks it, the faster
+        here); "mfma": FP32 MFMA candidates with an exact, proven re-rank.
+        stats (a dict) receives the number of queries the re-rank could not
+        prove and recomputed exactly."""
         info = self.info()
         nq = info["num_queries"] if source == "queries" else info["num_v"]
         ids = np.empty((nq, k), np.uint32)
         dists = np.empty((nq, k), np.float32)
-        _chk(self.lib.panns_exact_knn(self.h, 1 if source == "queries" else 0, k, ptr(ids), ptr(dists)))
+        redo = _native.c_u32(0)
+        _chk(self.lib.panns_exact_knn_ex(self.h, 1 if source == "queries" else 0, k,
+                                         {"auto": 0, "valu": 1, "mfma": 2}[method], ptr(ids), ptr(dists), redo))
+        if stats is not None:
+            stats["recomputed"] = redo.value
         return ids, dists
+
+    def mfma_f32_probe(self, A, B):
+        """C (16x16) = A (16x4) @ B (4x16) by one FP32 MFMA tile."""
+        A = np.ascontiguousarray(A, np.float32)
+        B = np.ascontiguousarray(B, np.float32)
+        C = np.zeros((16, 16), np.float32)
+        _chk(self.lib.panns_mfma_f32_probe(self.h, ptr(A), ptr(B), ptr(C)))
+        return C
 
     def subsearch_with_top_m(self, M_max, query_id, local_L, queue_ids, queue_dists, queue_checked,
                              visited_words):

@@ -46,3 +46,59 @@ def test_exact_knn_ties_go_to_lower_id():
     for q in range(10):
         assert list(gi[q, :3]) == [q, q + 50, q + 100]
     idx.close()
+
+
+def test_mfma_f32_layout():
+    """The fragment layout the MFMA k-NN kernel assumes: C = A (16x4) @ B (4x16)."""
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((16, 4)).astype(np.float32)
+    B = rng.standard_normal((4, 16)).astype(np.float32)
+    idx = Index(0)
+    print("case mfma f32 probe", flush=True)
+    C = idx.mfma_f32_probe(A, B)
+    idx.close()
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    assert np.abs(C - ref).max() <= 1e-5 * (np.abs(A).max() * np.abs(B).max() * 4)
+
+
+@pytest.mark.parametrize("n,dim,k,src", [(20000, 96, 100, "queries"), (20000, 96, 64, "base"),
+                                         (6000, 40, 50, "queries"), (4000, 128, 120, "base"),
+                                         (3000, 16, 10, "queries")])
+def test_mfma_candidates_equal_exact_chain(n, dim, k, src):
+    """MFMA candidates + exact re-rank (+ exact recompute where the margin
+    does not prove the result) give the exact kernel's ids and distance bits."""
+    print(f"case knn mfma n={n} dim={dim} k={k} {src}", flush=True)
+    base = tools.gen_mixture(n, dim, 16, 0.3, 0, 41)
+    queries = tools.gen_mixture(300, dim, 16, 0.3, 0, 42)
+    idx = Index(0)
+    idx.set_base(base)
+    idx.set_queries(queries)
+    st = {}
+    a_i, a_d = idx.exact_knn(k, source=src, method="mfma", stats=st)
+    v_i, v_d = idx.exact_knn(k, source=src, method="valu")
+    idx.close()
+    assert (a_i == v_i).all()
+    assert (a_d.view(np.uint32) == v_d.view(np.uint32)).all()
+    nq = queries.shape[0] if src == "queries" else n
+    assert st["recomputed"] < nq  # the bound proves most queries
+
+
+def test_mfma_candidates_with_exact_ties():
+    """Every row repeated 100 times: the k-th and the last candidate share one
+    distance, so the margin cannot prove the result; those queries are
+    recomputed with the exact chain and the ids still equal the oracle's
+    (ties to the lower id).  Integer rows keep every distance exact."""
+    rng = np.random.default_rng(9)
+    uniq = rng.integers(-2, 3, (30, 32)).astype(np.float32)
+    base = np.tile(uniq, (100, 1))
+    queries = rng.integers(-2, 3, (64, 32)).astype(np.float32)
+    idx = Index(0)
+    idx.set_base(base)
+    idx.set_queries(queries)
+    st = {}
+    print("case knn mfma ties", flush=True)
+    gi, gd = idx.exact_knn(50, source="queries", method="mfma", stats=st)
+    idx.close()
+    oi, od = po.brute_force_knn(base, queries, 50)
+    assert (gi == oi).all() and (gd.view(np.uint32) == od.view(np.uint32)).all()
+    assert st["recomputed"] > 0
