@@ -174,7 +174,8 @@ def timed(index, qids, K, L, M, gt, steps, warmup, **kw):
     plan = index.plan(qids, K, L, M, want_queue=False, options=opt, **kw)
     desc = plan.describe()
     desc["autotuned"] = {"table_eighths": opt.table_eighths, "n_buckets": opt.n_buckets,
-                         "hash_slots": plan.buckets()[0]["hash_slots"], "rows_per_wave": int(opt.rows_per_wave)}
+                         "hash_slots": plan.buckets()[0]["hash_slots"], "rows_per_wave": int(opt.rows_per_wave),
+                         "overflow_slots": int(opt.overflow_slots)}
     wall, kms = time_plan(plan, steps, warmup)
     res = plan.fetch()
     plan.close()
@@ -183,7 +184,8 @@ def timed(index, qids, K, L, M, gt, steps, warmup, **kw):
                        "recall": round(tools.recall(gt, res["ids"], L), 4),
                        "kernel_ms": round(float(np.mean(kms)), 4),
                        "ndist": round(float(res["ndist"].mean()), 1), "table_eighths": int(opt.table_eighths),
-                       "waves": int(max(1, opt.waves)), "rows_per_wave": int(opt.rows_per_wave)}
+                       "waves": int(max(1, opt.waves)), "rows_per_wave": int(opt.rows_per_wave),
+                       "overflow_slots": int(opt.overflow_slots)}
 
 
 def deficit_distribution(d):

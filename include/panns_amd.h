@@ -211,7 +211,10 @@ typedef struct {
     uint32_t rows_per_wave;                    /* neighbour rows a wave gathers per distance step:
                                                   32, 64 or 96 (0: 96, or 64 in the learned-stop
                                                   kernel, whose registers hold the model) */
-    uint32_t reserved[2];
+    uint32_t overflow_slots;                   /* LDS overflow entries (full ids) for visited ids whose
+                                                  buckets are full, before the global bitmap: 0 or
+                                                  a power of two up to 4096 */
+    uint32_t reserved[1];
 } panns_plan_options;
 
 int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *query_ids,
@@ -224,7 +227,8 @@ int panns_plan_describe_buckets(panns_plan *plan, uint32_t *n_buckets, uint32_t 
                                 uint32_t *workgroups, uint32_t *lds_bytes, uint32_t *hash_slots);
 /* The autotuner: times candidate bucketings (1, 2, 3, 4, 6 buckets of equal
  * summed L; side by side or not; visited tables at 8/8 down to 4/8 of the
- * automatic size; 1, 2 or 4 waves per query where the search allows it;
+ * automatic size, each with 0, 256 or 512 overflow entries; 1, 2 or 4 waves
+ * per query where the search allows it;
  * 96, 64 or 32 neighbour rows per wave and distance step;
  * always as a graph) on every `sample_stride`-th query of the batch (0: a stride that
  * leaves about 20,000 queries) and returns the fastest in *best, with its

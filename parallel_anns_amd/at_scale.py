@@ -105,7 +105,7 @@ def main():
                 "algorithmic_bytes_per_launch": int(B), "achieved_GBps": round(B / (kms * 1e-3) / 1e9, 1),
                 "hbm_frac": round(B / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "launches": len(nb),
                 "table_eighths": opt.table_eighths, "concurrent": bool(opt.concurrent),
-                "rows_per_wave": opt.rows_per_wave,
+                "rows_per_wave": opt.rows_per_wave, "overflow_slots": opt.overflow_slots,
                 "tune_and_time_s": round(time.time() - t0, 1)}
         log(f"{name}: {info}")
         return res, rec, info

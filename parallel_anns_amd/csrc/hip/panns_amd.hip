@@ -130,6 +130,7 @@ struct panns_plan {
     uint32_t grid = 0, lds = 0, hash_slots = 0, bucket_slots = 8;
     uint32_t table_eighths = 8;
     uint32_t rows_per_wave = 0;  // neighbour rows per distance step (0: 96, as registers allow)
+    uint32_t overflow_slots = 0;  // LDS overflow entries of the visited set (0: none)
     uint32_t waves = 1;  // wavefronts per query
     // work item i is the caller's query perm[i] (empty: the caller's order)
     std::vector<uint32_t> perm;
@@ -499,7 +500,10 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     pl->hash_slots = hs;
     // Lcap rounded to 8 keys keeps every LDS region 16-byte aligned.
     const uint32_t Lcap = (pl->maxL + 7u) & ~7u;
-    pl->lds = W > 1 ? lds_bytes_mw(Lcap, nb_cap, hs, Mcap, h->dim, W) : lds_bytes(Lcap, nb_cap, hs, Mcap, h->dim);
+    const uint32_t ovf = pl->shared_bitmap ? 0u : pl->overflow_slots;
+    // the overflow entries are 32-bit, i.e. two 16-bit slots each in the carving
+    pl->lds = W > 1 ? lds_bytes_mw(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim, W)
+                    : lds_bytes(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim);
     if (pl->lds > 65536) {
         HIP_TRY(hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->lds));
     }
@@ -600,7 +604,8 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     a.spill_log = h->spill_log.p + (size_t)pl->slot0 * kLogCap;
     a.log_cap = kLogCap;
     a.Lcap = Lcap;
-    a.tbl_words = hs / 2u;
+    a.ovf_slots = pl->shared_bitmap ? 0u : pl->overflow_slots;
+    a.tbl_words = hs / 2u + a.ovf_slots;
     a.tbl_buckets = nb;
     a.tbl_bshift = bslots == 8u ? 2u : 1u;
     a.tbl_magic = (uint32_t)((1ull << 32) / nb);
@@ -1112,6 +1117,8 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
     if (o.table_eighths > 8) return fail(PANNS_ERR_ARG, "table_eighths must be 1..8");
     if (o.rows_per_wave && o.rows_per_wave != 32 && o.rows_per_wave != 64 && o.rows_per_wave != 96)
         return fail(PANNS_ERR_ARG, "rows_per_wave must be 0, 32, 64 or 96");
+    if (o.overflow_slots && ((o.overflow_slots & (o.overflow_slots - 1u)) || o.overflow_slots > 4096u))
+        return fail(PANNS_ERR_ARG, "overflow_slots must be 0 or a power of two up to 4096");
     HIP_TRY(hipSetDevice(h->device));
     std::unique_ptr<panns_plan> pl(new panns_plan());
     pl->h = h;
@@ -1122,6 +1129,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
     pl->want_queue = o.want_queue != 0;
     pl->table_eighths = o.table_eighths;
     pl->rows_per_wave = o.rows_per_wave;
+    pl->overflow_slots = o.overflow_slots;
     if (o.n_buckets == 1 && !o.use_graph && !o.concurrent) {
         rc = build_plan(pl.get(), query_ids, L_per_query, K_per_query, init_ids, n_init);
         if (rc) return rc;
@@ -1187,6 +1195,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
         part->want_queue = pl->want_queue;
         part->table_eighths = o.table_eighths;
         part->rows_per_wave = o.rows_per_wave;
+        part->overflow_slots = o.overflow_slots;
         part->slot0 = o.concurrent ? slot : 0u;
         part->counter_idx = counter++;
         std::vector<uint32_t> q(m.size()), lq(m.size()), kq(m.size());
@@ -1311,13 +1320,18 @@ int panns_plan_autotune(panns_index *h, const panns_params *p, const uint32_t *q
             if (rc) return rc;
         }
     }
-    // stage 2: smaller visited tables (more workgroups per CU, more spills)
+    // stage 2: smaller visited tables (more workgroups per CU, more ids
+    // past their buckets), with or without an LDS overflow area for those
     const panns_plan_options s1 = *best;
-    for (uint32_t eighths : {7u, 6u, 5u, 4u}) {
-        panns_plan_options o = s1;
-        o.table_eighths = eighths;
-        int rc = consider(o);
-        if (rc) return rc;
+    for (uint32_t eighths : {8u, 7u, 6u, 5u, 4u}) {
+        for (uint32_t ovf : {0u, 256u, 512u}) {
+            if (eighths == 8u && ovf == 0u) continue;  // stage 1's shape
+            panns_plan_options o = s1;
+            o.table_eighths = eighths;
+            o.overflow_slots = ovf;
+            int rc = consider(o);
+            if (rc) return rc;
+        }
     }
     // stage 3: several waves per query, where the search allows it
     if (p->mode != PANNS_MODE_PSS_V5 && p->stop == PANNS_STOP_FIXED) {

@@ -93,6 +93,7 @@ struct KArgs {
     uint32_t Lcap;
     uint32_t tbl_words, tbl_buckets, tbl_magic, id_bits, use_table;  // LDS visited table
     uint32_t tbl_bshift;  // log2 of 32-bit words per bucket: 2 (8 slots) or 1 (4 slots)
+    uint32_t ovf_slots;   // LDS overflow entries (full ids) after the buckets: 0 or a power of two
     uint32_t Mcap, nb_cap, group_picks;
     uint32_t group_rows;  // neighbour rows a wave gathers per distance step: 32, 64 or 96
     // subsearch on a caller-held queue and visited bitmap
@@ -494,6 +495,8 @@ struct Visited {
     uint32_t *tbl;      // LDS: buckets x (1 << bshift) words
     uint32_t nb, magic, idbits, idmask, xshift, use;  // buckets, floor(2^32 / nb)
     uint32_t bshift;    // 2: buckets of 8 slots, 1: of 4 slots
+    uint32_t *ovf;      // LDS: overflow entries (full ids, kSent = empty), or null
+    uint32_t ovf_mask;  // entries - 1
     uint32_t *bitmap;   // global
     uint32_t *log;      // global: words touched in the bitmap, may be null
     uint32_t log_cap;
@@ -578,13 +581,37 @@ struct Visited {
         uint32_t b2 = b1 + 1u + __umulhi(tag * 0x85EBCA77u, nb - 1u);
         return b2 >= nb ? b2 - nb : b2;
     }
+    // Ids whose buckets are full try up to kOvfProbes entries of the
+    // overflow area (linear probing from a hash of the id, full ids, so no
+    // tag is needed): exact for the bucket argument -- entries only fill,
+    // so an id inserted at probe j finds the j entries before it taken, and
+    // an id sent to the bitmap finds all of its entries taken again.
+    static constexpr int kOvfProbes = 8;
+    __device__ __forceinline__ int probe_overflow(uint32_t id, uint32_t h) const
+    {
+        if (!ovf) return 2;
+#pragma unroll 1
+        for (int j = 0; j < kOvfProbes; ++j) {
+            uint32_t *w = ovf + ((h + (uint32_t)j) & ovf_mask);
+            uint32_t v = *w;
+            if (v == id) return 0;
+            if (v == kSent) {
+                v = atomicCAS(w, kSent, id);
+                if (v == kSent) return 1;
+                if (v == id) return 0;
+            }
+        }
+        return 2;
+    }
     __device__ __forceinline__ int probe(uint32_t id) const
     {
         uint32_t b1, tag;
         locate(id, b1, tag);
-        const int r = probe_bucket(b1, tag);
-        if (r != 2 || tag == 0x7FFFu) return r;
-        return probe_bucket(second(b1, tag), tag | 0x8000u);
+        int r = probe_bucket(b1, tag);
+        if (r != 2) return r;
+        if (tag != 0x7FFFu) r = probe_bucket(second(b1, tag), tag | 0x8000u);
+        if (r != 2) return r;
+        return probe_overflow(id, b1 * 0x9E3779B1u + tag);
     }
 };
 
@@ -830,6 +857,8 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
     vis.nb = a.tbl_buckets;
     vis.magic = a.tbl_magic;
     vis.bshift = a.tbl_bshift;
+    vis.ovf = a.ovf_slots ? tbl + (a.tbl_buckets << a.tbl_bshift) : nullptr;
+    vis.ovf_mask = a.ovf_slots - 1u;
     vis.idbits = a.id_bits;
     vis.idmask = a.id_bits >= 32 ? ~0u : (1u << a.id_bits) - 1u;
     vis.xshift = a.id_bits / 2 + 1;

@@ -62,6 +62,8 @@ __global__ __launch_bounds__(64 * W, 2) void search_kernel_mw(KArgs a)
     vis.nb = a.tbl_buckets;
     vis.magic = a.tbl_magic;
     vis.bshift = a.tbl_bshift;
+    vis.ovf = a.ovf_slots ? tbl + (a.tbl_buckets << a.tbl_bshift) : nullptr;
+    vis.ovf_mask = a.ovf_slots - 1u;
     vis.idbits = a.id_bits;
     vis.idmask = a.id_bits >= 32 ? ~0u : (1u << a.id_bits) - 1u;
     vis.xshift = a.id_bits / 2 + 1;

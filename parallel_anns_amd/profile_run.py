@@ -1,7 +1,7 @@
 """A fixed sequence of search launches for hardware profilers.
 
-  python -m parallel_anns_amd.profile_run --op M,L[,eighths[,rows]]
-                                          [--learned-op M,Lmax,threshold[,eighths[,min_iter,every[,rows]]]]
+  python -m parallel_anns_amd.profile_run --op M,L[,eighths[,rows[,overflow]]]
+                                          [--learned-op M,Lmax,threshold[,eighths[,min_iter,every[,rows[,overflow]]]]]
                                           [--config c1] [--steps 5]
 
 Loads the prepared workload from the cache (and the stop model bench.py
@@ -42,9 +42,9 @@ def main():
     idx.set_graph(ds["adj"], ds["deg"], ds["ep"])
     qids = np.arange(nq, dtype=np.uint32)
 
-    def run(M, L, eighths, rows=0, **kw):
+    def run(M, L, eighths, rows=0, ovf=0, **kw):
         pl = idx.plan(qids, K, L, M, want_queue=False,
-                      options=Index.plan_options(table_eighths=eighths, rows_per_wave=rows), **kw)
+                      options=Index.plan_options(table_eighths=eighths, rows_per_wave=rows, overflow_slots=ovf), **kw)
         pl.run()
         ms = [pl.run() for _ in range(args.steps)]
         res = pl.fetch()
@@ -52,19 +52,20 @@ def main():
         c = res["counters"]
         alg = 4.0 * dim * c["distance_computations"] + 4.0 * (c["expansions"] + c["adjacency_slots"]) \
             + nq * (4.0 * dim + 8.0 * K)
-        return {"M": M, "L": L, "table_eighths": eighths, "rows_per_wave": rows,
+        return {"M": M, "L": L, "table_eighths": eighths, "rows_per_wave": rows, "overflow_slots": ovf,
                 "kernel_ms": round(float(np.mean(ms)), 4),
                 "algorithmic_bytes_per_launch": int(alg)}
 
     out = {"config": args.config, "steps": args.steps}
     f = [x for x in args.op.split(",")]
-    out["fixed"] = run(int(f[0]), int(f[1]), int(f[2]) if len(f) > 2 else 8, int(f[3]) if len(f) > 3 else 0)
+    out["fixed"] = run(int(f[0]), int(f[1]), int(f[2]) if len(f) > 2 else 8, int(f[3]) if len(f) > 3 else 0,
+                       int(f[4]) if len(f) > 4 else 0)
     if args.learned_op:
         g = args.learned_op.split(",")
         model = np.load(os.path.join(ds["dir"] or args.cache_dir, "stop_model.npy"))
         idx.set_stop_model(model)
         out["learned"] = run(int(g[0]), int(g[1]), int(g[3]) if len(g) > 3 else 8,
-                             int(g[6]) if len(g) > 6 else 0, stop="learned",
+                             int(g[6]) if len(g) > 6 else 0, int(g[7]) if len(g) > 7 else 0, stop="learned",
                              stop_threshold=float(g[2]), stop_min_iter=int(g[4]) if len(g) > 4 else 0,
                              stop_every=int(g[5]) if len(g) > 5 else 1)
         out["learned"]["threshold"] = float(g[2])

@@ -230,7 +230,7 @@ class Index:
 
     @staticmethod
     def plan_options(buckets=1, bucket_max_L=None, table_eighths=8, use_graph=False, concurrent=False,
-                     want_queue=False, waves=0, rows_per_wave=0):
+                     want_queue=False, waves=0, rows_per_wave=0, overflow_slots=0):
         o = _native.PannsPlanOptions()
         o.n_buckets = len(bucket_max_L) if bucket_max_L else buckets
         for i, v in enumerate(bucket_max_L or ()):
@@ -239,6 +239,7 @@ class Index:
         o.want_queue = int(want_queue)
         o.waves = waves
         o.rows_per_wave = rows_per_wave
+        o.overflow_slots = overflow_slots
         return o
 
     def plan(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, L_per_query=None,

@@ -88,7 +88,7 @@ def main():
     tune = {"seconds": round(time.time() - t0, 2), "sample_ms": round(sample_ms, 3),
             "n_buckets": opt.n_buckets, "bucket_max_L": [opt.bucket_max_L[i] for i in range(opt.n_buckets)],
             "concurrent": opt.concurrent, "table_eighths": opt.table_eighths, "waves": max(1, opt.waves), "use_graph": opt.use_graph,
-            "rows_per_wave": opt.rows_per_wave}
+            "rows_per_wave": opt.rows_per_wave, "overflow_slots": opt.overflow_slots}
     log(f"autotuner: {tune}")
     res_tuned, forms["autotuned"] = timed(opt, "autotuned")
     forms["autotuned"]["choice"] = tune

@@ -125,6 +125,39 @@ def test_four_slot_buckets_keep_results(small_set, gpu_index, slots):
         gpu_index.search(qids[:2], K, L, M, init_ids=init, bucket_slots=6)
 
 
+@pytest.mark.parametrize("slots,ovf,bs", [(64, 256, 8), (192, 64, 8), (1536, 1024, 8), (128, 512, 4)])
+def test_overflow_area_keeps_results(small_set, gpu_index, slots, ovf, bs):
+    """Ids whose buckets are full go to the LDS overflow entries, then to the
+    bitmap; the results stay the oracle's and fewer ids reach the bitmap."""
+    ds = small_set
+    K, L, M = 100, 160, 4
+    qids = np.arange(48, dtype=np.uint32)
+    _print_case(test="overflow", hash_slots=slots, overflow=ovf, bucket_slots=bs)
+    init = gpu_index.prepare_init_ids(L)
+    base = gpu_index.search(qids, K, L, M, init_ids=init, hash_slots=slots, bucket_slots=bs)
+    pl = gpu_index.plan(qids, K, L, M, init_ids=init, hash_slots=slots, bucket_slots=bs, want_queue=True,
+                        options=Index.plan_options(overflow_slots=ovf))
+    pl.run()
+    res = pl.fetch(want_queue=True)
+    pl.close()
+    sp0, sp1 = base["counters"]["visited_spills"], res["counters"]["visited_spills"]
+    assert sp1 < sp0 or sp0 == sp1 == 0
+    ref = oracle_run(ds, qids, K, L, M, init)
+    for i in range(qids.shape[0]):
+        assert (res["ids"][i] == ref[i]["ids"]).all(), f"query {i}"
+        assert (res["queue_ids"][i] == ref[i]["queue"]["id"]).all()
+        assert res["ndist"][i] == ref[i]["ndist"]
+    for w in (2, 4):
+        pm = gpu_index.plan(qids, K, L, M, init_ids=init, hash_slots=slots, bucket_slots=bs, want_queue=False,
+                            options=Index.plan_options(overflow_slots=ovf, waves=w))
+        pm.run()
+        r2 = pm.fetch()
+        pm.close()
+        assert (r2["ids"] == res["ids"]).all() and (r2["ndist"] == res["ndist"]).all()
+    with pytest.raises(SearchError):
+        gpu_index.plan(qids[:2], K, L, M, init_ids=init, options=Index.plan_options(overflow_slots=100))
+
+
 def test_uneven_plan_matches_uniform(small_set, gpu_index):
     ds = small_set
     rng = np.random.default_rng(5)
