@@ -112,13 +112,19 @@ def find_L(index, qids, K, M, gt, target, lo, hi, **kw):
     return lo
 
 
-def cpu_baseline(ds, K, L, M, init_ids, budget_s=12.0):
+def host_cores():
+    """CPUs this process may run on (at most 16, the host share of one GPU).
+    Read before any OpenMP region: with OMP_PROC_BIND the runtime binds the
+    calling thread to one place, after which the affinity mask shows one CPU."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(ds, K, L, M, init_ids, cores, budget_s=12.0):
     """The oracle restatement of top-M search, inter-query parallel on the
     host cores (num_t = 1 per query), timed on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    cores = max(1, min(cores, 16))
     g = po.CsrGraph(ds["adj"], ds["deg"], ds["ep"])
     q = ds["queries"]
     n_probe = min(200, q.shape[0])
@@ -224,10 +230,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
 
-    # the CPU baseline's threads pinned one per core, close together (set
-    # before any OpenMP runtime starts)
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    os.environ.setdefault("OMP_PLACES", "cores")
+    # host CPUs for the CPU baseline, counted before any OpenMP region runs
+    # (a binding policy in the environment would pin this thread to one)
+    cores = host_cores()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; queries shard across GPUs with no "
                                    "exchange step, so N>1 is not offered", "n_gpus": args.gpus}))
@@ -385,7 +390,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         init = index.prepare_init_ids(fL)
-        cpu, cpu_ids = cpu_baseline(ds, K, fL, fM, init)
+        cpu, cpu_ids = cpu_baseline(ds, K, fL, fM, init, cores)
         cpu["ids_identical_to_gpu"] = bool((cpu_ids == fres["ids"][:cpu_ids.shape[0]]).all())
         log(f"cpu baseline {cpu['value']:,.0f} QPS on {cpu['cores']} cores; ids identical: {cpu['ids_identical_to_gpu']}")
     kernel = f"search_kernel<{dim}, {'true' if lrn and not lrn['fallback_to_fixed'] else 'false'}>"
