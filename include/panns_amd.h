@@ -65,7 +65,9 @@ typedef struct {
     uint32_t stop;         /* panns_stop */
     float stop_threshold;  /* LEARNED: stop when sigmoid output >= threshold */
     uint32_t hash_slots;   /* 0 = automatic; else a multiple of 64 in [64, 32768]: 16-bit slots of
-                              the per-query LDS visited table (raised when the id space needs) */
+                              the per-query LDS visited table (raised when the id space needs).
+                              Automatic: 8,192 up to L = 512, 10,240 above; without plan options
+                              and up to L = 400, 7,168 plus 256 overflow entries */
     uint32_t waves;        /* wavefronts per query: 0 or 1, else 2 or 4 (TOP_M and SCALE_M with the
                               fixed stop): the picks of an iteration are expanded by several waves
                               and merged into the master queue, the deterministic form of the
