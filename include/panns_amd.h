@@ -215,7 +215,7 @@ typedef struct {
                                                   kernel, whose registers hold the model) */
     uint32_t overflow_slots;                   /* LDS overflow entries (full ids) for visited ids whose
                                                   buckets are full, before the global bitmap: 0 or
-                                                  a power of two up to 4096 */
+                                                  a power of two in [8, 4096] */
     uint32_t reserved[1];
 } panns_plan_options;
 

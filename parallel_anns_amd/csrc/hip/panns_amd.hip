@@ -1125,8 +1125,9 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
     if (o.table_eighths > 8) return fail(PANNS_ERR_ARG, "table_eighths must be 1..8");
     if (o.rows_per_wave && o.rows_per_wave != 32 && o.rows_per_wave != 64 && o.rows_per_wave != 96)
         return fail(PANNS_ERR_ARG, "rows_per_wave must be 0, 32, 64 or 96");
-    if (o.overflow_slots && ((o.overflow_slots & (o.overflow_slots - 1u)) || o.overflow_slots > 4096u))
-        return fail(PANNS_ERR_ARG, "overflow_slots must be 0 or a power of two up to 4096");
+    if (o.overflow_slots &&
+        ((o.overflow_slots & (o.overflow_slots - 1u)) || o.overflow_slots < 8u || o.overflow_slots > 4096u))
+        return fail(PANNS_ERR_ARG, "overflow_slots must be 0 or a power of two in [8, 4096]");
     HIP_TRY(hipSetDevice(h->device));
     std::unique_ptr<panns_plan> pl(new panns_plan());
     pl->h = h;

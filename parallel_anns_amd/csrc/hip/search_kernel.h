@@ -581,24 +581,31 @@ struct Visited {
         uint32_t b2 = b1 + 1u + __umulhi(tag * 0x85EBCA77u, nb - 1u);
         return b2 >= nb ? b2 - nb : b2;
     }
-    // Ids whose buckets are full try up to kOvfProbes entries of the
-    // overflow area (linear probing from a hash of the id, full ids, so no
-    // tag is needed): exact for the bucket argument -- entries only fill,
-    // so an id inserted at probe j finds the j entries before it taken, and
-    // an id sent to the bitmap finds all of its entries taken again.
-    static constexpr int kOvfProbes = 8;
+    // Ids whose buckets are full try the overflow area: full ids (no tag
+    // needed) in groups of four entries, one 16-byte read per group, two
+    // consecutive groups from a hash of the id.  Exact for the bucket
+    // argument: entries only fill and are taken in order within a group, so
+    // an id inserted somewhere finds every entry before it taken, and an id
+    // sent to the bitmap finds all of its entries taken again.  ovf_mask is
+    // the entry count - 1 (entries a multiple of 4).
+    static constexpr int kOvfGroups = 2;
     __device__ __forceinline__ int probe_overflow(uint32_t id, uint32_t h) const
     {
         if (!ovf) return 2;
+        const uint32_t g0 = h & (ovf_mask >> 2);
 #pragma unroll 1
-        for (int j = 0; j < kOvfProbes; ++j) {
-            uint32_t *w = ovf + ((h + (uint32_t)j) & ovf_mask);
-            uint32_t v = *w;
-            if (v == id) return 0;
-            if (v == kSent) {
-                v = atomicCAS(w, kSent, id);
-                if (v == kSent) return 1;
-                if (v == id) return 0;
+        for (int gi = 0; gi < kOvfGroups; ++gi) {
+            uint32_t *w = ovf + (((g0 + (uint32_t)gi) & (ovf_mask >> 2)) << 2);
+#pragma unroll 1
+            for (int attempt = 0; attempt < 5; ++attempt) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(w);
+                if (v.x == id || v.y == id || v.z == id || v.w == id) return 0;
+                const int e = v.x == kSent ? 0 : v.y == kSent ? 1 : v.z == kSent ? 2 : v.w == kSent ? 3 : -1;
+                if (e < 0) break;  // group full: the next one
+                const uint32_t old = atomicCAS(w + e, kSent, id);
+                if (old == kSent) return 1;
+                if (old == id) return 0;
+                // another id took entry e: read the group again
             }
         }
         return 2;
