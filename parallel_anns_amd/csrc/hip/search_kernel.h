@@ -512,9 +512,20 @@ struct Visited {
     // was inserted, 2 if the bucket is full without it.  Every failed CAS
     // means another slot of the bucket was filled, so at most 2 * NW retries
     // precede a full bucket or a success.
+    // The slot tests are word-parallel and written as selects, so a probe is
+    // straight-line code between its LDS read and its CAS: x has a zero
+    // 16-bit half iff (x - 0x00010001) & ~x & 0x80008000 != 0 (a borrow into
+    // the high half happens only when the low half is zero, which already
+    // makes the test true).  Matches are zero halves of word ^ (tag, tag),
+    // empty slots (0xFFFF) zero halves of ~word.
+    static __device__ __forceinline__ bool has_zero_half(uint32_t x)
+    {
+        return ((x - 0x00010001u) & ~x & 0x80008000u) != 0u;
+    }
     template <int NW>
     __device__ __forceinline__ int probe_words(uint32_t *w, uint32_t tag) const
     {
+        const uint32_t t2 = tag * 0x00010001u;
 #pragma unroll 1
         for (int attempt = 0; attempt <= 2 * NW; ++attempt) {
             uint32_t words[NW];
@@ -529,19 +540,18 @@ struct Visited {
                 words[0] = v.x;
                 words[1] = v.y;
             }
-            int ew = -1;
-            uint32_t eshift = 0, eold = 0;
+            uint32_t hm = 0, ew = NW, eold = 0;  // ew: first word with an empty slot
 #pragma unroll
-            for (int i = 0; i < NW; ++i) {
-                const uint32_t lo = words[i] & 0xFFFFu, hi = words[i] >> 16;
-                if (lo == tag || hi == tag) return 0;
-                if (ew < 0 && (lo == 0xFFFFu || hi == 0xFFFFu)) {
-                    ew = i;
-                    eshift = lo == 0xFFFFu ? 0u : 16u;
-                    eold = words[i];
-                }
+            for (int i = NW - 1; i >= 0; --i) {
+                const uint32_t x = words[i] ^ t2;
+                hm |= (x - 0x00010001u) & ~x;
+                const bool e = has_zero_half(~words[i]);
+                ew = e ? (uint32_t)i : ew;
+                eold = e ? words[i] : eold;
             }
-            if (ew < 0) return 2;
+            if (hm & 0x80008000u) return 0;
+            if (ew == NW) return 2;
+            const uint32_t eshift = (eold & 0xFFFFu) == 0xFFFFu ? 0u : 16u;
             const uint32_t nw = (eold & ~(0xFFFFu << eshift)) | (tag << eshift);
             if (atomicCAS(&w[ew], eold, nw) == eold) return 1;
         }
