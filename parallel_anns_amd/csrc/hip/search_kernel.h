@@ -295,6 +295,24 @@ __device__ __forceinline__ uint64_t sort64(uint64_t x, int lane, const XorSel &s
     return bitonic_merge<64>(x, lane, s);
 }
 
+// sort64 for a batch held in lanes [0, n) with kKeyMax in every lane above:
+// the merges of width k sort each k-lane block on its own, and the block
+// [0, k) ascending, so once k >= n that block holds the whole batch in order
+// and every lane above it still holds kKeyMax -- the wider merges would
+// change nothing.  n is wave-uniform.
+__device__ __forceinline__ uint64_t sort64_first(uint64_t x, uint32_t n, int lane, const XorSel &s)
+{
+    x = bitonic_merge<2>(x, lane, s);
+    x = bitonic_merge<4>(x, lane, s);
+    x = bitonic_merge<8>(x, lane, s);
+    if (n <= 8u) return x;
+    x = bitonic_merge<16>(x, lane, s);
+    if (n <= 16u) return x;
+    x = bitonic_merge<32>(x, lane, s);
+    if (n <= 32u) return x;
+    return bitonic_merge<64>(x, lane, s);
+}
+
 // Block order of the reference's 8-wide accumulation: the tail block first
 // when (D/8) is odd, then blocks 0, 1, ...
 template <int D>
@@ -1046,7 +1064,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                 for (uint32_t b0 = 0; b0 < ncand; b0 += 64) {
                     const uint32_t cnt = min(64u, ncand - b0);
                     uint64_t x = (uint32_t)lane < cnt ? cand[b0 + lane] : kKeyMax;
-                    x = sort64(x, lane, xs);
+                    x = sort64_first(x, cnt, lane, xs);
                     const uint32_t p0 = merge_batch(Q, S, L, x, cnt, bb, lane);
                     if (p0 != kSent) nk = min(nk, p0);
                 }

@@ -193,7 +193,7 @@ __global__ __launch_bounds__(64 * W, 2) void search_kernel_mw(KArgs a)
                     for (uint32_t b0 = 0; b0 < nr; b0 += 64) {
                         const uint32_t cnt = min(64u, nr - b0);
                         uint64_t x = (uint32_t)lane < cnt ? cr[b0 + lane] : kKeyMax;
-                        x = sort64(x, lane, xs);
+                        x = sort64_first(x, cnt, lane, xs);
                         const uint32_t p0 = merge_batch<true>(Q, S, L, x, cnt, bb, lane);
                         if (p0 != kSent) nk = min(nk, p0);
                     }
