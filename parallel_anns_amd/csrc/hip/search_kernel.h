@@ -638,10 +638,51 @@ struct Visited {
         }
         return 2;
     }
+    // 8-slot buckets: both buckets of the id read at once and decided in
+    // straight-line code.  The tag is looked for in both (as t in b1, as
+    // t | 0x8000 in b2); if absent it goes into the first empty slot of b1,
+    // or, b1 being full, of b2, by one CAS; a failed CAS means a slot of the
+    // two buckets was filled meanwhile, so at most 16 retries precede a
+    // success, a hit or two full buckets.  Slots fill in order (every insert
+    // takes the lowest empty slot of its snapshot, and a stale snapshot fails
+    // its CAS), so a word has an empty slot iff its high half is empty, and
+    // the low half is empty too iff the whole word is.  t = 0x7FFF has no
+    // second bucket: b1 stands in for it, which finds and inserts the same.
+    __device__ __forceinline__ int probe_pair(uint32_t id, uint32_t b1, uint32_t tag) const
+    {
+        const bool has2 = tag != 0x7FFFu;
+        const uint32_t tg2 = has2 ? (tag | 0x8000u) : tag;
+        uint32_t *w1 = tbl + (b1 << 2);
+        uint32_t *w2 = tbl + ((has2 ? second(b1, tag) : b1) << 2);
+        const uint32_t t1 = tag * 0x00010001u, t2 = tg2 * 0x00010001u;
+#pragma unroll 1
+        for (int attempt = 0; attempt <= 16; ++attempt) {
+            const uint4 v1 = *reinterpret_cast<const uint4 *>(w1);
+            const uint4 v2 = *reinterpret_cast<const uint4 *>(w2);
+            const uint32_t a[8] = {v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, v2.z, v2.w};
+            uint32_t hm = 0, ew = 8, eold = 0;
+#pragma unroll
+            for (int i = 7; i >= 0; --i) {
+                const uint32_t x = a[i] ^ (i < 4 ? t1 : t2);
+                hm |= (x - 0x00010001u) & ~x;
+                const bool e = a[i] >= 0xFFFF0000u;
+                ew = e ? (uint32_t)i : ew;
+                eold = e ? a[i] : eold;
+            }
+            if (hm & 0x80008000u) return 0;
+            if (ew == 8u) return probe_overflow(id, b1 * 0x9E3779B1u + tag);
+            const uint32_t t = ew < 4u ? tag : tg2;
+            const uint32_t nw = eold == kSent ? (0xFFFF0000u | t) : ((eold & 0xFFFFu) | (t << 16));
+            uint32_t *w = ew < 4u ? w1 + ew : w2 + (ew - 4u);
+            if (atomicCAS(w, eold, nw) == eold) return 1;
+        }
+        return probe_overflow(id, b1 * 0x9E3779B1u + tag);
+    }
     __device__ __forceinline__ int probe(uint32_t id) const
     {
         uint32_t b1, tag;
         locate(id, b1, tag);
+        if (bshift == 2u) return probe_pair(id, b1, tag);
         int r = probe_bucket(b1, tag);
         if (r != 2) return r;
         if (tag != 0x7FFFu) r = probe_bucket(second(b1, tag), tag | 0x8000u);
