@@ -166,10 +166,10 @@ struct panns_plan {
 
 namespace {
 
-template <int D, bool kLearned, bool kNorm>
+template <int D, bool kLearned, bool kNorm, bool kTrace = false>
 hipError_t launch_search(const panns::KArgs &a, uint32_t grid, uint32_t lds, hipStream_t s)
 {
-    hipLaunchKernelGGL((panns::search_kernel<D, kLearned, kNorm>), dim3(grid), dim3(64), lds, s, a);
+    hipLaunchKernelGGL((panns::search_kernel<D, kLearned, kNorm, kTrace>), dim3(grid), dim3(64), lds, s, a);
     return hipGetLastError();
 }
 
@@ -185,11 +185,16 @@ using KernelPtr = const void *;
 
 // The search kernel for a dimension; `learned` selects the variant that
 // evaluates the stop model (squared L2 only), `norm` the opt-graph distance.
-bool kernel_for_dim(uint32_t dim, bool learned, bool norm, LaunchFn *fn, KernelPtr *kp)
+bool kernel_for_dim(uint32_t dim, bool learned, bool norm, bool trace, LaunchFn *fn, KernelPtr *kp)
 {
     switch (dim) {
 #define PANNS_DIM(DD)                                                                                   \
     case DD:                                                                                            \
+        if (trace) {                                                                                    \
+            *fn = launch_search<DD, false, false, true>;                                                \
+            *kp = reinterpret_cast<const void *>(&panns::search_kernel<DD, false, false, true>);        \
+            return true;                                                                                \
+        }                                                                                               \
         *fn = norm ? launch_search<DD, false, true>                                                     \
                    : (learned ? launch_search<DD, true, false> : launch_search<DD, false, false>);       \
         *kp = norm ? reinterpret_cast<const void *>(&panns::search_kernel<DD, false, true>)             \
@@ -236,9 +241,10 @@ bool kernel_for_dim_mw(uint32_t dim, uint32_t waves, bool norm, LaunchFn *fn, Ke
     }
 }
 
-bool select_kernel(uint32_t dim, bool learned, bool norm, uint32_t waves, LaunchFn *fn, KernelPtr *kp)
+bool select_kernel(uint32_t dim, bool learned, bool norm, bool trace, uint32_t waves, LaunchFn *fn, KernelPtr *kp)
 {
-    return waves > 1 ? kernel_for_dim_mw(dim, waves, norm, fn, kp) : kernel_for_dim(dim, learned, norm, fn, kp);
+    return waves > 1 ? kernel_for_dim_mw(dim, waves, norm, fn, kp)
+                     : kernel_for_dim(dim, learned, norm, trace, fn, kp);
 }
 
 using KnnLaunch = void (*)(const float *, uint32_t, const float *, uint32_t, uint32_t, int, uint32_t,
@@ -446,7 +452,7 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     if (norm && pl->trace_cap) return fail(PANNS_ERR_UNSUPPORTED, "traces are taken with squared-L2 distances");
     LaunchFn fn;
     KernelPtr kp;
-    if (!select_kernel(h->dim, p.stop == PANNS_STOP_LEARNED, norm, pl->waves, &fn, &kp))
+    if (!select_kernel(h->dim, p.stop == PANNS_STOP_LEARNED, norm, pl->trace_cap != 0, pl->waves, &fn, &kp))
         return fail(PANNS_ERR_DIM, "dimension " + std::to_string(h->dim) + " has no kernel (16, 32, 40, 64, 96, 128)");
     pl->maxL = p.L;
     pl->maxK = p.K;
@@ -665,7 +671,8 @@ hipError_t launch_one(const panns_plan *pl, hipStream_t st)
 {
     LaunchFn fn;
     KernelPtr kp;
-    select_kernel(pl->h->dim, pl->args.stop_learned != 0, pl->args.norms != nullptr, pl->waves, &fn, &kp);
+    select_kernel(pl->h->dim, pl->args.stop_learned != 0, pl->args.norms != nullptr, pl->args.trace_feat != nullptr,
+                  pl->waves, &fn, &kp);
     return fn(pl->args, pl->grid, pl->lds, st);
 }
 

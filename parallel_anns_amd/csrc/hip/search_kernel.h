@@ -908,8 +908,11 @@ __device__ __forceinline__ uint32_t next_work(const KArgs &a, uint32_t &r, uint3
 
 // kLearned: the variant that evaluates the stop model (a.stop_learned);
 // the other ignores it.  kNorm: the opt-graph distance norm - 2 v.q
-// (a.norms) instead of squared L2.
-template <int D, bool kLearned, bool kNorm>
+// (a.norms) instead of squared L2.  kTrace: the fixed-L search that also
+// records the stop features and top-K hashes (a.trace_feat, a.trace_topk);
+// the search variants carry none of that code, which keeps their uniform
+// state within the scalar registers.
+template <int D, bool kLearned, bool kNorm, bool kTrace = false>
 __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
 {
     extern __shared__ uint64_t smem[];
@@ -1108,7 +1111,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
             if (a.mode == kModeScaleM && Mcur < a.M) Mcur <<= 1;
 
             const bool eval_stop = kLearned && niter >= a.stop_min_iter && niter % a.stop_every == 0;
-            if (eval_stop || a.trace_feat) {
+            if (eval_stop || kTrace) {
                 // features: step, top-1 dist, K-th dist, ratio, new keys below
                 // the queue's last key in the last step, iterations since the
                 // top-K set last changed, first unchecked position, unchecked
@@ -1128,7 +1131,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                 f[5] = (float)since_topk;
                 f[6] = (float)k;
                 f[7] = (float)unc;
-                if (a.trace_feat && niter <= a.trace_cap) {
+                if (kTrace && niter <= a.trace_cap) {
                     float *tf = a.trace_feat + ((size_t)w * a.trace_cap + (niter - 1)) * kFeat;
                     if (lane < kFeat) tf[lane] = f[lane];
                     uint32_t hsh = 0;

@@ -58,8 +58,8 @@ def report(a):
         if name not in run:
             continue
         r = run[name]
-        # squared-L2 instantiation: search_kernel<D, learned, norm = false>
-        match = f"search_kernel<{dim}, {variant}, false>"
+        # squared-L2 search instantiation: search_kernel<D, learned, norm = false, trace = false>
+        match = f"search_kernel<{dim}, {variant}, false, false>"
         d = [x[1] / 1e3 for x in kernel_durations(a.trace, match)][-steps:]
         fv = [x[2] for x in counter_values(a.fetch, "FETCH_SIZE", match)][-steps:]
         wv = [x[2] for x in counter_values(a.write, "WRITE_SIZE", match)][-steps:]
