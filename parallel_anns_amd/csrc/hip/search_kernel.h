@@ -284,9 +284,20 @@ __device__ __forceinline__ uint64_t bitonic_merge(uint64_t x, int lane, const Xo
     return bitonic_step<K, 1>(x, lane, s);
 }
 
+// The lane id as a value the compiler cannot see through, so the step
+// directions derived from it are computed where they are used (two VALU
+// instructions a step) instead of being hoisted out of the query loop as 21
+// lane masks that outgrow the scalar registers and are spilled.
+__device__ __forceinline__ int opaque_lane(int lane)
+{
+    asm volatile("" : "+v"(lane));
+    return lane;
+}
+
 // Bitonic sort of one key per lane, ascending across the 64 lanes.
 __device__ __forceinline__ uint64_t sort64(uint64_t x, int lane, const XorSel &s)
 {
+    lane = opaque_lane(lane);
     x = bitonic_merge<2>(x, lane, s);
     x = bitonic_merge<4>(x, lane, s);
     x = bitonic_merge<8>(x, lane, s);
@@ -302,6 +313,7 @@ __device__ __forceinline__ uint64_t sort64(uint64_t x, int lane, const XorSel &s
 // change nothing.  n is wave-uniform.
 __device__ __forceinline__ uint64_t sort64_first(uint64_t x, uint32_t n, int lane, const XorSel &s)
 {
+    lane = opaque_lane(lane);
     x = bitonic_merge<2>(x, lane, s);
     x = bitonic_merge<4>(x, lane, s);
     x = bitonic_merge<8>(x, lane, s);
