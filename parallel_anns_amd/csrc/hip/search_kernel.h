@@ -767,12 +767,12 @@ __device__ __forceinline__ uint32_t claim_chunks(Visited &vis, uint32_t (&ids)[k
 // The per-lane part of the stop model, kept in registers for the kernel's
 // life: lane j's row of W1, its biases, its output weight, and its row of
 // the 64x64 layer as 32 pairs of bf16 (round to nearest even; element 2i in
-// the low half of w2[i]); the feature means and scales are the same for all
-// lanes.
+// the low half of w2[i]); the feature means and inverse scales are the same
+// for all lanes.
 struct LaneModel {
     float w1[kFeat];
     float b1, b2, w3, b3;
-    float mu[kFeat], sd[kFeat];
+    float mu[kFeat], isd[kFeat];
     uint32_t w2[kHid / 2];
 };
 
@@ -794,7 +794,7 @@ __device__ __forceinline__ LaneModel load_lane_model(const StopModel &m, int lan
 #pragma unroll
     for (int i = 0; i < kFeat; ++i) {
         r.mu[i] = m.mu[i];
-        r.sd[i] = m.sd[i];
+        r.isd[i] = 1.0f / m.sd[i];
     }
     const float2 *row = reinterpret_cast<const float2 *>(m.w2 + lane * kHid);
 #pragma unroll
@@ -807,16 +807,16 @@ __device__ __forceinline__ LaneModel load_lane_model(const StopModel &m, int lan
 
 // One-wave MLP: lane j computes hidden unit j of both layers, in fp32 with
 // the 64x64 layer's weights in bf16 (as the training kernels use them);
-// features enter as (f - mu) / sd, the sums run in index order.  Returns the
-// sigmoid output in every lane.  (Folding the scaling into the first layer
-// would cancel catastrophically for a feature of near-zero spread, such as
-// the norm of normalised queries.)
+// features enter as (f - mu) * (1 / sd) (a multiply per feature instead of
+// an IEEE division), the sums run in index order.  Returns the sigmoid
+// output in every lane.  (Folding the scaling into the first layer would
+// cancel catastrophically for a feature of near-zero spread.)
 __device__ __forceinline__ float mlp_eval(const LaneModel &lm, const float (&f)[kFeat], float *h1s, int lane,
                                           const XorSel &xs)
 {
     float h = lm.b1;
 #pragma unroll
-    for (int i = 0; i < kFeat; ++i) h = __builtin_fmaf(lm.w1[i], (f[i] - lm.mu[i]) / lm.sd[i], h);
+    for (int i = 0; i < kFeat; ++i) h = __builtin_fmaf(lm.w1[i], (f[i] - lm.mu[i]) * lm.isd[i], h);
     h = h > 0.f ? h : 0.f;
     h1s[lane] = h;
     __syncthreads();
