@@ -980,6 +980,11 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
         uint32_t S = 0, ndist = 0, nexp = 0, niter = 0, nadj = 0;
         uint32_t ins_last = 0;
         uint32_t since_topk = 0;  // iterations since the top-K set last changed
+        // the first iteration >= max(1, stop_min_iter) that is a multiple of
+        // stop_every (>= 1); later ones follow by addition
+        uint32_t next_eval = 0;
+        if constexpr (kLearned)
+            next_eval = (max(a.stop_min_iter, 1u) + a.stop_every - 1u) / a.stop_every * a.stop_every;
         if (!a.init_queue) {
             // initialize_set_L_para (PSS5LG.cpp:1055-1096): mark every init id
             // visited, compute its distance, sort the L keys.
@@ -1122,7 +1127,8 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
             ins_last = ins;
             if (a.mode == kModeScaleM && Mcur < a.M) Mcur <<= 1;
 
-            const bool eval_stop = kLearned && niter >= a.stop_min_iter && niter % a.stop_every == 0;
+            const bool eval_stop = kLearned && niter == next_eval;
+            if (eval_stop) next_eval += a.stop_every;
             if (eval_stop || kTrace) {
                 // features: step, top-1 dist, K-th dist, ratio, new keys below
                 // the queue's last key in the last step, iterations since the
