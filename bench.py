@@ -367,7 +367,18 @@ def main():
         for t in sorted(fixed_other):
             if t in picks:
                 lL, thr, skw = picks[t]
-                stop = dict(stop="learned", stop_threshold=thr, **skw) if thr < 1.0 else {}
+                if thr >= 1.0:
+                    # no threshold won: the learned line is the fixed-L line
+                    # itself (its own M, which may differ from the model's)
+                    fo = fixed_other[t]
+                    lrn_other[t] = {"fixed_L": fo,
+                                    "learned": {"M": fo["M"], "L_max": fo["L"], "threshold": thr, "qps": fo["qps"],
+                                                "recall": fo["recall"], "ndist": fo["ndist"],
+                                                "fallback_to_fixed": True},
+                                    "learned_over_fixed": 1.0}
+                    log(f"target {t}: {lrn_other[t]}")
+                    continue
+                stop = dict(stop="learned", stop_threshold=thr, **skw)
                 r = timed(index, qids, K, lL, lM, gt, 3, 1, **stop)[2]
                 lrn_other[t] = {"fixed_L": fixed_other[t],
                                 "learned": {"M": lM, "L_max": lL, "threshold": thr, "qps": r["qps"],
