@@ -874,8 +874,10 @@ __device__ __forceinline__ uint32_t distance_group(const float *__restrict__ bas
 // Distances for ids[0, cnt) in groups of `rows` neighbour rows per wave
 // (one, two or three rows per lane pair in flight): with rows = 96, groups
 // of 96 while more than 64 remain, then groups of 64; with 64, groups of 64;
-// with 32, groups of 32.  kWide = false caps groups at 64 (fewer registers;
-// the learned-stop kernel holds its model).
+// with 32, groups of 32 -- and a remainder of at most 32 rows always as one
+// group of 32, so a short pass does not compute 32 idle rows.  kWide = false
+// caps groups at 64 (fewer registers; the learned-stop kernel holds its
+// model).
 template <int D, bool kWide, bool kNorm>
 __device__ __forceinline__ uint32_t distance_pass(const float *__restrict__ base, const float *__restrict__ norms,
                                                   const uint32_t *ids, uint32_t cnt, const float4 *qs,
@@ -887,7 +889,7 @@ __device__ __forceinline__ uint32_t distance_pass(const float *__restrict__ base
         if (kWide && rows >= 96u && cnt - p > 64u) {
             ncand = distance_group<D, 3, kNorm>(base, norms, ids, p, cnt, qs, bound, cand, ncand, lane, lt);
             p += 96u;
-        } else if (rows >= 64u) {
+        } else if (rows >= 64u && cnt - p > 32u) {
             ncand = distance_group<D, 2, kNorm>(base, norms, ids, p, cnt, qs, bound, cand, ncand, lane, lt);
             p += 64u;
         } else {
