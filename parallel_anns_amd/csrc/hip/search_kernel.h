@@ -730,31 +730,39 @@ __device__ __forceinline__ uint32_t claim_chunks(Visited &vis, uint32_t (&ids)[k
             if (r == 2) gbits |= 1u << u;
         }
     }
+    // The bitmap path, only when some lane of the wave has an id there.
     // Every lane issues the atomic of a chunk that any lane needs, so the
     // batch is not split by exec-mask joins; lanes with nothing to claim OR 0
     // into a word of their own (their id's word, or word `lane` for an empty
     // slot), which changes nothing.
-    uint32_t old[kClaimChunks];
+    if (__ballot(gbits != 0u)) {
+        uint32_t old[kClaimChunks];
 #pragma unroll
-    for (int u = 0; u < kClaimChunks; ++u) {
-        old[u] = 0;
-        const bool g = (gbits >> u) & 1u;
-        if (__ballot(g)) {
-            const uint32_t wi = ids[u] != kSent ? ids[u] >> 5 : (uint32_t)(threadIdx.x & 63);
-            old[u] = atomicOr(&vis.bitmap[wi], g ? 1u << (ids[u] & 31) : 0u);
+        for (int u = 0; u < kClaimChunks; ++u) {
+            old[u] = 0;
+            const bool g = (gbits >> u) & 1u;
+            if (__ballot(g)) {
+                const uint32_t wi = ids[u] != kSent ? ids[u] >> 5 : (uint32_t)(threadIdx.x & 63);
+                old[u] = atomicOr(&vis.bitmap[wi], g ? 1u << (ids[u] & 31) : 0u);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kClaimChunks; ++u) {
+            if ((uint32_t)u < nch) {
+                const bool g = ((gbits >> u) & 1u) && !(old[u] & (1u << (ids[u] & 31)));
+                const uint64_t gm = __ballot(g);
+                if (g) {
+                    nbits |= 1u << u;
+                    const uint32_t i = vis.spills + (uint32_t)__popcll(gm & lt);
+                    if (vis.log && i < vis.log_cap) vis.log[i] = ids[u] >> 5;
+                }
+                vis.spills += (uint32_t)__popcll(gm);
+            }
         }
     }
 #pragma unroll
     for (int u = 0; u < kClaimChunks; ++u) {
         if ((uint32_t)u < nch) {
-            const bool g = ((gbits >> u) & 1u) && !(old[u] & (1u << (ids[u] & 31)));
-            const uint64_t gm = __ballot(g);
-            if (g) {
-                nbits |= 1u << u;
-                const uint32_t i = vis.spills + (uint32_t)__popcll(gm & lt);
-                if (vis.log && i < vis.log_cap) vis.log[i] = ids[u] >> 5;
-            }
-            vis.spills += (uint32_t)__popcll(gm);
             const bool nw = (nbits >> u) & 1u;
             const uint64_t nm = __ballot(nw);
             if (nw) newids[nnew + (uint32_t)__popcll(nm & lt)] = ids[u];
