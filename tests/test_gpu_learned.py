@@ -137,6 +137,12 @@ def test_stop_schedule(small_set, gpu_index):
     assert (late["ids"] == fixed["ids"]).all() and (late["niter"] == fixed["niter"]).all()
     five = gpu_index.search(q, K, L, M, stop="learned", stop_threshold=0.0, stop_min_iter=5)
     assert ((five["niter"] == 5) | ((five["niter"] < 5) & (five["niter"] == fixed["niter"]))).all()
+    # both restrictions: the first evaluation is the first multiple of
+    # stop_every at or after max(1, stop_min_iter)
+    for mi, ev, first in ((5, 2, 6), (6, 3, 6), (0, 4, 4), (7, 1, 7)):
+        both = gpu_index.search(q, K, L, M, stop="learned", stop_threshold=0.0, stop_min_iter=mi, stop_every=ev)
+        assert ((both["niter"] == first) | ((both["niter"] < first) & (both["niter"] == fixed["niter"]))).all(), \
+            (mi, ev)
     every = gpu_index.search(q, K, L, M, stop="learned", stop_threshold=0.4, stop_every=3)
     stopped_early = every["niter"] < fixed["niter"]
     assert stopped_early.any()
