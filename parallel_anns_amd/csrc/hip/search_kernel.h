@@ -512,15 +512,17 @@ __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32
 // bitmap for what the table cannot hold.
 //
 // The table has nb buckets of eight 16-bit slots (one 16-byte LDS read per
-// lookup).  An id is scrambled by a bijection of [0, 2^idbits)
+// bucket), or four where the id space needs more buckets.  An id is scrambled by a bijection of [0, 2^idbits)
 // (multiplication by an odd constant, then an xor-shift); the remainder of
 // the result by nb picks the bucket and the quotient (< 2^15) is the tag
 // stored in it (0xFFFF marks an empty slot), so bucket and tag together
-// identify the id exactly.  An id whose bucket is full tries one second
-// bucket (probe()).  Slots are only ever filled during a query, so once both
-// buckets of an id are full, an id absent from them is decided by the
-// bitmap, and an id is never both in the table and in the bitmap.  spills is
-// a wave-uniform count kept in registers.
+// identify the id exactly.  An id whose bucket is full goes to one second
+// bucket (probe(); with 8-slot buckets both are read at once), then to the
+// LDS overflow area of full ids when there is one.  Slots are only ever
+// filled during a query, so once both buckets of an id are full (and its
+// overflow entries), an id absent from them is decided by the bitmap, and
+// an id is never both in the table and in the bitmap.  spills is a
+// wave-uniform count kept in registers.
 struct Visited {
     uint32_t *tbl;      // LDS: buckets x (1 << bshift) words
     uint32_t nb, magic, idbits, idmask, xshift, use;  // buckets, floor(2^32 / nb)
