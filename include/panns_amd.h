@@ -274,8 +274,7 @@ int panns_exact_knn_ex(panns_index *h, int source, uint32_t k, int method, uint3
  * panns_search_trace: the fixed-L search of panns_search, also recording
  * after every iteration s < trace_cap the 8 stop features
  *   [iteration, top-1 dist, K-th dist, K-th/top-1, new keys that beat the
- *    queue's last key in the iteration, iterations since the top-K set last
- *    changed, first unchecked queue position, unchecked entries]
+ *    queue's last key in the iteration, best initial dist, query norm, unchecked entries]
  * into feat[(q*trace_cap + s)*8 ...] and an order-free hash of the top-K id
  * set into topk_hash[q*trace_cap + s]; niter[q] receives the iteration count.
  * A step is labelled "final" when its hash equals the last step's.

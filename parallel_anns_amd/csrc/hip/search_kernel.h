@@ -991,7 +991,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
 
         uint32_t S = 0, ndist = 0, nexp = 0, niter = 0, nadj = 0;
         uint32_t ins_last = 0;
-        uint32_t since_topk = 0;  // iterations since the top-K set last changed
+        float ep_dist = 0.f;
         // the first iteration >= max(1, stop_min_iter) that is a multiple of
         // stop_every (>= 1); later ones follow by addition
         uint32_t next_eval = 0;
@@ -1030,6 +1030,16 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
             for (uint32_t i = lane; i < L; i += 64) Q[i] = iq[i];
             S = L;
             __syncthreads();
+        }
+        ep_dist = kdist<kNorm>(Q[0]);
+        float qnorm = 0.f;
+        if constexpr (kLearned || kTrace) {
+            if (lane < D / 4) {
+                const float4 q = qs[lane];
+                qnorm = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
+            }
+            qnorm = wave_sum(qnorm, xs);
+            qnorm = sqrtf(qnorm);
         }
         PANNS_PT(kPhInit);
 
@@ -1134,8 +1144,6 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                 PANNS_PT(kPhMerge);
             }
             k = (nk <= last_k) ? nk : last_k + 1u;
-            // a key inserted above position K changed the top-K set
-            since_topk = (nk < K) ? 0u : since_topk + 1u;
             ins_last = ins;
             if (a.mode == kModeScaleM && Mcur < a.M) Mcur <<= 1;
 
@@ -1143,10 +1151,8 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
             if (eval_stop) next_eval += a.stop_every;
             if (eval_stop || kTrace) {
                 // features: step, top-1 dist, K-th dist, ratio, new keys below
-                // the queue's last key in the last step, iterations since the
-                // top-K set last changed, first unchecked position, unchecked
-                // count (the last two are the quantities of the reference's
-                // num_unchecked and last_update_step stopping studies)
+                // the queue's last key in the last step, best initial (entry)
+                // dist, query norm, unchecked count
                 uint32_t unc = 0;
                 for (uint32_t j = lane; j < S; j += 64) unc += (Q[j] & kChk) ? 0u : 1u;
                 unc = wave_sum(unc, xs);
@@ -1158,8 +1164,8 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                 f[2] = dk;
                 f[3] = dk / fmaxf(d1, 1e-12f);
                 f[4] = (float)ins_last;
-                f[5] = (float)since_topk;
-                f[6] = (float)k;
+                f[5] = ep_dist;
+                f[6] = qnorm;
                 f[7] = (float)unc;
                 if (kTrace && niter <= a.trace_cap) {
                     float *tf = a.trace_feat + ((size_t)w * a.trace_cap + (niter - 1)) * kFeat;

@@ -23,7 +23,7 @@ import numpy as np
 from . import tools
 
 FEATURES = ("iteration", "top1_dist", "kth_dist", "kth_over_top1", "inserted_last_step",
-            "iterations_since_topk_changed", "first_unchecked", "unchecked")
+            "best_initial_dist", "query_norm", "unchecked")
 MODEL_FLOATS = 4820
 
 
