@@ -228,6 +228,9 @@ def main():
                     help="L_max candidates for the learned stop, as multiples of the fixed-L operating L")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--curve-L", default="100,150,200,300,400",
+                    help="L values of the fixed-L QPS-vs-recall curve (empty: none)")
+    ap.add_argument("--curve-M", default="1,4,16,64", help="M values of the curve")
     args = ap.parse_args()
 
     # host CPUs for the CPU baseline, counted before any OpenMP region runs
@@ -268,6 +271,21 @@ def main():
             fixed_other[t] = timed(index, qids, K, o[1], o[0], gt, 3, 1)[2]
             L_at[t] = o[2].get(fM, o[1])
             log(f"fixed-L at {t}: {fixed_other[t]}")
+
+    # ---- fixed-L QPS-vs-recall curve over the configured L and M values
+    # (plain plans, each timed over 3 launches after 1 warm-up)
+    curve = []
+    for Lc in [int(x) for x in args.curve_L.split(",") if x.strip()]:
+        for Mc in [int(x) for x in args.curve_M.split(",") if x.strip()]:
+            plan = index.plan(qids, K, Lc, Mc, want_queue=False)
+            wall, kms = time_plan(plan, 3, 1)
+            r = plan.fetch()
+            plan.close()
+            curve.append({"L": Lc, "M": Mc, "qps": round(3 * nq / wall, 1),
+                          "kernel_ms": round(float(np.mean(kms)), 4),
+                          "recall": round(tools.recall(gt, r["ids"], Lc), 4),
+                          "ndist": round(float(r["ndist"].mean()), 1)})
+            log(f"curve: {curve[-1]}")
 
     # ---- per-query bound on an L grid around the operating point
     grid = sorted({max(K, int(fL * f)) for f in (0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.15, 1.3, 1.6, 2.0)})
@@ -431,6 +449,7 @@ def main():
                    "fixed_L": fixed, "learned": lrn,
                    "learned_over_fixed": round(lrn["qps"] / fixed["qps"], 3) if lrn else None,
                    "per_query_stop_bound": oracle,
+                   "fixed_L_curve": curve or None,
                    "host_inclusive": host_inclusive,
                    "other_targets": lrn_other or ({str(t): {"fixed_L": v} for t, v in fixed_other.items()}
                                                   if fixed_other else None),
