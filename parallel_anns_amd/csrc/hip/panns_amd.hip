@@ -112,6 +112,13 @@ struct panns_index {
     uint32_t distance = PANNS_DISTANCE_L2;
     DevBuf<float> norms;
     bool norms_valid = false;
+    // device buffers of the last panns_search call, lent to the next one so
+    // that a stream of calls of the same or smaller sizes allocates nothing
+    struct OneShot {
+        DevBuf<uint32_t> qids, init_ids, out_ids, out_stats, out_qids;
+        DevBuf<float> out_dists, out_qdists;
+        DevBuf<uint8_t> out_qchk;
+    } oneshot;
 };
 
 struct panns_plan {
@@ -898,6 +905,35 @@ int upload_graph(panns_index *h, uint32_t width, uint32_t ep, uint32_t n, uint32
 
 }  // namespace
 
+namespace {
+template <typename T>
+void swap_buf(DevBuf<T> &a, DevBuf<T> &b)
+{
+    std::swap(a.p, b.p);
+    std::swap(a.n, b.n);
+}
+void swap_oneshot(panns_plan *pl, panns_index::OneShot &o)
+{
+    swap_buf(pl->qids, o.qids);
+    swap_buf(pl->init_ids, o.init_ids);
+    swap_buf(pl->out_ids, o.out_ids);
+    swap_buf(pl->out_stats, o.out_stats);
+    swap_buf(pl->out_qids, o.out_qids);
+    swap_buf(pl->out_dists, o.out_dists);
+    swap_buf(pl->out_qdists, o.out_qdists);
+    swap_buf(pl->out_qchk, o.out_qchk);
+}
+// Lends the handle's one-shot buffers to a plan and takes them back when the
+// call ends, on every return path.  Work left on the handle's stream by a
+// failed call stays ordered before the next call's use of the buffers.
+struct OneShotLease {
+    panns_plan *pl;
+    panns_index *h;
+    OneShotLease(panns_plan *p, panns_index *x) : pl(p), h(x) { swap_oneshot(pl, h->oneshot); }
+    ~OneShotLease() { swap_oneshot(pl, h->oneshot); }
+};
+}  // namespace
+
 extern "C" {
 
 const char *panns_last_error(void) { return g_err.c_str(); }
@@ -1428,6 +1464,7 @@ int panns_search(panns_index *h, const panns_params *p, const uint32_t *query_id
     HIP_TRY(hipSetDevice(h->device));
     std::unique_ptr<panns_plan> own(new panns_plan());
     pl = own.get();
+    OneShotLease lease(pl, h);
     pl->h = h;
     pl->p = *p;
     pl->nq = nq;
