@@ -334,3 +334,22 @@ def test_error_paths(small_set, gpu_index):
     with pytest.raises(SearchError):
         idx.set_base(np.zeros((10, 12), np.float32))  # D not a multiple of 8
     idx.close()
+
+
+def test_one_shot_buffers_across_sizes(small_set, gpu_index):
+    """panns_search lends its device buffers from the handle to each call:
+    calls that grow, shrink, change L and turn the queue outputs on and off
+    return what a resident plan (its own buffers) returns for the same ids."""
+    ds = small_set
+    q = np.arange(ds["queries"].shape[0], dtype=np.uint32)
+    for n, L, wq in ((3, 160, False), (200, 160, True), (50, 120, True), (1, 200, False), (200, 100, False)):
+        r = gpu_index.search(q[:n], 100, L, 4, want_queue=wq)
+        pl = gpu_index.plan(q[:n], 100, L, 4, want_queue=wq)
+        pl.run()
+        e = pl.fetch(want_queue=wq)
+        pl.close()
+        for f in ("ids", "dists", "ndist", "nexp", "niter"):
+            assert (r[f] == e[f]).all(), (n, L, wq, f)
+        if wq:
+            for f in ("queue_ids", "queue_dists", "queue_checked"):
+                assert (r[f] == e[f]).all(), (n, L, wq, f)
