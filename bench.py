@@ -346,12 +346,14 @@ def main():
                 # candidates are compared by validation device time (the
                 # model's own cost included); the model runs from the
                 # iteration by which 1 % of training queries had their final
-                # top-K, at every iteration or every second one
+                # top-K, at every fourth, every second or every iteration --
+                # tried in that order, so a more frequent evaluation must pay
+                # for itself clearly (the 2 % rule below) to be chosen
                 _, _, ms_fixed = learned.recall_work(index, vq, vgt, K, Lf, lM, None, runs=3)
                 best_l = (ms_fixed, Lf, 2.0, {})
                 for f in (float(x) for x in args.learned_L.split(",")):
                     lL = min(4096, int(round(Lf * f)))
-                    for every in (1, 2):
+                    for every in (4, 2, 1):
                         kw = dict(stop_min_iter=min_iter, stop_every=every)
                         # a margin of about two standard errors of the
                         # validation recall, so the test recall meets t
