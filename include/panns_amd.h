@@ -174,7 +174,12 @@ int panns_subsearch_with_top_m(panns_index *h, uint32_t M_max, uint32_t query_id
 /* Resident batches: inputs uploaded once, outputs kept in HBM until fetched,
  * so repeated runs time only the device work.  L_per_query / K_per_query may
  * be NULL (uniform p->L / p->K); otherwise each entry overrides them for its
- * query (uneven batches) and init_ids must hold max L entries. */
+ * query (uneven batches) and init_ids must hold max L entries.
+ * A plan reads the handle's base rows, queries and graph where they were
+ * when it was built: after panns_set_base, panns_set_queries, a graph load
+ * or their file forms, panns_plan_run returns PANNS_ERR_STATE for plans
+ * built before (create them again); fetching results of an earlier run
+ * still works. */
 int panns_plan_create(panns_index *h, const panns_params *p, const uint32_t *query_ids,
                       uint32_t nq, const uint32_t *L_per_query, const uint32_t *K_per_query,
                       const uint32_t *init_ids, uint32_t n_init, panns_plan **out);

@@ -339,6 +339,244 @@ void po_search_pss_v5_t1(const po_graph *g, const float *base, uint32_t dim, con
     }
 }
 
+/* PSS5LG.cpp:630-643 insert_one_element_at: shift [insert_index, size-1)
+ * right by one (the last element falls off) and place cand. */
+static void insert_one_element_at(po_cand cand, po_cand *queue, uint32_t insert_index, uint32_t queue_start,
+                                  uint32_t queue_size)
+{
+    const uint32_t dest = queue_start + insert_index;
+    memmove(queue + dest + 1, queue + dest, (size_t)(queue_size - insert_index - 1) * sizeof(po_cand));
+    queue[dest] = cand;
+}
+
+/* PSS5LG.cpp:611-628 add_into_queue_at: insert at insert_index, growing the
+ * queue unless it is at queue_length (then the last element falls off). */
+static void add_into_queue_at(po_cand cand, po_cand *queue, uint32_t insert_index, uint32_t queue_start,
+                              uint32_t *queue_size, uint32_t queue_length)
+{
+    const uint32_t dest = queue_start + insert_index;
+    if (*queue_size == queue_length) --*queue_size;
+    memmove(queue + dest + 1, queue + dest, (size_t)(*queue_size - insert_index) * sizeof(po_cand));
+    queue[dest] = cand;
+    ++*queue_size;
+}
+
+/* PSS5LG.cpp:648-719 merge_two_queues_into_1st_queue_seq_fixed. */
+uint32_t po_merge_two_queues_fixed(po_cand *queue1, uint32_t queue1_start, uint32_t queue1_size,
+                                   po_cand *queue2, uint32_t queue2_start, uint32_t queue2_size)
+{
+    const uint32_t it_loc = lower_bound_cand(queue1, queue1_start, queue1_start + queue1_size,
+                                             &queue2[queue2_start]);
+    const uint32_t insert_index = it_loc - queue1_start;
+    if (insert_index == queue1_size) return insert_index;
+    if (insert_index == queue1_size - 1) {
+        queue1[queue1_start + insert_index] = queue2[queue2_start];
+        return insert_index;
+    }
+    /* the first element of queue2 */
+    if (queue2[queue2_start].id != queue1[it_loc].id) {
+        insert_one_element_at(queue2[queue2_start], queue1, insert_index, queue1_start, queue1_size);
+    } else if (!queue2[queue2_start].checked && queue1[it_loc].checked) {
+        queue1[it_loc].checked = 0;
+    }
+    if (queue2_size == 1) return insert_index;
+    uint32_t q_i_1 = insert_index + 1 + queue1_start;
+    uint32_t q_i_2 = queue2_start + 1;
+    const uint32_t q_i_1_bound = queue1_start + queue1_size;
+    const uint32_t q_i_2_bound = queue2_start + queue2_size;
+    for (uint32_t insert_i = insert_index + 1; insert_i < queue1_size; ++insert_i) {
+        if (q_i_1 >= q_i_1_bound || q_i_2 >= q_i_2_bound) {
+            break;
+        } else if (po_cand_less(&queue1[q_i_1], &queue2[q_i_2])) {
+            ++q_i_1;
+        } else if (po_cand_less(&queue2[q_i_2], &queue1[q_i_1])) {
+            insert_one_element_at(queue2[q_i_2++], queue1, insert_i, queue1_start, queue1_size);
+            ++q_i_1;
+        } else {
+            /* duplicate: the first queue keeps its copy, unchecked if the
+             * second queue's copy is */
+            if (!queue2[q_i_2].checked && queue1[q_i_1].checked) queue1[q_i_1].checked = 0;
+            ++q_i_2;
+            ++q_i_1;
+        }
+    }
+    return insert_index;
+}
+
+/* PSS5LG.cpp:725-805 merge_two_queues_into_1st_queue_seq_incr (the merge of
+ * the reference's tree-shaped merge_all_queues_to_master, kept at
+ * PSS5LG.cpp:844-909 as a backup; the live merge uses the fixed form). */
+uint32_t po_merge_two_queues_incr(po_cand *queue1, uint32_t queue1_start, uint32_t *queue1_size,
+                                  uint32_t queue1_length, po_cand *queue2, uint32_t queue2_start,
+                                  uint32_t queue2_size)
+{
+    const uint32_t it_loc = lower_bound_cand(queue1, queue1_start, queue1_start + *queue1_size,
+                                             &queue2[queue2_start]);
+    const uint32_t insert_index = it_loc - queue1_start;
+    if (insert_index == *queue1_size) {
+        const uint32_t copy_count = (*queue1_size + queue2_size > queue1_length) ? queue1_length - *queue1_size
+                                                                                 : queue2_size;
+        memmove(queue1 + queue1_start + *queue1_size, queue2 + queue2_start, (size_t)copy_count * sizeof(po_cand));
+        *queue1_size += copy_count;
+        return insert_index;
+    }
+    if (queue2[queue2_start].id != queue1[it_loc].id) {
+        add_into_queue_at(queue2[queue2_start], queue1, insert_index, queue1_start, queue1_size, queue1_length);
+    } else if (!queue2[queue2_start].checked && queue1[it_loc].checked) {
+        queue1[it_loc].checked = 0;
+    }
+    if (queue2_size == 1) return insert_index;
+    uint32_t q_i_1 = insert_index + 1 + queue1_start;
+    uint32_t q_i_2 = queue2_start + 1;
+    uint32_t q_i_1_bound = queue1_start + *queue1_size;
+    const uint32_t q_i_2_bound = queue2_start + queue2_size;
+    for (uint32_t insert_i = insert_index + 1; insert_i < queue1_length; ++insert_i) {
+        if (q_i_1 >= q_i_1_bound) {
+            const uint32_t a = queue1_length - insert_i, b = q_i_2_bound - q_i_2;
+            *queue1_size += a < b ? a : b;
+            for (; insert_i < *queue1_size; ++insert_i) queue1[queue1_start + insert_i] = queue2[q_i_2++];
+            break;
+        } else if (q_i_2 >= q_i_2_bound) {
+            break;
+        } else if (po_cand_less(&queue1[q_i_1], &queue2[q_i_2])) {
+            ++q_i_1;
+        } else if (po_cand_less(&queue2[q_i_2], &queue1[q_i_1])) {
+            add_into_queue_at(queue2[q_i_2++], queue1, insert_i, queue1_start, queue1_size, queue1_length);
+            ++q_i_1;
+            q_i_1_bound = queue1_start + *queue1_size;
+        } else {
+            if (!queue2[q_i_2].checked && queue1[q_i_1].checked) queue1[q_i_1].checked = 0;
+            ++q_i_2;
+            ++q_i_1;
+        }
+    }
+    return insert_index;
+}
+
+/* PSS5LG.cpp:811-842 merge_all_queues_to_master: worker queues 0..T-2 in
+ * order into the master (the last queue, fixed length L); returns the
+ * lowest insert index (L when nothing was merged); worker sizes reset. */
+static uint32_t merge_all_queues_to_master(po_cand *set_L, const uint32_t *starts, uint32_t *sizes, uint32_t T,
+                                           uint32_t L)
+{
+    uint32_t nk = L;
+    const uint32_t master_start = starts[T - 1];
+    for (uint32_t w = 0; w + 1 < T; ++w) {
+        if (sizes[w] == 0) continue;
+        const uint32_t r = po_merge_two_queues_fixed(set_L, master_start, L, set_L, starts[w], sizes[w]);
+        if (r < nk) nk = r;
+        sizes[w] = 0;
+    }
+    return nk;
+}
+
+/* PSS5LG.cpp:917-947 pick_top_m_to_workers: every unchecked master entry
+ * from k_uc on, dealt round-robin to workers 0..T-2 with every T-th left
+ * to the master; a dealt entry is copied (unchecked) and marked checked in
+ * the master.  Stops when a worker queue reaches local_capacity.  Returns
+ * the number of unchecked entries seen. */
+static uint32_t pick_top_m_to_workers(po_cand *set_L, const uint32_t *starts, uint32_t *sizes, uint32_t T,
+                                      uint32_t local_capacity, uint32_t k_uc)
+{
+    const uint32_t last_start = starts[T - 1];
+    const uint32_t c_bound = last_start + sizes[T - 1];
+    uint32_t count = 0, dest = 0;
+    for (uint32_t c = k_uc + last_start; c < c_bound; ++c) {
+        if (set_L[c].checked) continue;
+        ++count;
+        if (T - 1 != dest) {
+            set_L[starts[dest] + sizes[dest]++] = set_L[c];
+            set_L[c].checked = 1;
+            if (sizes[dest] == local_capacity) break;
+            ++dest;
+        } else {
+            dest = 0;
+        }
+    }
+    return count;
+}
+
+/* PSS5LG.cpp:1376-1724 with T = num_threads_ queues.  The reference runs
+ * the T expansion loops (PSS5LG.cpp:1584-1646) as OpenMP threads that race
+ * on the shared visited bitset and on the master's L-th distance; here they
+ * run in lockstep in a fixed order, which is one schedule the reference can
+ * take: step s = 1..X expands the s-th candidate of worker 0, then worker
+ * 1, ..., then the master (queue T-1), each to completion, so a neighbour
+ * reached by two queues in one step goes to the lower-numbered one, and every
+ * worker's bound is the master's L-th distance as the previous step left it.
+ * set_L holds (T-1)*local_L + L entries: worker w's queue at w*local_L, the
+ * master at (T-1)*local_L (APP_PSS:136-141 uses local_L = L).  At T = 1 this
+ * is po_search_pss_v5_t1. */
+void po_search_pss_v5(const po_graph *g, const float *base, uint32_t dim, const float *query, uint32_t K,
+                      uint32_t L, uint32_t X, uint32_t T, uint32_t local_L, const uint32_t *init_ids, po_cand *set_L,
+                      uint32_t *set_K, uint64_t *visited, po_counters *cnt)
+{
+    uint32_t *starts = (uint32_t *)malloc(sizeof(uint32_t) * T);
+    uint32_t *sizes = (uint32_t *)calloc(T, sizeof(uint32_t));
+    uint32_t *kuc = (uint32_t *)malloc(sizeof(uint32_t) * T);
+    for (uint32_t w = 0; w < T; ++w) starts[w] = w * local_L;
+    const uint32_t ms = starts[T - 1];
+    uint64_t ndist = 0, merges = 0, exps = 0;
+    po_initialize_set_L(base, dim, query, L, set_L, ms, &sizes[T - 1], init_ids, visited, &ndist, g->norms);
+    /* the live bound: the master's last entry (PSS5LG.cpp:1412) */
+    const float *last_dist = &set_L[ms + sizes[T - 1] - 1].dist;
+    uint32_t k_master = 0;
+    int no_need = 0;
+    /* sequential start: one master step (PSS5LG.cpp:1478-1534) */
+    if (k_master == L) {
+        no_need = 1;
+    } else {
+        uint32_t r;
+        po_cand *c = &set_L[ms + k_master];
+        if (!c->checked) {
+            c->checked = 1;
+            r = expand_one_candidate(g, base, dim, c->id, query, last_dist, set_L, ms, &sizes[T - 1], L, visited,
+                                     &ndist);
+            ++exps;
+        } else {
+            r = L;
+        }
+        k_master = (r <= k_master) ? r : k_master + 1;
+    }
+    while (!no_need) {
+        if (!pick_top_m_to_workers(set_L, starts, sizes, T, local_L, k_master)) break;
+        /* the expansion loops (PSS5LG.cpp:1584-1646), in lockstep */
+        for (uint32_t w = 0; w < T; ++w) kuc[w] = (w == T - 1) ? k_master : 0;
+        for (uint32_t step = 0; step < X; ++step) {
+            int any = 0;
+            for (uint32_t w = 0; w < T; ++w) {
+                const uint32_t st = starts[w];
+                const uint32_t cap = (w == T - 1) ? L : local_L;
+                while (kuc[w] < sizes[w] && set_L[st + kuc[w]].checked) ++kuc[w];
+                if (kuc[w] >= sizes[w]) continue;
+                set_L[st + kuc[w]].checked = 1;
+                const uint32_t id = set_L[st + kuc[w]].id;
+                const uint32_t r = expand_one_candidate(g, base, dim, id, query, last_dist, set_L, st, &sizes[w],
+                                                        cap, visited, &ndist);
+                ++exps;
+                kuc[w] = (r <= kuc[w]) ? r : kuc[w] + 1;
+                any = 1;
+            }
+            if (!any) break;
+        }
+        k_master = kuc[T - 1];
+        /* PSS5LG.cpp:1655-1667 */
+        ++merges;
+        const uint32_t r = merge_all_queues_to_master(set_L, starts, sizes, T, L);
+        if (r <= k_master) k_master = r;
+    }
+    for (uint32_t i = 0; i < K; ++i) set_K[i] = set_L[ms + i].id;
+    memset(visited, 0, sizeof(uint64_t) * ((g->num_v + 63) / 64));
+    free(starts);
+    free(sizes);
+    free(kuc);
+    if (cnt) {
+        cnt->distance_computations += ndist;
+        cnt->merges += merges;
+        cnt->expansions += exps;
+    }
+}
+
 /* TOPMLG.cpp:1077-1131: the bound passed to expand_one_candidate is the
  * element at index set_L_size-1 at the time of each call (a live reference). */
 static void subsearch_one_iteration(const po_graph *g, const float *base, uint32_t dim,

@@ -32,6 +32,15 @@
  *                          restatement in numpy; invariants
  *                          (#dist = L + |visited \ init|, final queue all checked).
  *   po_search_pss_v5_t1    equals po_search_top_m with M = 1 for every X.
+ *   po_merge_two_queues_fixed / _incr   hand-made queues (duplicates with
+ *                          every checked/unchecked pair, the last-index and
+ *                          past-the-end cases) against a set-merge model.
+ *   po_search_pss_v5       T = 1 equals po_search_pss_v5_t1; for T > 1 an
+ *                          independent list-based restatement of the lockstep
+ *                          schedule in Python (ids, distances, checked
+ *                          flags, counts); invariants (final queue all
+ *                          checked, #dist = L + |visited \ init|); recall
+ *                          parity with the top-M search.
  *   po_subsearch_with_top_m   scale-M equals the same set semantics with the
  *                          doubling M schedule.
  *   po_brute_force_knn     tiny inputs against numpy float64 ranking with ties
@@ -117,6 +126,33 @@ void po_search_top_m(const po_graph *g, const float *base, uint32_t dim, const f
 void po_search_pss_v5_t1(const po_graph *g, const float *base, uint32_t dim, const float *query,
                          uint32_t K, uint32_t L, uint32_t X, const uint32_t *init_ids,
                          po_cand *set_L, uint32_t *set_K, uint64_t *visited, po_counters *cnt);
+
+/* PSS5LG.cpp:648-719 merge_two_queues_into_1st_queue_seq_fixed: merges the
+ * sorted queue2 into queue1 (queue1_size entries, size kept).  A duplicate
+ * id keeps queue1's copy, unchecked when queue2's copy is; an element that
+ * lands on queue1's last index replaces it.  Returns the lower-bound index
+ * of queue2's first element in queue1 (queue1_size: nothing merged). */
+uint32_t po_merge_two_queues_fixed(po_cand *queue1, uint32_t queue1_start, uint32_t queue1_size,
+                                   po_cand *queue2, uint32_t queue2_start, uint32_t queue2_size);
+
+/* PSS5LG.cpp:725-805 merge_two_queues_into_1st_queue_seq_incr: the same
+ * merge into a queue that may grow up to queue1_length entries. */
+uint32_t po_merge_two_queues_incr(po_cand *queue1, uint32_t queue1_start, uint32_t *queue1_size,
+                                  uint32_t queue1_length, po_cand *queue2, uint32_t queue2_start,
+                                  uint32_t queue2_size);
+
+/* PSS5LG.cpp:1376-1724 para_search_PSS_v5_large_graph_dist_thresh_profiling
+ * with num_threads_ = T: pick_top_m_to_workers (PSS5LG.cpp:917-947), up to
+ * X expansions per queue (PSS5LG.cpp:1584-1646) and
+ * merge_all_queues_to_master (PSS5LG.cpp:811-842).  The reference's threads
+ * race; this runs them in lockstep in a fixed order (step s expands the
+ * s-th candidate of worker 0, 1, ..., T-2, then of the master), one of the
+ * schedules the reference can take, so results are reproducible.  set_L
+ * holds (T-1)*local_L + L entries, the master last; set_K gets its first K
+ * ids.  T = 1 is po_search_pss_v5_t1. */
+void po_search_pss_v5(const po_graph *g, const float *base, uint32_t dim, const float *query, uint32_t K,
+                      uint32_t L, uint32_t X, uint32_t T, uint32_t local_L, const uint32_t *init_ids, po_cand *set_L,
+                      uint32_t *set_K, uint64_t *visited, po_counters *cnt);
 
 /* TOPMLG.cpp:1023-1071 subsearch_with_top_m + :1077-1131
  * subsearch_top_m_for_one_iteration.  Works in place on set_L[set_L_start ..

@@ -47,6 +47,10 @@ _API = {
                                        ctypes.POINTER(u32), vp, ctypes.POINTER(u64)]),
     "po_initialize_set_L": (None, [vp, u32, vp, u32, vp, u32, ctypes.POINTER(u32), vp, vp,
                                    ctypes.POINTER(u64), vp]),
+    "po_merge_two_queues_fixed": (u32, [vp, u32, u32, vp, u32, u32]),
+    "po_merge_two_queues_incr": (u32, [vp, u32, ctypes.POINTER(u32), u32, vp, u32, u32]),
+    "po_search_pss_v5": (None, [ctypes.POINTER(Graph), vp, u32, vp, u32, u32, u32, u32, u32, vp, vp, vp, vp,
+                                ctypes.POINTER(Counters)]),
     "po_search_batch": (u64, [ctypes.POINTER(Graph), vp, u32, vp, u32, u32, u32, u32, cint, vp, vp,
                               vp, vp, cint]),
     "po_brute_force_knn": (None, [vp, u32, u32, vp, u32, u32, vp, vp, cint]),
@@ -147,6 +151,37 @@ def search_one(graph, base, query, K, L, M, init_ids, mode="top_m", X=1):
       _p(init_ids), _p(set_L), _p(set_K), _p(visited), ctypes.byref(c))
     return {"ids": set_K, "queue": set_L, "ndist": c.distance_computations, "merges": c.merges,
             "nexp": c.expansions, "visited_clear": not visited.any()}
+
+
+def search_pss_v5(graph, base, query, K, L, X, T, init_ids, local_L=None):
+    """PSS_v5 with T queues in the oracle's lockstep order; returns the
+    master queue and counters as search_one does, plus the whole set_L."""
+    base = np.ascontiguousarray(base, np.float32)
+    query = np.ascontiguousarray(query, np.float32)
+    init_ids = np.ascontiguousarray(init_ids, np.uint32)
+    local_L = L if local_L is None else local_L
+    set_L = np.zeros((T - 1) * local_L + L, CAND_DT)
+    set_K = np.zeros(K, np.uint32)
+    visited = np.zeros((graph.n + 63) // 64, np.uint64)
+    c = Counters()
+    lib().po_search_pss_v5(ctypes.byref(graph.g), _p(base), base.shape[1], _p(query), K, L, X, T, local_L,
+                           _p(init_ids), _p(set_L), _p(set_K), _p(visited), ctypes.byref(c))
+    ms = (T - 1) * local_L
+    return {"ids": set_K, "queue": set_L[ms:ms + L].copy(), "ndist": c.distance_computations, "merges": c.merges,
+            "nexp": c.expansions, "visited_clear": not visited.any()}
+
+
+def merge_two_queues_fixed(q1, q2):
+    """q1, q2: CAND_DT arrays (q1 in place, its length fixed); returns the index."""
+    return int(lib().po_merge_two_queues_fixed(_p(q1), 0, q1.shape[0], _p(q2), 0, q2.shape[0]))
+
+
+def merge_two_queues_incr(q1, size1, q2):
+    """q1: CAND_DT array of capacity len(q1) holding size1 entries (in place);
+    returns (index, new size)."""
+    s = u32(size1)
+    r = lib().po_merge_two_queues_incr(_p(q1), 0, ctypes.byref(s), q1.shape[0], _p(q2), 0, q2.shape[0])
+    return int(r), int(s.value)
 
 
 def initialize_set_L(base, query, L, init_ids, n, norms=None):

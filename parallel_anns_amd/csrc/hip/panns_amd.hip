@@ -98,6 +98,13 @@ struct panns_index {
     DevBuf<uint32_t> bitmaps, spill_log;
     uint32_t bitmap_slots = 0;
     uint64_t bitmap_words = 0;
+    // bumped whenever the scratch buffers are reallocated (a captured graph
+    // holds their addresses and must be captured again)
+    uint64_t scratch_gen = 0;
+    // bumped whenever base rows, queries, graph or norms are replaced: a
+    // plan holds their device addresses and its query ids were checked
+    // against the query count of its build
+    uint64_t input_gen = 0;
     DevBuf<uint32_t> counter;
     // learned stop model (device copy), 8-64-64-1
     DevBuf<float> model;
@@ -159,7 +166,8 @@ struct panns_plan {
     hipEvent_t fork_ev = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    const void *graph_scratch = nullptr;  // bitmap buffer the graph was captured with
+    uint64_t graph_gen = 0;  // the handle's scratch_gen when the graph was captured
+    uint64_t input_gen = 0;  // the handle's input_gen when the plan was built
 
     ~panns_plan()
     {
@@ -355,10 +363,16 @@ uint32_t lds_bytes_mw(uint32_t Lcap, uint32_t nb_cap, uint32_t tbl_slots, uint32
            4u * panns::kCtrlWords;
 }
 
+// One N-bit bitmap per persistent workgroup, padded to at least 64 words:
+// in the claim's bitmap path every lane of the wave issues an atomic, and a
+// lane with no id ORs 0 into word `lane` of its own workgroup's bitmap.
+uint64_t bitmap_words_for(uint32_t n) { return std::max<uint64_t>(((uint64_t)n + 31) / 32, 64); }
+
 int ensure_scratch(panns_index *h, uint32_t grid)
 {
-    const uint64_t words = ((uint64_t)h->n + 31) / 32;
+    const uint64_t words = bitmap_words_for(h->n);
     if (h->bitmap_slots < grid || h->bitmap_words != words) {
+        ++h->scratch_gen;
         h->bitmaps.release();
         h->spill_log.release();
         HIP_TRY(h->bitmaps.ensure((size_t)grid * words));
@@ -657,6 +671,7 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     a.trace_topk = pl->trace_cap ? pl->trace_topk.p : nullptr;
     a.trace_cap = pl->trace_cap;
     a.phase_cycles = pl->phase_cycles.p;
+    pl->input_gen = h->input_gen;
     return PANNS_OK;
 }
 
@@ -709,6 +724,9 @@ int launch_parts(panns_plan *pl)
 int run_plan(panns_plan *pl, int sync, float *ms)
 {
     panns_index *h = pl->h;
+    if (pl->input_gen != h->input_gen)
+        return fail(PANNS_ERR_STATE, "the plan was built before the base set, queries, graph or distance changed; "
+                                     "create it again");
     if (pl->parts.empty()) {
         int rc = bind_scratch(pl);
         if (rc) return rc;
@@ -723,7 +741,7 @@ int run_plan(panns_plan *pl, int sync, float *ms)
             int rc = bind_scratch(part.get());
             if (rc) return rc;
         }
-        if (pl->opt.use_graph && (!pl->gexec || pl->graph_scratch != (const void *)h->bitmaps.p)) {
+        if (pl->opt.use_graph && (!pl->gexec || pl->graph_gen != h->scratch_gen)) {
             if (pl->gexec) HIP_TRY(hipGraphExecDestroy(pl->gexec));
             if (pl->graph) HIP_TRY(hipGraphDestroy(pl->graph));
             pl->gexec = nullptr;
@@ -739,7 +757,7 @@ int run_plan(panns_plan *pl, int sync, float *ms)
             HIP_TRY(e);
             pl->graph = g;
             HIP_TRY(hipGraphInstantiate(&pl->gexec, pl->graph, nullptr, nullptr, 0));
-            pl->graph_scratch = h->bitmaps.p;
+            pl->graph_gen = h->scratch_gen;
         }
         HIP_TRY(hipEventRecord(h->ev0, h->stream));
         if (pl->opt.use_graph) {
@@ -891,6 +909,7 @@ int upload_graph(panns_index *h, uint32_t width, uint32_t ep, uint32_t n, uint32
     std::vector<uint32_t> rows((size_t)n * stride, panns::kSent);
     for (uint32_t v = 0; v < n; ++v)
         std::memcpy(&rows[(size_t)v * stride], adj + (size_t)v * stride_in, deg[v] * 4ull);
+    ++h->input_gen;
     HIP_TRY(h->adj.ensure(rows.size()));
     HIP_TRY(hipMemcpy(h->adj.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
     h->width = width;
@@ -981,6 +1000,7 @@ int panns_set_base(panns_index *h, const float *data, uint32_t n, uint32_t dim)
     HIP_TRY(h->base.ensure((size_t)n * dim));
     HIP_TRY(hipMemcpy(h->base.p, data, (size_t)n * dim * 4, hipMemcpyHostToDevice));
     if (h->n != n) h->has_graph = false;
+    ++h->input_gen;
     h->n = n;
     h->dim = dim;
     h->has_pivots = false;
@@ -998,6 +1018,7 @@ int panns_set_queries(panns_index *h, const float *data, uint32_t nq, uint32_t d
     HIP_TRY(hipSetDevice(h->device));
     HIP_TRY(h->queries.ensure((size_t)nq * dim));
     HIP_TRY(hipMemcpy(h->queries.p, data, (size_t)nq * dim * 4, hipMemcpyHostToDevice));
+    ++h->input_gen;
     h->nq = nq;
     h->dim = dim;
     return PANNS_OK;
@@ -1275,6 +1296,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
             pl->side_ev.push_back(e);
         }
     }
+    pl->input_gen = h->input_gen;
     pl->grid = 0;
     for (auto &part : pl->parts) {
         pl->grid += part->grid;
@@ -1516,9 +1538,12 @@ int panns_subsearch_with_top_m(panns_index *h, uint32_t M_max, uint32_t query_id
     }
     HIP_TRY(hipSetDevice(h->device));
     const size_t words32 = ((size_t)h->n + 31) / 32;
+    // padded as the per-workgroup bitmaps are (bitmap_words_for): lanes with
+    // no id OR 0 into word `lane`
+    const size_t bm_words = bitmap_words_for(h->n);
     DevBuf<uint32_t> bm;
-    HIP_TRY(bm.ensure(words32 + 1));
-    HIP_TRY(hipMemset(bm.p, 0, (words32 + 1) * 4));
+    HIP_TRY(bm.ensure(bm_words));
+    HIP_TRY(hipMemset(bm.p, 0, bm_words * 4));
     HIP_TRY(hipMemcpy(bm.p, visited_words, words32 * 4, hipMemcpyHostToDevice));
     std::unique_ptr<panns_plan> pl(new panns_plan());
     pl->h = h;

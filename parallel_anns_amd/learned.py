@@ -97,23 +97,39 @@ def recall_work(index, qids, gt, K, L, M, threshold, runs=1, L_per_query=None, o
     return tools.recall(gt, res["ids"], Lr, R=min(K, gt.shape[1])), float(res["ndist"].mean()), ms
 
 
+# The stop threshold that means "the plain fixed-L search": the kernel's
+# sigmoid output never exceeds 1.0f (it can round to exactly 1.0f, so a
+# threshold of 1.0 may still stop a query), so any threshold above 1 never
+# fires.  Every caller uses this one value for the fallback.
+FIXED_FALLBACK = 2.0
+
+
+def is_fallback(threshold):
+    """True when a calibrated threshold means "run the fixed-L search"."""
+    return threshold is None or threshold > 1.0
+
+
 def calibrate(index, qids, gt, K, L, M, target, lo=1e-4, hi=1.0, steps=14, **stop_kw):
     """Largest-saving threshold whose recall on (qids, gt) reaches target: the
-    recall rises with the threshold, so bisect on it (log scale).  stop_kw
-    also passes L_per_query and options through to recall_work."""
+    recall rises with the threshold, so bisect on it (log scale).  None when
+    no threshold below `hi` reaches the target (only `hi` itself, where the
+    stop is all but the fixed search, or not even that).  stop_kw also
+    passes L_per_query and options through to recall_work."""
     r_hi = recall_work(index, qids, gt, K, L, M, hi, **stop_kw)[0]
     if r_hi < target:
         return None
     a, b = np.log(lo), np.log(hi)
     if recall_work(index, qids, gt, K, L, M, lo, **stop_kw)[0] >= target:
         return lo
+    passed = False
     for _ in range(steps):
         m = 0.5 * (a + b)
         if recall_work(index, qids, gt, K, L, M, float(np.exp(m)), **stop_kw)[0] >= target:
             b = m
+            passed = True
         else:
             a = m
-    return float(np.exp(b))
+    return float(np.exp(b)) if passed else None
 
 
 def oracle_bound(per_query_recall, per_query_ndist, target):

@@ -139,3 +139,32 @@ def test_rows_per_wave_keep_results(small_set, gpu_index, uneven, rows):
         pl.close()
     with pytest.raises(SearchError):
         gpu_index.plan(qids[:4], 100, 200, 4, init_ids=init, options=Index.plan_options(rows_per_wave=48))
+
+
+def test_plans_are_refused_after_inputs_change(small_set, gpu_index, uneven):
+    """A plan holds the device addresses of the queries, base rows and graph
+    of its build: after set_queries (or set_base, or a graph load) running
+    it is an error (PANNS_ERR_STATE, no kernel launched) until it is created
+    again; results of an earlier run can still be fetched.  A graph plan is
+    covered the same way."""
+    qids, Ls, Ks = uneven
+    init = gpu_index.prepare_init_ids(int(Ls.max()))
+    plain = gpu_index.plan(qids[:16], 50, 100, 2, want_queue=False)
+    graph = gpu_index.plan(qids, 100, int(Ls.max()), 2, init_ids=init, L_per_query=Ls, K_per_query=Ks,
+                           options=Index.plan_options(buckets=2, use_graph=True), want_queue=False)
+    plain.run()
+    graph.run()
+    r0, g0 = plain.fetch(), graph.fetch()
+    gpu_index.set_queries(small_set["queries"])
+    for pl in (plain, graph):
+        with pytest.raises(SearchError) as e:
+            pl.run()
+        assert e.value.code == -9
+    _same(plain.fetch(), r0)
+    _same(graph.fetch(), g0)
+    plain.close()
+    graph.close()
+    again = gpu_index.plan(qids[:16], 50, 100, 2, want_queue=False)
+    again.run()
+    _same(again.fetch(), r0)
+    again.close()

@@ -80,6 +80,18 @@ def test_calibrate_bisects_a_monotone_recall(monkeypatch):
     assert learned.calibrate(None, None, None, 100, 100, 4, 1.01) is None
 
 
+def test_calibrate_refuses_a_threshold_of_one(monkeypatch):
+    """The kernel's sigmoid can round to exactly 1.0, so a threshold of 1.0
+    may still stop queries: when only the upper end reaches the target there
+    is no usable threshold, and every caller's fallback is the one sentinel."""
+    def fake(index, qids, gt, K, L, M, threshold, runs=1, L_per_query=None, options=None, **kw):
+        return (0.95 if threshold >= 1.0 else 0.9), 0.0, 0.0
+    monkeypatch.setattr(learned, "recall_work", fake)
+    assert learned.calibrate(None, None, None, 100, 100, 4, 0.93) is None
+    assert learned.is_fallback(None) and learned.is_fallback(learned.FIXED_FALLBACK)
+    assert not learned.is_fallback(1.0) and not learned.is_fallback(0.4)
+
+
 def test_uneven_draws_stay_in_range_and_are_seeded():
     cfg = datasets.CONFIGS["c2"]
     L, K = datasets.uneven_L_K(cfg, 20000)
