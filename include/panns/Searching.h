@@ -23,8 +23,13 @@
  *  - Errors print "Error: ..." to stderr and exit(EXIT_FAILURE) as the
  *    reference does; define PANNS_SEARCHING_THROW before including this
  *    header to get std::runtime_error instead.
- *  - num_threads_ must be 1: the GPU reproduces the reference's
- *    single-thread run, whose ids do not depend on the subsearch count.
+ *  - num_threads_: the top-M searches need 1 (the GPU reproduces the
+ *    reference's single-thread run).  The PSS_v5 searches take 1 to 16:
+ *    the reference's threads race, so its T > 1 ids vary from run to run;
+ *    here the T queues run in lockstep in a fixed order (step s expands the
+ *    s-th candidate of worker 0, 1, ..., then of the master), one of the
+ *    schedules the reference can take, and worker queues hold
+ *    local_queue_capacity = L entries as APP_PSS sets them.
  *  - time_expand_ receives the device time of the search (expansion and
  *    merging are one kernel); time_merge_ and time_seq_ stay 0.
  */
@@ -246,19 +251,27 @@ public:
         for (int a = 0; a < 6; ++a) recalls[ats[a]] = hits[a] / (ats[a] * (double)num_queries_);
     }
 
-    /* PSS5LG.cpp:1376-1724 at num_threads_ = 1 (the reference's benchmark
-     * search).  local_queue_capacity and local_queues_* are accepted for
-     * signature compatibility: at one thread there are no worker queues, and
-     * the ids do not depend on subsearch_iterations. */
+    /* PSS5LG.cpp:1376-1724 with num_threads_ = T queues (1 to 16).  The
+     * master queue is set_L[local_queues_starts[T-1], +L); worker queues
+     * hold local_queue_capacity entries, which must be L when T > 1 (the
+     * reference's drivers set it so, APP_PSS:136-141).  At one thread there
+     * are no worker queues, and the ids do not depend on
+     * subsearch_iterations. */
     void para_search_PSS_v5_large_graph_dist_thresh_profiling(
         const idi query_id, const idi K, const idi L, std::vector<Candidate> &set_L,
         const std::vector<idi> &init_ids, std::vector<idi> &set_K, const idi local_queue_capacity,
         const std::vector<idi> &local_queues_starts, std::vector<idi> &local_queues_sizes,
         dynamic_bitset &is_visited, const idi subsearch_iterations)
     {
-        (void)local_queue_capacity;
-        // M carries the subsearch count X: it sets count_merge_ as the
-        // reference counts merges, ceil((expansions - 1) / X).
+        if (num_threads_ > 1) {
+            if (local_queue_capacity != L)
+                fail("local_queue_capacity is " + std::to_string(local_queue_capacity) + "; with num_threads_ > 1 "
+                     "the GPU search keeps worker queues of L entries (set it to L)");
+            if (local_queues_starts.size() < (size_t)num_threads_ || local_queues_sizes.size() < (size_t)num_threads_)
+                fail("local_queues_starts and local_queues_sizes need num_threads_ entries");
+        }
+        // M carries the subsearch count X (at T = 1 it also sets count_merge_
+        // as the reference counts merges, ceil((expansions - 1) / X))
         panns_params p = params(PANNS_MODE_PSS_V5, K, L, subsearch_iterations);
         one_query(p, query_id, set_L, init_ids, set_K, local_queues_starts, local_queues_sizes, is_visited);
     }
@@ -416,15 +429,17 @@ private:
 
     panns_params params(uint32_t mode, idi K, idi L, idi M) const
     {
-        if (num_threads_ != 1)
+        if (mode == PANNS_MODE_PSS_V5 ? (num_threads_ < 1 || num_threads_ > 16) : num_threads_ != 1)
             fail("num_threads_ is " + std::to_string(num_threads_) +
-                 "; the GPU search reproduces the single-thread run only (set num_threads_ = 1)");
+                 (mode == PANNS_MODE_PSS_V5 ? "; the PSS_v5 search runs with 1 to 16"
+                                            : "; the GPU top-M search reproduces the single-thread run only "
+                                              "(set num_threads_ = 1)"));
         panns_params p{};
         p.K = K;
         p.L = L;
         p.M = M;
         p.mode = mode;
-        p.num_threads = 1;
+        p.num_threads = mode == PANNS_MODE_PSS_V5 ? (uint32_t)num_threads_ : 1u;
         p.stop = PANNS_STOP_FIXED;
         return p;
     }
@@ -452,7 +467,11 @@ private:
         check(panns_search(h_, &p, &query_id, 1, init_ids.data(), (uint32_t)init_ids.size(), &out, &cnt),
               "search");
         for (idi i = 0; i < p.L; ++i) set_L[master + i] = Candidate(qids[i], qd[i], qc[i] != 0);
-        if (local_queues_sizes.size() >= (size_t)num_threads_) local_queues_sizes[num_threads_ - 1] = p.L;
+        // the workers' queues end merged (empty), the master full
+        if (local_queues_sizes.size() >= (size_t)num_threads_) {
+            for (int w = 0; w + 1 < num_threads_; ++w) local_queues_sizes[w] = 0;
+            local_queues_sizes[num_threads_ - 1] = p.L;
+        }
         count_distance_computation_ += cnt.distance_computations;
         if (count_merges) count_merge_ += cnt.merges;
         time_expand_ += cnt.kernel_ms * 1e-3;

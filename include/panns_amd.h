@@ -61,7 +61,9 @@ typedef struct {
     uint32_t L;            /* queue length, L <= N */
     uint32_t M;            /* TOP_M: picks per iteration; SCALE_M: M_max; PSS_V5: X (sub-search steps) */
     uint32_t mode;         /* panns_mode */
-    uint32_t num_threads;  /* PSS_V5: the reference's num_threads_; only 1 is offered */
+    uint32_t num_threads;  /* PSS_V5: the reference's num_threads_, 1 to 16 queues.  Above 1 the
+                              queues run in lockstep in a fixed order (see panns_search); the
+                              other modes take 1 */
     uint32_t stop;         /* panns_stop */
     float stop_threshold;  /* LEARNED: stop when sigmoid output >= threshold */
     uint32_t hash_slots;   /* 0 = automatic; else a multiple of 64 in [64, 32768]: 16-bit slots of
@@ -152,10 +154,20 @@ int panns_prepare_init_ids(panns_index *h, uint32_t L, uint32_t *init_ids);
 /* One batch of queries (ids into the loaded query set) searched with the same
  * parameters and the same init_ids (n_init >= L).  Covers, per query:
  *   TOP_M   -> para_search_with_top_m_pure_large_graph (TOPMLG.cpp:2022-2207)
- *   PSS_V5  -> para_search_PSS_v5_large_graph_dist_thresh_profiling at
- *              num_threads_ = 1 (PSS5LG.cpp:1376-1724)
+ *   PSS_V5  -> para_search_PSS_v5_large_graph_dist_thresh_profiling
+ *              (PSS5LG.cpp:1376-1724) with num_threads_ = p->num_threads
+ *              queues and X = p->M sub-search steps
  *   SCALE_M -> initialize_set_L_para then subsearch_with_top_m
- * Results are identical in ids to the reference's num_threads_ = 1 run. */
+ * Results are identical in ids to the reference's num_threads_ = 1 run.
+ * PSS_V5 with T > 1 queues: the reference's T threads race on the visited
+ * bitset and the master's last distance, so its ids vary between runs;
+ * here pick_top_m_to_workers (PSS5LG.cpp:917-947) deals the master's
+ * unchecked entries to the workers, then the T expansion loops
+ * (PSS5LG.cpp:1584-1646) run in lockstep -- step s expands the s-th
+ * candidate of worker 0, 1, ..., T-2, then of the master -- and
+ * merge_all_queues_to_master (PSS5LG.cpp:811-842) folds the worker queues
+ * (L entries each) into the master.  That is one schedule the reference can
+ * take, and every run gives the same ids; count_merge_ counts the rounds. */
 int panns_search(panns_index *h, const panns_params *p, const uint32_t *query_ids, uint32_t nq,
                  const uint32_t *init_ids, uint32_t n_init, panns_outputs *out,
                  panns_counters *cnt);

@@ -15,6 +15,7 @@
 #include "panns_tools.h"
 #include "search_kernel.h"
 #include "search_kernel_mw.h"
+#include "search_kernel_pss.h"
 #include "knn_kernel.h"
 #include "mlp_kernels.h"
 
@@ -147,6 +148,7 @@ struct panns_plan {
     uint32_t overflow_slots = 0;  // LDS overflow entries of the visited set (0: none)
     bool shape_given = false;     // launch shape from panns_plan_options (else automatic)
     uint32_t waves = 1;  // wavefronts per query
+    bool pss = false;    // PSS_v5 with num_threads > 1 (search_kernel_pss)
     // work item i is the caller's query perm[i] (empty: the caller's order)
     std::vector<uint32_t> perm;
     bool locality = true;
@@ -192,6 +194,13 @@ template <int D, int W, bool kNorm>
 hipError_t launch_search_mw(const panns::KArgs &a, uint32_t grid, uint32_t lds, hipStream_t s)
 {
     hipLaunchKernelGGL((panns::search_kernel_mw<D, W, kNorm>), dim3(grid), dim3(64 * W), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int D, bool kNorm>
+hipError_t launch_search_pss(const panns::KArgs &a, uint32_t grid, uint32_t lds, hipStream_t s)
+{
+    hipLaunchKernelGGL((panns::search_kernel_pss<D, kNorm>), dim3(grid), dim3(64), lds, s, a);
     return hipGetLastError();
 }
 
@@ -256,8 +265,32 @@ bool kernel_for_dim_mw(uint32_t dim, uint32_t waves, bool norm, LaunchFn *fn, Ke
     }
 }
 
-bool select_kernel(uint32_t dim, bool learned, bool norm, bool trace, uint32_t waves, LaunchFn *fn, KernelPtr *kp)
+// The PSS_v5 kernel with T > 1 queues for a dimension.
+bool kernel_for_dim_pss(uint32_t dim, bool norm, LaunchFn *fn, KernelPtr *kp)
 {
+    switch (dim) {
+#define PANNS_DIM(DD)                                                                                      \
+    case DD:                                                                                               \
+        *fn = norm ? launch_search_pss<DD, true> : launch_search_pss<DD, false>;                            \
+        *kp = norm ? reinterpret_cast<const void *>(&panns::search_kernel_pss<DD, true>)                    \
+                   : reinterpret_cast<const void *>(&panns::search_kernel_pss<DD, false>);                  \
+        return true;
+        PANNS_DIM(16)
+        PANNS_DIM(32)
+        PANNS_DIM(40)
+        PANNS_DIM(64)
+        PANNS_DIM(96)
+        PANNS_DIM(128)
+#undef PANNS_DIM
+    default:
+        return false;
+    }
+}
+
+bool select_kernel(uint32_t dim, bool learned, bool norm, bool trace, uint32_t waves, bool pss, LaunchFn *fn,
+                   KernelPtr *kp)
+{
+    if (pss) return kernel_for_dim_pss(dim, norm, fn, kp);
     return waves > 1 ? kernel_for_dim_mw(dim, waves, norm, fn, kp)
                      : kernel_for_dim(dim, learned, norm, trace, fn, kp);
 }
@@ -322,9 +355,17 @@ int check_params(panns_index *h, const panns_params *p, uint32_t L, uint32_t K)
 {
     if (!p) return fail(PANNS_ERR_ARG, "null params");
     if (p->mode > 2) return fail(PANNS_ERR_ARG, "unknown mode");
-    if (p->mode == PANNS_MODE_PSS_V5 && p->num_threads != 1)
-        return fail(PANNS_ERR_UNSUPPORTED,
-                    "PSS_v5 with num_threads_ > 1 is nondeterministic in the reference; only num_threads_ = 1 is offered");
+    if (p->mode == PANNS_MODE_PSS_V5 && (p->num_threads == 0 || p->num_threads > panns::kPssMaxQueues))
+        return fail(PANNS_ERR_UNSUPPORTED, "PSS_v5 runs with num_threads_ from 1 to 16");
+    if (p->mode != PANNS_MODE_PSS_V5 && p->num_threads > 1)
+        return fail(PANNS_ERR_UNSUPPORTED, "the top-M searches run with num_threads_ = 1 (PSS_v5 takes more)");
+    if (p->mode == PANNS_MODE_PSS_V5 && p->num_threads > 1) {
+        if (p->waves > 1) return fail(PANNS_ERR_UNSUPPORTED, "PSS_v5 with num_threads_ > 1 runs with waves = 1");
+        if (p->stop != PANNS_STOP_FIXED)
+            return fail(PANNS_ERR_UNSUPPORTED, "PSS_v5 with num_threads_ > 1 runs with the fixed stop");
+        if (h->stride_log2 > 8)
+            return fail(PANNS_ERR_UNSUPPORTED, "PSS_v5 with num_threads_ > 1 needs a graph degree of at most 256");
+    }
     if (L == 0 || K == 0) return fail(PANNS_ERR_ARG, "K and L must be positive");
     if (K > L) return fail(PANNS_ERR_ARG, "K > L");
     if (L > h->n) return fail(PANNS_ERR_ARG, "L > number of base vectors");
@@ -361,6 +402,15 @@ uint32_t lds_bytes_mw(uint32_t Lcap, uint32_t nb_cap, uint32_t tbl_slots, uint32
 {
     return 8u * (Lcap + W * nb_cap + 64u) + 4u * dim + 2u * tbl_slots + 4u * (W * nb_cap + Mcap) +
            4u * panns::kCtrlWords;
+}
+
+// search_kernel_pss's carving: Q[T][Lcap] cand[nb_cap] bb[64] (8 B each),
+// query row, visited table, newids/newown/candown[nb_cap], per-queue words
+// and counts (4 B each).
+uint32_t lds_bytes_pss(uint32_t Lcap, uint32_t nb_cap, uint32_t tbl_slots, uint32_t dim, uint32_t T)
+{
+    return 8u * (T * Lcap + nb_cap + 64u) + 4u * dim + 2u * tbl_slots + 4u * 3u * nb_cap +
+           4u * (panns::kWsN * panns::kPssMaxQueues + 4u);
 }
 
 // One N-bit bitmap per persistent workgroup, padded to at least 64 words:
@@ -469,11 +519,14 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     pl->waves = std::max<uint32_t>(1u, p.waves);
     if (pl->waves > 1 && (pl->init_queue.p || pl->trace_cap || pl->shared_bitmap))
         return fail(PANNS_ERR_UNSUPPORTED, "traces and caller-held queues run with waves = 1");
+    pl->pss = p.mode == PANNS_MODE_PSS_V5 && p.num_threads > 1;
+    if (pl->pss && (pl->init_queue.p || pl->trace_cap || pl->shared_bitmap))
+        return fail(PANNS_ERR_UNSUPPORTED, "traces and caller-held queues run with num_threads_ = 1");
     const bool norm = h->distance == PANNS_DISTANCE_NORM_DOT;
     if (norm && pl->trace_cap) return fail(PANNS_ERR_UNSUPPORTED, "traces are taken with squared-L2 distances");
     LaunchFn fn;
     KernelPtr kp;
-    if (!select_kernel(h->dim, p.stop == PANNS_STOP_LEARNED, norm, pl->trace_cap != 0, pl->waves, &fn, &kp))
+    if (!select_kernel(h->dim, p.stop == PANNS_STOP_LEARNED, norm, pl->trace_cap != 0, pl->waves, pl->pss, &fn, &kp))
         return fail(PANNS_ERR_DIM, "dimension " + std::to_string(h->dim) + " has no kernel (16, 32, 40, 64, 96, 128)");
     pl->maxL = p.L;
     pl->maxK = p.K;
@@ -500,8 +553,12 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     const uint32_t stride = 1u << h->stride_log2;
     // multi-wave: per-wave capacity for its share of the picks' neighbours
     const uint32_t W = pl->waves;
-    const uint32_t nb_cap = W > 1 ? std::max<uint32_t>(panns::kGroupSlots, (p.M + W - 1) / W * stride)
-                                  : std::max<uint32_t>(panns::kGroupSlots, stride);
+    // PSS_v5 with T queues: one step's neighbours (T rows), and a worker
+    // queue's entries staged for the merge into the master
+    const uint32_t nb_cap =
+        pl->pss ? (std::max<uint32_t>({panns::kGroupSlots, p.num_threads * stride, (pl->maxL + 7u) & ~7u}) + 7u) & ~7u
+        : W > 1 ? std::max<uint32_t>(panns::kGroupSlots, (p.M + W - 1) / W * stride)
+                : std::max<uint32_t>(panns::kGroupSlots, stride);
     const uint32_t Mcap = std::max<uint32_t>(4u, (p.M + 3u) & ~3u);
     // visited table: hs 16-bit slots in nb buckets of 8; the tag of an id
     // is its scrambled value divided by nb and must stay below 2^15 (0xFFFF
@@ -537,8 +594,9 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     const uint32_t Lcap = (pl->maxL + 7u) & ~7u;
     const uint32_t ovf = pl->shared_bitmap ? 0u : pl->overflow_slots;
     // the overflow entries are 32-bit, i.e. two 16-bit slots each in the carving
-    pl->lds = W > 1 ? lds_bytes_mw(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim, W)
-                    : lds_bytes(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim);
+    pl->lds = pl->pss ? lds_bytes_pss(Lcap, nb_cap, hs + 2u * ovf, h->dim, p.num_threads)
+              : W > 1 ? lds_bytes_mw(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim, W)
+                      : lds_bytes(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim);
     if (pl->lds > 65536) {
         HIP_TRY(hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->lds));
     }
@@ -650,6 +708,7 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     a.nb_cap = nb_cap;
     a.group_picks = std::max<uint32_t>(1u, nb_cap / stride);
     a.group_rows = pl->rows_per_wave ? pl->rows_per_wave : 96u;
+    a.num_threads = p.num_threads;
     a.init_queue = pl->init_queue.p;
     a.shared_bitmap = pl->shared_bitmap;
     a.stop_learned = p.stop == PANNS_STOP_LEARNED ? 1u : 0u;
@@ -694,7 +753,7 @@ hipError_t launch_one(const panns_plan *pl, hipStream_t st)
     LaunchFn fn;
     KernelPtr kp;
     select_kernel(pl->h->dim, pl->args.stop_learned != 0, pl->args.norms != nullptr, pl->args.trace_feat != nullptr,
-                  pl->waves, &fn, &kp);
+                  pl->waves, pl->pss, &fn, &kp);
     return fn(pl->args, pl->grid, pl->lds, st);
 }
 
@@ -817,6 +876,9 @@ int fetch_single(panns_plan *pl, panns_outputs *out, panns_counters *cnt, float 
             cnt->distance_computations += s[0];
             cnt->expansions += s[1];
             if (pl->p.mode == PANNS_MODE_TOP_M) {
+                cnt->merges += s[2];
+            } else if (pl->pss) {
+                // rounds of pick, expansion and merge, counted by the kernel
                 cnt->merges += s[2];
             } else if (pl->p.mode == PANNS_MODE_PSS_V5) {
                 // PSS5LG.cpp:1543-1673 at T=1: one sequential expansion, then

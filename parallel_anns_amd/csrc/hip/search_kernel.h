@@ -96,6 +96,7 @@ struct KArgs {
     uint32_t ovf_slots;   // LDS overflow entries (full ids) after the buckets: 0 or a power of two
     uint32_t Mcap, nb_cap, group_picks;
     uint32_t group_rows;  // neighbour rows a wave gathers per distance step: 32, 64 or 96
+    uint32_t num_threads; // PSS_v5 queues (the reference's num_threads_), search_kernel_pss only
     // subsearch on a caller-held queue and visited bitmap
     const uint64_t *init_queue;  // [n_work][L] keys, or null
     uint32_t *shared_bitmap;     // used instead of bitmaps when non-null; never cleared
@@ -458,7 +459,9 @@ __device__ __forceinline__ void merge_sync()
     }
 }
 
-template <bool kWave = false>
+// kFlags: the batch's keys carry checked flags (bit 31), which are kept in
+// the queue and left out of every comparison.
+template <bool kWave = false, bool kFlags = false>
 __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32_t L, uint64_t x,
                                                 uint32_t nb, uint64_t *bb, int lane,
                                                 uint32_t *ninserted = nullptr)
@@ -466,7 +469,7 @@ __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32
     uint32_t pos = kSent;
     bool valid = false;
     if ((uint32_t)lane < nb) {
-        pos = lb_queue(Q, S, x) + (uint32_t)lane;
+        pos = lb_queue(Q, S, kFlags ? km(x) : x) + (uint32_t)lane;
         valid = pos < L;
     }
     const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
@@ -488,7 +491,7 @@ __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32
             while (cnt > 0) {
                 const uint32_t step = cnt >> 1;
                 const uint32_t mid = c + step;
-                if (bb[mid] < ek) {
+                if ((kFlags ? km(bb[mid]) : bb[mid]) < ek) {
                     c = mid + 1;
                     cnt -= step + 1;
                 } else {
@@ -851,12 +854,14 @@ __device__ __forceinline__ float mlp_eval(const LaneModel &lm, const float (&f)[
 
 // Distances for ids[cnt0, cnt0 + 32 * NV) of ids[0, cnt) (in LDS), one
 // lane pair per id with NV rows per pair in flight; keys below `bound` are
-// appended to cand[ncand...].  Returns the new ncand.
+// appended to cand[ncand...].  Returns the new ncand.  With own_in, each
+// appended key's owner own_in[i] goes to own_out at the same position.
 template <int D, int NV, bool kNorm>
 __device__ __forceinline__ uint32_t distance_group(const float *__restrict__ base, const float *__restrict__ norms,
                                                    const uint32_t *ids, uint32_t cnt0, uint32_t cnt,
                                                    const float4 *qs, uint64_t bound, uint64_t *cand,
-                                                   uint32_t ncand, int lane, uint64_t lt)
+                                                   uint32_t ncand, int lane, uint64_t lt,
+                                                   const uint32_t *own_in = nullptr, uint32_t *own_out = nullptr)
 {
     const int half = lane & 1;
     uint32_t id[NV];
@@ -875,7 +880,11 @@ __device__ __forceinline__ uint32_t distance_group(const float *__restrict__ bas
         const uint64_t kj = ((uint64_t)dkey<kNorm>(d[j]) << 32) | (uint64_t)id[j];
         const bool sj = ok[j] && half && kj < bound;
         const uint64_t mj = __ballot(sj);
-        if (sj) cand[ncand + (uint32_t)__popcll(mj & lt)] = kj;
+        if (sj) {
+            const uint32_t at = ncand + (uint32_t)__popcll(mj & lt);
+            cand[at] = kj;
+            if (own_in) own_out[at] = own_in[cnt0 + (uint32_t)(lane >> 1) + 32u * (uint32_t)j];
+        }
         ncand += (uint32_t)__popcll(mj);
     }
     return ncand;
@@ -892,18 +901,22 @@ template <int D, bool kWide, bool kNorm>
 __device__ __forceinline__ uint32_t distance_pass(const float *__restrict__ base, const float *__restrict__ norms,
                                                   const uint32_t *ids, uint32_t cnt, const float4 *qs,
                                                   uint64_t bound, uint64_t *cand, uint32_t ncand, int lane,
-                                                  uint64_t lt, uint32_t rows)
+                                                  uint64_t lt, uint32_t rows, const uint32_t *own_in = nullptr,
+                                                  uint32_t *own_out = nullptr)
 {
     uint32_t p = 0;
     while (p < cnt) {
         if (kWide && rows >= 96u && cnt - p > 64u) {
-            ncand = distance_group<D, 3, kNorm>(base, norms, ids, p, cnt, qs, bound, cand, ncand, lane, lt);
+            ncand = distance_group<D, 3, kNorm>(base, norms, ids, p, cnt, qs, bound, cand, ncand, lane, lt, own_in,
+                                                own_out);
             p += 96u;
         } else if (rows >= 64u && cnt - p > 32u) {
-            ncand = distance_group<D, 2, kNorm>(base, norms, ids, p, cnt, qs, bound, cand, ncand, lane, lt);
+            ncand = distance_group<D, 2, kNorm>(base, norms, ids, p, cnt, qs, bound, cand, ncand, lane, lt, own_in,
+                                                own_out);
             p += 64u;
         } else {
-            ncand = distance_group<D, 1, kNorm>(base, norms, ids, p, cnt, qs, bound, cand, ncand, lane, lt);
+            ncand = distance_group<D, 1, kNorm>(base, norms, ids, p, cnt, qs, bound, cand, ncand, lane, lt, own_in,
+                                                own_out);
             p += 32u;
         }
     }

@@ -196,7 +196,9 @@ class Index:
                num_threads=1, stop="fixed", stop_threshold=0.5, hash_slots=0, waves=1, stop_min_iter=0,
                stop_every=1, bucket_slots=0):
         """Batch form of para_search_with_top_m_pure_large_graph /
-        para_search_PSS_v5_large_graph_dist_thresh_profiling (num_threads_=1)."""
+        para_search_PSS_v5_large_graph_dist_thresh_profiling; mode "pss_v5"
+        takes M as X and num_threads as the reference's num_threads_ (its
+        T queues run in lockstep in a fixed order)."""
         qids = np.ascontiguousarray(np.atleast_1d(query_ids), dtype=np.uint32)
         nq = qids.shape[0]
         if init_ids is None:
@@ -244,7 +246,7 @@ class Index:
 
     def plan(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, L_per_query=None,
              K_per_query=None, stop="fixed", stop_threshold=0.5, hash_slots=0, options=None,
-             want_queue=True, waves=1, stop_min_iter=0, stop_every=1, bucket_slots=0):
+             want_queue=True, waves=1, stop_min_iter=0, stop_every=1, bucket_slots=0, num_threads=1):
         """A resident batch (panns_plan_create_ex).  options: a
         PannsPlanOptions (plan_options() or autotune()) for queue-size
         buckets, graph replay and side-by-side buckets; want_queue keeps the
@@ -253,8 +255,8 @@ class Index:
         if init_ids is None:
             init_ids = self.prepare_init_ids(maxL)
         init_ids = np.ascontiguousarray(init_ids, dtype=np.uint32)
-        p = make_params(K, L, M, mode, 1, stop, stop_threshold, hash_slots, waves, stop_min_iter, stop_every,
-                        bucket_slots)
+        p = make_params(K, L, M, mode, num_threads, stop, stop_threshold, hash_slots, waves, stop_min_iter,
+                        stop_every, bucket_slots)
         o = _native.PannsPlanOptions() if options is None else options
         o.want_queue = int(want_queue)
         h = _native.vp()
@@ -264,14 +266,14 @@ class Index:
 
     def autotune(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, L_per_query=None,
                  K_per_query=None, stop="fixed", stop_threshold=0.5, sample_stride=0, stop_min_iter=0,
-                 stop_every=1):
+                 stop_every=1, num_threads=1):
         """The fastest bucketing for this batch (panns_plan_autotune): returns
         (PannsPlanOptions, device ms on the sample)."""
         qids, Lq, Kq, maxL, maxK = self._batch_args(query_ids, K, L, L_per_query, K_per_query)
         if init_ids is None:
             init_ids = self.prepare_init_ids(maxL)
         init_ids = np.ascontiguousarray(init_ids, dtype=np.uint32)
-        p = make_params(K, L, M, mode, 1, stop, stop_threshold, 0, 1, stop_min_iter, stop_every)
+        p = make_params(K, L, M, mode, num_threads, stop, stop_threshold, 0, 1, stop_min_iter, stop_every)
         best = _native.PannsPlanOptions()
         ms = _native.c_float(0.0)
         _chk(self.lib.panns_plan_autotune(self.h, p, ptr(qids), qids.shape[0], ptr(Lq), ptr(Kq),

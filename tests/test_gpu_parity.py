@@ -320,7 +320,7 @@ def test_error_paths(small_set, gpu_index):
     q = np.arange(2, dtype=np.uint32)
     init = gpu_index.prepare_init_ids(100)
     bad = [dict(K=101, L=100, M=1), dict(K=10, L=100, M=0), dict(K=10, L=30000, M=1),
-           dict(K=10, L=100, M=1, mode="pss_v5", num_threads=4)]
+           dict(K=10, L=100, M=1, mode="pss_v5", num_threads=17), dict(K=10, L=100, M=1, num_threads=4)]
     for kw in bad:
         with pytest.raises(SearchError):
             gpu_index.search(q, init_ids=init, **kw)
