@@ -172,6 +172,23 @@ int panns_search(panns_index *h, const panns_params *p, const uint32_t *query_id
                  const uint32_t *init_ids, uint32_t n_init, panns_outputs *out,
                  panns_counters *cnt);
 
+/* panns_search without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops that must not call hipMalloc, harnesses that forbid it).  Same
+ * arguments, results and errors as panns_search; every device buffer the
+ * search needs beyond the handle's resident base rows, queries and graph
+ * (query ids, init ids, outputs, work counters, visited bitmaps and spill
+ * logs) comes from one 64 MiB array in the library, one copy per GPU, held
+ * until the process ends.  A batch whose buffers do not fit (with at least
+ * one workgroup's N-bit bitmap) returns PANNS_ERR_ARG; within the limit the
+ * number of persistent workgroups is what the rest of the array holds.
+ * Queries run in the caller's order (no locality reordering).  The opt-graph
+ * distance needs its norms computed first (panns_set_distance).  All calls
+ * on one GPU share the array: calls that overlap on one GPU, from any handle
+ * or thread, race and must not be made. */
+int panns_search_noalloc(panns_index *h, const panns_params *p, const uint32_t *query_ids, uint32_t nq,
+                         const uint32_t *init_ids, uint32_t n_init, panns_outputs *out, panns_counters *cnt);
+
 /* TOPMLG.cpp:1023-1071 subsearch_with_top_m on a caller-held, already
  * initialised queue.  queue_* hold local_L entries sorted by (dist, id);
  * *queue_size must equal local_L (a full queue; the reference's callers

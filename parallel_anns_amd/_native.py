@@ -93,6 +93,8 @@ AMD_API = {
     "panns_get_info": (c_int, [vp, P(c_u32), P(c_u32), P(c_u32), P(c_u32), P(c_u32), P(c_u64)]),
     "panns_prepare_init_ids": (c_int, [vp, c_u32, vp]),
     "panns_search": (c_int, [vp, P(PannsParams), vp, c_u32, vp, c_u32, P(PannsOutputs), P(PannsCounters)]),
+    "panns_search_noalloc": (c_int, [vp, P(PannsParams), vp, c_u32, vp, c_u32, P(PannsOutputs),
+                                     P(PannsCounters)]),
     "panns_subsearch_with_top_m": (c_int, [vp, c_u32, c_u32, c_u32, vp, vp, vp, P(c_u32), vp, P(c_u64)]),
     "panns_plan_create": (c_int, [vp, P(PannsParams), vp, c_u32, vp, vp, vp, c_u32, P(vp)]),
     "panns_plan_run": (c_int, [vp, c_int, P(c_float)]),

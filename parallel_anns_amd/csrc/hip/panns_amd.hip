@@ -8,6 +8,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -79,6 +80,39 @@ constexpr uint32_t kRanges = 8;         // work ranges (one per XCD)
 constexpr uint32_t kPivots = 256;       // locality pivots
 constexpr uint32_t kLocalityMin = 512;  // smallest batch that is reordered
 
+}  // namespace
+
+// Device memory of panns_search_noalloc: every buffer that search needs
+// beyond the handle's resident base rows, queries and graph (query ids, init
+// ids, outputs, work counters, visited bitmaps and spill logs) is carved
+// from this file-scope array.  Each GPU the library runs on holds its own
+// copy until the process ends; host code must use the per-device address
+// from hipGetSymbolAddress, never the array's name.
+constexpr size_t kNoAllocBytes = 64ull << 20;
+constexpr uint32_t kNoAllocLogCap = 1024;  // spill log entries per workgroup (more: the whole bitmap is cleared)
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kNoAllocDevices] = {};
+
+// The address of this device's copy of `scratch`, looked up once per device
+// with that device current (the caller has made it current).
+int scratch_address(int device, uint8_t **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kNoAllocDevices) return fail(PANNS_ERR_DEVICE, "device ordinal above 63");
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(PANNS_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = static_cast<uint8_t *>(g_scratch_ptr[device]);
+    return PANNS_OK;
+}
 }  // namespace
 
 struct panns_index {
@@ -511,8 +545,15 @@ int locality_order(panns_index *h, const uint32_t *query_ids, uint32_t nq, std::
     return PANNS_OK;
 }
 
-int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, const uint32_t *Kq,
-               const uint32_t *init_ids, uint32_t n_init)
+// Launch geometry of a plan (no device memory): the parameter checks, the
+// kernel, the LDS carving and the persistent grid.  Fills *geo.
+struct Geometry {
+    uint32_t Lcap = 0, nb_cap = 0, Mcap = 0, nb = 0, bslots = 8, id_bits = 1;
+    bool norm = false;
+};
+
+int plan_geometry(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, const uint32_t *Kq,
+                  const uint32_t *init_ids, uint32_t n_init, Geometry *geo)
 {
     panns_index *h = pl->h;
     const panns_params &p = pl->p;
@@ -605,6 +646,41 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     if (per_cu <= 0) return fail(PANNS_ERR_UNSUPPORTED, "kernel does not fit on a CU with this L");
     pl->grid = std::min<uint32_t>(pl->nq, (uint32_t)per_cu * (uint32_t)h->num_cu);
     if (pl->grid == 0) pl->grid = 1;
+    geo->Lcap = Lcap;
+    geo->nb_cap = nb_cap;
+    geo->Mcap = Mcap;
+    geo->nb = nb;
+    geo->bslots = bslots;
+    geo->id_bits = id_bits;
+    geo->norm = norm;
+    if (norm) {
+        rc = ensure_norms(h);
+        if (rc) return rc;
+    }
+    return PANNS_OK;
+}
+
+// Device buffers a plan's kernel reads and writes.
+struct PlanBufs {
+    const uint32_t *qids = nullptr, *qL = nullptr, *qK = nullptr, *init_ids = nullptr;
+    uint32_t *work_counter = nullptr;
+    uint32_t *out_ids = nullptr, *out_stats = nullptr, *out_qids = nullptr;
+    float *out_dists = nullptr, *out_qdists = nullptr;
+    uint8_t *out_qchk = nullptr;
+    uint32_t *bitmaps = nullptr, *spill_log = nullptr;
+    uint64_t bitmap_words = 0;
+    uint32_t log_cap = 0;
+};
+
+void fill_args(panns_plan *pl, const Geometry &geo, const PlanBufs &b);
+
+int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, const uint32_t *Kq,
+               const uint32_t *init_ids, uint32_t n_init)
+{
+    panns_index *h = pl->h;
+    Geometry geo;
+    int rc = plan_geometry(pl, query_ids, Lq, Kq, init_ids, n_init, &geo);
+    if (rc) return rc;
     rc = ensure_scratch(h, pl->slot0 + pl->grid);
     if (rc) return rc;
 
@@ -660,22 +736,44 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     }
     HIP_TRY(hipStreamSynchronize(h->stream));
 
+    PlanBufs b;
+    b.qids = pl->qids.p;
+    b.qL = Lq ? pl->qL.p : nullptr;
+    b.qK = Kq ? pl->qK.p : nullptr;
+    b.init_ids = pl->init_ids.p;
+    b.work_counter = h->counter.p + (size_t)pl->counter_idx * kRanges;
+    b.out_ids = pl->out_ids.p;
+    b.out_dists = pl->out_dists.p;
+    b.out_stats = pl->out_stats.p;
+    if (pl->want_queue) {
+        b.out_qids = pl->out_qids.p;
+        b.out_qdists = pl->out_qdists.p;
+        b.out_qchk = pl->out_qchk.p;
+    }
+    b.bitmaps = h->bitmaps.p + (size_t)pl->slot0 * h->bitmap_words;
+    b.bitmap_words = h->bitmap_words;
+    b.spill_log = h->spill_log.p + (size_t)pl->slot0 * kLogCap;
+    b.log_cap = kLogCap;
+    fill_args(pl, geo, b);
+    return PANNS_OK;
+}
+
+// The kernel arguments of a plan from its geometry and device buffers.
+void fill_args(panns_plan *pl, const Geometry &geo, const PlanBufs &b)
+{
+    panns_index *h = pl->h;
+    const panns_params &p = pl->p;
     panns::KArgs &a = pl->args;
     a = panns::KArgs{};
     a.base = h->base.p;
-    a.norms = nullptr;
-    if (norm) {
-        int nrc = ensure_norms(h);
-        if (nrc) return nrc;
-        a.norms = h->norms.p;
-    }
+    a.norms = geo.norm ? h->norms.p : nullptr;
     a.adj = h->adj.p;
     a.n = h->n;
     a.stride_log2 = h->stride_log2;
     a.queries = h->queries.p;
-    a.qids = pl->qids.p;
-    a.qL = Lq ? pl->qL.p : nullptr;
-    a.qK = Kq ? pl->qK.p : nullptr;
+    a.qids = b.qids;
+    a.qL = b.qL;
+    a.qK = b.qK;
     a.n_work = pl->nq;
     a.L = p.L;
     a.K = p.K;
@@ -683,30 +781,30 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     a.mode = p.mode;
     a.Kstride = pl->maxK;
     a.Lstride = pl->maxL;
-    a.init_ids = pl->init_ids.p;
-    a.work_counter = h->counter.p + (size_t)pl->counter_idx * kRanges;
+    a.init_ids = b.init_ids;
+    a.work_counter = b.work_counter;
     a.n_ranges = kRanges;
-    a.out_ids = pl->out_ids.p;
-    a.out_dists = pl->out_dists.p;
-    a.out_qids = pl->want_queue ? pl->out_qids.p : nullptr;
-    a.out_qdists = pl->want_queue ? pl->out_qdists.p : nullptr;
-    a.out_qchk = pl->want_queue ? pl->out_qchk.p : nullptr;
-    a.out_stats = pl->out_stats.p;
-    a.bitmaps = h->bitmaps.p + (size_t)pl->slot0 * h->bitmap_words;
-    a.bitmap_words = h->bitmap_words;
-    a.spill_log = h->spill_log.p + (size_t)pl->slot0 * kLogCap;
-    a.log_cap = kLogCap;
-    a.Lcap = Lcap;
+    a.out_ids = b.out_ids;
+    a.out_dists = b.out_dists;
+    a.out_qids = b.out_qids;
+    a.out_qdists = b.out_qdists;
+    a.out_qchk = b.out_qchk;
+    a.out_stats = b.out_stats;
+    a.bitmaps = b.bitmaps;
+    a.bitmap_words = b.bitmap_words;
+    a.spill_log = b.spill_log;
+    a.log_cap = b.log_cap;
+    a.Lcap = geo.Lcap;
     a.ovf_slots = pl->shared_bitmap ? 0u : pl->overflow_slots;
-    a.tbl_words = hs / 2u + a.ovf_slots;
-    a.tbl_buckets = nb;
-    a.tbl_bshift = bslots == 8u ? 2u : 1u;
-    a.tbl_magic = (uint32_t)((1ull << 32) / nb);
-    a.id_bits = id_bits;
+    a.tbl_words = pl->hash_slots / 2u + a.ovf_slots;
+    a.tbl_buckets = geo.nb;
+    a.tbl_bshift = geo.bslots == 8u ? 2u : 1u;
+    a.tbl_magic = (uint32_t)((1ull << 32) / geo.nb);
+    a.id_bits = geo.id_bits;
     a.use_table = pl->shared_bitmap ? 0u : 1u;
-    a.Mcap = Mcap;
-    a.nb_cap = nb_cap;
-    a.group_picks = std::max<uint32_t>(1u, nb_cap / stride);
+    a.Mcap = geo.Mcap;
+    a.nb_cap = geo.nb_cap;
+    a.group_picks = std::max<uint32_t>(1u, geo.nb_cap >> h->stride_log2);
     a.group_rows = pl->rows_per_wave ? pl->rows_per_wave : 96u;
     a.num_threads = p.num_threads;
     a.init_queue = pl->init_queue.p;
@@ -731,7 +829,6 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     a.trace_cap = pl->trace_cap;
     a.phase_cycles = pl->phase_cycles.p;
     pl->input_gen = h->input_gen;
-    return PANNS_OK;
 }
 
 // Points a plan's kernel arguments at the handle's current scratch (another
@@ -836,6 +933,30 @@ int run_plan(panns_plan *pl, int sync, float *ms)
     return PANNS_OK;
 }
 
+// Adds n queries' per-query stats ([ndist, nexp, niter, spills, adjacency
+// slots] each) to the counters, count_merge_ as the reference counts it.
+void add_counters(const panns_plan *pl, const uint32_t *st, uint32_t n, panns_counters *cnt)
+{
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t *s = &st[(size_t)i * 5];
+        cnt->distance_computations += s[0];
+        cnt->expansions += s[1];
+        if (pl->p.mode == PANNS_MODE_TOP_M) {
+            cnt->merges += s[2];
+        } else if (pl->pss) {
+            // rounds of pick, expansion and merge, counted by the kernel
+            cnt->merges += s[2];
+        } else if (pl->p.mode == PANNS_MODE_PSS_V5) {
+            // PSS5LG.cpp:1543-1673 at T=1: one sequential expansion, then
+            // rounds of up to X expansions, one count_merge_ per round.
+            const uint32_t X = pl->p.M;
+            if (s[1] > 1) cnt->merges += (s[1] - 1 + X - 1) / X;
+        }
+        cnt->visited_spills += s[3];
+        cnt->adjacency_slots += s[4];
+    }
+}
+
 int fetch_single(panns_plan *pl, panns_outputs *out, panns_counters *cnt, float kernel_ms)
 {
     panns_index *h = pl->h;
@@ -871,24 +992,7 @@ int fetch_single(panns_plan *pl, panns_outputs *out, panns_counters *cnt, float 
         }
     }
     if (cnt) {
-        for (uint32_t i = 0; i < pl->nq; ++i) {
-            const uint32_t *s = &st[(size_t)i * 5];
-            cnt->distance_computations += s[0];
-            cnt->expansions += s[1];
-            if (pl->p.mode == PANNS_MODE_TOP_M) {
-                cnt->merges += s[2];
-            } else if (pl->pss) {
-                // rounds of pick, expansion and merge, counted by the kernel
-                cnt->merges += s[2];
-            } else if (pl->p.mode == PANNS_MODE_PSS_V5) {
-                // PSS5LG.cpp:1543-1673 at T=1: one sequential expansion, then
-                // rounds of up to X expansions, one count_merge_ per round.
-                const uint32_t X = pl->p.M;
-                if (s[1] > 1) cnt->merges += (s[1] - 1 + X - 1) / X;
-            }
-            cnt->visited_spills += s[3];
-            cnt->adjacency_slots += s[4];
-        }
+        add_counters(pl, st.data(), pl->nq, cnt);
         cnt->kernel_ms += kernel_ms;
     }
     return PANNS_OK;
@@ -1034,6 +1138,11 @@ int panns_create(int device, panns_index **out)
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     h->num_cu = prop.multiProcessorCount;
+    // this device's allocation-free scratch (panns_search_noalloc), looked
+    // up once per device; a failure here is reported by that entry point
+    uint8_t *scr = nullptr;
+    (void)scratch_address(device, &scr);
+    g_err.clear();
     *out = h.release();
     return PANNS_OK;
 }
@@ -1559,6 +1668,112 @@ int panns_search(panns_index *h, const panns_params *p, const uint32_t *query_id
     rc = run_plan(pl, 1, &ms);
     if (rc) return rc;
     return fetch_plan(pl, out, cnt, ms);
+}
+
+int panns_search_noalloc(panns_index *h, const panns_params *p, const uint32_t *query_ids, uint32_t nq,
+                         const uint32_t *init_ids, uint32_t n_init, panns_outputs *out, panns_counters *cnt)
+{
+    if (!out || !out->ids) return fail(PANNS_ERR_ARG, "outputs.ids is required");
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!p || !query_ids || !init_ids || nq == 0) return fail(PANNS_ERR_ARG, "bad search arguments");
+    if (!h->nq) return fail(PANNS_ERR_STATE, "queries not loaded");
+    HIP_TRY(hipSetDevice(h->device));
+    if (h->distance == PANNS_DISTANCE_NORM_DOT && !h->norms_valid)
+        return fail(PANNS_ERR_STATE, "vertex norms not computed: call panns_set_distance(NORM_DOT) first");
+    uint8_t *ptr = nullptr;
+    rc = scratch_address(h->device, &ptr);
+    if (rc) return rc;
+    // the plan lives on the stack and owns no device memory
+    panns_plan pl;
+    pl.h = h;
+    pl.p = *p;
+    pl.nq = nq;
+    pl.locality = false;
+    pl.want_queue = out->queue_ids || out->queue_dists || out->queue_checked;
+    Geometry geo;
+    rc = plan_geometry(&pl, query_ids, nullptr, nullptr, init_ids, n_init, &geo);
+    if (rc) return rc;
+    // carve the scratch: fixed buffers first, then as many workgroups'
+    // bitmaps and spill logs as fit (at most the occupancy grid)
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t K = pl.maxK, L = pl.maxL;
+    const size_t o_cnt = take(kRanges * 4), o_q = take((size_t)nq * 4), o_init = take(L * 4);
+    const size_t o_ids = take(nq * K * 4), o_d = take(nq * K * 4), o_st = take((size_t)nq * 5 * 4);
+    size_t o_qi = 0, o_qd = 0, o_qc = 0;
+    if (pl.want_queue) {
+        o_qi = take(nq * L * 4);
+        o_qd = take(nq * L * 4);
+        o_qc = take(nq * L);
+    }
+    const uint64_t words = bitmap_words_for(h->n);
+    const size_t bm_bytes = ((size_t)words * 4 + 255) & ~(size_t)255, log_bytes = (size_t)kNoAllocLogCap * 4;
+    if (off + bm_bytes + log_bytes > kNoAllocBytes)
+        return fail(PANNS_ERR_ARG, "this batch needs " + std::to_string(off + bm_bytes + log_bytes) +
+                                       " bytes of device scratch; the allocation-free search has " +
+                                       std::to_string(kNoAllocBytes) +
+                                       " (use fewer queries, a smaller K or L, or no queue outputs)");
+    pl.grid = std::min<uint32_t>(pl.grid, (uint32_t)((kNoAllocBytes - off) / (bm_bytes + log_bytes)));
+    const size_t o_bm = take((size_t)pl.grid * bm_bytes), o_log = take((size_t)pl.grid * log_bytes);
+    hipStream_t st = h->stream;
+    HIP_TRY(hipMemcpyAsync(ptr + o_q, query_ids, (size_t)nq * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ptr + o_init, init_ids, L * 4, hipMemcpyHostToDevice, st));
+    // the bitmaps start empty (the kernel leaves them so after every query,
+    // but an earlier call's layout may have put other data there)
+    const uint64_t n16 = (uint64_t)pl.grid * bm_bytes / 16;
+    hipLaunchKernelGGL(panns::zero_kernel, dim3((uint32_t)std::min<uint64_t>((n16 + 255) / 256, 2048)), dim3(256), 0,
+                       st, reinterpret_cast<uint4 *>(ptr + o_bm), n16);
+    HIP_TRY(hipGetLastError());
+    PlanBufs b;
+    b.qids = reinterpret_cast<const uint32_t *>(ptr + o_q);
+    b.init_ids = reinterpret_cast<const uint32_t *>(ptr + o_init);
+    b.work_counter = reinterpret_cast<uint32_t *>(ptr + o_cnt);
+    b.out_ids = reinterpret_cast<uint32_t *>(ptr + o_ids);
+    b.out_dists = reinterpret_cast<float *>(ptr + o_d);
+    b.out_stats = reinterpret_cast<uint32_t *>(ptr + o_st);
+    if (pl.want_queue) {
+        b.out_qids = reinterpret_cast<uint32_t *>(ptr + o_qi);
+        b.out_qdists = reinterpret_cast<float *>(ptr + o_qd);
+        b.out_qchk = ptr + o_qc;
+    }
+    b.bitmaps = reinterpret_cast<uint32_t *>(ptr + o_bm);
+    b.bitmap_words = bm_bytes / 4;
+    b.spill_log = reinterpret_cast<uint32_t *>(ptr + o_log);
+    b.log_cap = kNoAllocLogCap;
+    fill_args(&pl, geo, b);
+    hipLaunchKernelGGL(panns::reset_counter_kernel, dim3(1), dim3(64), 0, st, b.work_counter, kRanges);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(h->ev0, st));
+    HIP_TRY(launch_one(&pl, st));
+    HIP_TRY(hipEventRecord(h->ev1, st));
+    HIP_TRY(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    // results in the caller's order (no locality reordering here)
+    HIP_TRY(hipMemcpy(out->ids, ptr + o_ids, nq * K * 4, hipMemcpyDeviceToHost));
+    if (out->dists) HIP_TRY(hipMemcpy(out->dists, ptr + o_d, nq * K * 4, hipMemcpyDeviceToHost));
+    if (out->queue_ids) HIP_TRY(hipMemcpy(out->queue_ids, ptr + o_qi, nq * L * 4, hipMemcpyDeviceToHost));
+    if (out->queue_dists) HIP_TRY(hipMemcpy(out->queue_dists, ptr + o_qd, nq * L * 4, hipMemcpyDeviceToHost));
+    if (out->queue_checked) HIP_TRY(hipMemcpy(out->queue_checked, ptr + o_qc, nq * L, hipMemcpyDeviceToHost));
+    // per-query stats through a stack buffer, a chunk of queries at a time
+    uint32_t stats[256 * 5];
+    for (uint32_t q0 = 0; q0 < nq; q0 += 256) {
+        const uint32_t n = std::min<uint32_t>(256u, nq - q0);
+        HIP_TRY(hipMemcpy(stats, ptr + o_st + (size_t)q0 * 20, (size_t)n * 20, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) {
+            if (out->ndist) out->ndist[q0 + i] = stats[i * 5 + 0];
+            if (out->nexp) out->nexp[q0 + i] = stats[i * 5 + 1];
+            if (out->niter) out->niter[q0 + i] = stats[i * 5 + 2];
+        }
+        if (cnt) add_counters(&pl, stats, n, cnt);
+    }
+    if (cnt) cnt->kernel_ms += ms;
+    return PANNS_OK;
 }
 
 int panns_subsearch_with_top_m(panns_index *h, uint32_t M_max, uint32_t query_id, uint32_t local_L,

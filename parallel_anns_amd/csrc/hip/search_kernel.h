@@ -1308,6 +1308,14 @@ __global__ void nearest_pivot_kernel(const float *__restrict__ queries, const ui
     if (live && slice == 0) out[i] = bi;
 }
 
+// Zeroes n16 16-byte words (a kernel, not a memset, like the counter reset
+// below).
+__global__ void zero_kernel(uint4 *__restrict__ p, uint64_t n16)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 // Zeroes the n work counters of a launch sequence (a kernel, not a memset,
 // so the reset can live inside a captured graph); n <= 64.
 __global__ void reset_counter_kernel(uint32_t *c, uint32_t n)

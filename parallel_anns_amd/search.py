@@ -194,11 +194,13 @@ class Index:
 
     def search(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, want_queue=False,
                num_threads=1, stop="fixed", stop_threshold=0.5, hash_slots=0, waves=1, stop_min_iter=0,
-               stop_every=1, bucket_slots=0):
+               stop_every=1, bucket_slots=0, no_alloc=False):
         """Batch form of para_search_with_top_m_pure_large_graph /
         para_search_PSS_v5_large_graph_dist_thresh_profiling; mode "pss_v5"
         takes M as X and num_threads as the reference's num_threads_ (its
-        T queues run in lockstep in a fixed order)."""
+        T queues run in lockstep in a fixed order).  no_alloc: the
+        allocation-free entry point (panns_search_noalloc), whose device
+        buffers come from the library's fixed scratch array."""
         qids = np.ascontiguousarray(np.atleast_1d(query_ids), dtype=np.uint32)
         nq = qids.shape[0]
         if init_ids is None:
@@ -217,7 +219,8 @@ class Index:
         for f, _ in PannsOutputs._fields_:
             setattr(o, f, ptr(res[f]) if f in res else None)
         c = PannsCounters()
-        _chk(self.lib.panns_search(self.h, p, ptr(qids), nq, ptr(init_ids), init_ids.shape[0], o, c))
+        f = self.lib.panns_search_noalloc if no_alloc else self.lib.panns_search
+        _chk(f(self.h, p, ptr(qids), nq, ptr(init_ids), init_ids.shape[0], o, c))
         res["counters"] = counters_dict(c)
         return res
 
