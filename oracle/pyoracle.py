@@ -92,16 +92,19 @@ class CsrGraph:
         adj = np.asarray(adj, dtype=np.uint32)
         deg = np.asarray(deg, dtype=np.uint32)
         n = adj.shape[0]
+        # vertex v's record [deg, ids...] starts at v + (sum of earlier degrees)
+        lens = deg.astype(np.uint64) + 1
         self.vertex_base = np.zeros(n, dtype=np.uint64)
+        if n:
+            self.vertex_base[1:] = np.cumsum(lens)[:-1]
         total = int(n + deg.sum())
         self.deg_ngbrs = np.empty(total, dtype=np.uint32)
-        off = 0
-        for v in range(n):
-            d = int(deg[v])
-            self.vertex_base[v] = off
-            self.deg_ngbrs[off] = d
-            self.deg_ngbrs[off + 1:off + 1 + d] = adj[v, :d]
-            off += 1 + d
+        self.deg_ngbrs[self.vertex_base.astype(np.int64)] = deg
+        if adj.shape[1]:
+            slot = np.arange(adj.shape[1], dtype=np.uint32)
+            real = slot[None, :] < deg[:, None]
+            pos = (self.vertex_base.astype(np.int64) + 1)[:, None] + slot[None, :].astype(np.int64)
+            self.deg_ngbrs[pos[real]] = adj[real]
         self.norms = None if base is None else norms(base)
         self.g = Graph(n, int(width if width is not None else (deg.max() if n else 0)), int(ep),
                        int(deg.sum()), _p(self.vertex_base), _p(self.deg_ngbrs), _p(self.norms))

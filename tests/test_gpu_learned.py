@@ -95,7 +95,8 @@ def test_in_loop_model_matches_oracle(small_set, gpu_index):
     network (64x64 layer weights rounded to bf16, as the kernel holds them),
     evaluated on the traced features, reaches the threshold; queries with a
     step within 1e-3 of the threshold before that are not judged (the kernel
-    folds the normalisation into the first layer in fp32)."""
+    scales features by a precomputed reciprocal of sd in fp32 and sums in
+    fp32, the oracle divides and sums in fp64)."""
     ds = small_set
     K, L, M = 100, 160, 4
     q = np.arange(ds["queries"].shape[0], dtype=np.uint32)

@@ -1,0 +1,176 @@
+"""GPU parity at the benchmark's own shapes, on the full 1M x 96 workload
+(64-component mixture, graph R <= 50 from exact 100-NN: adjacency stride 64,
+four picks per neighbour group), prepared on the GPU in about ten seconds.
+
+- uniform L in {100, 200, 300, 400} and M in {1, 4, 16, 64}: a 500-query
+  sample bit-exact against the oracle (ids, distance bits, distance counts;
+  full queues and flags on a subsample);
+- the uneven shape of the 10M and 50M workloads on this graph: per-query L
+  log-uniform in [100, 2000] and K in {10, 100} over all 10,000 queries in
+  queue-size buckets (the 10,240-slot tables above L = 512), with 8-slot and
+  forced 4-slot buckets: a sample against the oracle at each query's own L
+  and K, the whole batch through properties that hold at any size;
+- the learned stop on this graph: each query stops where the FP64 oracle of
+  the network says, and its result is the fixed search's state at that
+  iteration (the trace kernel's top-K hash).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+import pyoracle as po
+from parallel_anns_amd import datasets, learned
+from parallel_anns_amd.search import Index
+
+pytestmark = pytest.mark.gpu
+
+CACHE = os.environ.get("PANNS_CACHE", "/tmp/panns_cache")
+
+
+def _case(**kw):
+    print("case", " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    _case(prepare="c1 1M x 96, R<=50")
+    ds = datasets.prepare("c1", cache_dir=CACHE, gpu_knn=datasets.GpuKnn(0), verbose=False)
+    assert ds["adj"].shape[0] == 1_000_000 and int(ds["deg"].max()) <= 50
+    idx = Index(0)
+    idx.set_base(ds["base"])
+    idx.set_queries(ds["queries"])
+    idx.set_graph(ds["adj"], ds["deg"], ds["ep"])
+    ds["index"] = idx
+    ds["graph"] = po.CsrGraph(ds["adj"], ds["deg"], ds["ep"])
+    yield ds
+    idx.close()
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.mark.parametrize("L", [100, 200, 300, 400])
+def test_c1_uniform_L_all_M(c1, L):
+    idx, g = c1["index"], c1["graph"]
+    K = 100
+    q = np.arange(0, 10_000, 20, dtype=np.uint32)  # 500 queries across the batch
+    init = idx.prepare_init_ids(L)
+    assert (init == po.prepare_init_ids(g, L)).all()
+    for M in (1, 4, 16, 64):
+        _case(test="c1_uniform", L=L, M=M)
+        res = idx.search(q, K, L, M, init_ids=init)
+        ids, dists, nd, _ = po.search_batch(g, c1["base"], c1["queries"][q], K, L, M, init, nthreads=16)
+        assert (res["ids"] == ids).all(), (L, M)
+        assert (_bits(res["dists"]) == _bits(dists)).all(), (L, M)
+        assert (res["ndist"] == nd).all(), (L, M)
+        # full queues, checked flags and iteration counts on a subsample
+        sub = q[::50]
+        rq = idx.search(sub, K, L, M, init_ids=init, want_queue=True)
+        for i, qi in enumerate(sub):
+            r = po.search_one(g, c1["base"], c1["queries"][qi], K, L, M, init)
+            assert (rq["queue_ids"][i] == r["queue"]["id"]).all(), (L, M, qi)
+            assert (_bits(rq["queue_dists"][i]) == _bits(r["queue"]["dist"])).all()
+            assert (rq["queue_checked"][i] == r["queue"]["checked"]).all()
+            assert rq["niter"][i] == r["merges"] and rq["nexp"][i] == r["nexp"]
+
+
+@pytest.fixture(scope="module")
+def uneven(c1):
+    Ls, Ks = datasets.uneven_L_K(dict(L_range=(100, 2000), K_choices=(10, 100)), 10_000, seed=11)
+    return Ls, Ks
+
+
+@pytest.mark.parametrize("bucket_slots", [8, 4])
+def test_c1_uneven_L_up_to_2000(c1, uneven, bucket_slots):
+    idx, g = c1["index"], c1["graph"]
+    Ls, Ks = uneven
+    q = np.arange(10_000, dtype=np.uint32)
+    maxL = int(Ls.max())
+    init = idx.prepare_init_ids(maxL)
+    _case(test="c1_uneven", bucket_slots=bucket_slots, maxL=maxL)
+    opt = Index.plan_options(buckets=4, use_graph=True, concurrent=True)
+    pl = idx.plan(q, 100, maxL, 4, init_ids=init, L_per_query=Ls, K_per_query=Ks, options=opt, want_queue=False,
+                  bucket_slots=bucket_slots)
+    bk = pl.buckets()
+    assert any(b["max_L"] > 512 for b in bk)
+    for b in bk:
+        # the 10,240-slot tables above L = 512 (rounded to whole buckets)
+        if b["max_L"] > 512:
+            assert b["hash_slots"] >= 10_240 - 64
+    pl.run()
+    pl.run()
+    res = pl.fetch()
+    pl.close()
+    # a sample against the oracle at each query's own L and K
+    sample = np.concatenate([np.argsort(Ls)[-40:], np.arange(0, 10_000, 100)])
+    for i in sample:
+        L, K = int(Ls[i]), int(Ks[i])
+        r = po.search_one(g, c1["base"], c1["queries"][i], K, L, 4, init[:L])
+        assert (res["ids"][i, :K] == r["ids"]).all(), (i, L, K)
+        assert (_bits(res["dists"][i, :K]) == _bits(r["queue"]["dist"][:K])).all(), i
+        assert res["ndist"][i] == r["ndist"] and res["niter"][i] == r["merges"], i
+        assert (res["ids"][i, K:] == 0xFFFFFFFF).all()
+    # the whole batch: properties that do not need the oracle
+    for i in range(10_000):
+        K = int(Ks[i])
+        row = res["ids"][i, :K]
+        assert np.unique(row).shape[0] == K
+        assert (np.diff(res["dists"][i, :K]) >= 0).all()
+    assert (res["ndist"] >= Ls).all()
+    # the same batch as one plain plan (one launch, automatic shape)
+    one = idx.plan(q, 100, maxL, 4, init_ids=init, L_per_query=Ls, K_per_query=Ks, want_queue=False)
+    one.run()
+    r1 = one.fetch()
+    one.close()
+    for f in ("ids", "ndist", "niter"):
+        assert (r1[f] == res[f]).all(), f
+    assert (_bits(r1["dists"]) == _bits(res["dists"])).all()
+
+
+def _topk_hash(ids):
+    """The trace kernel's order-free hash of a top-K id set."""
+    x = ids.astype(np.uint32) * np.uint32(0x9E3779B1)
+    x ^= x >> np.uint32(15)
+    return (x * np.uint32(0x85EBCA77)).sum(axis=-1, dtype=np.uint32)
+
+
+def test_c1_learned_stop(c1):
+    idx = c1["index"]
+    K, M, L = 100, 4, 289
+    cfg = c1["cfg"]
+    tq = datasets.generate(cfg, "train", n=3000)
+    idx.set_queries(tq)
+    try:
+        feat, hsh, niter, _ = idx.search_trace(np.arange(3000, dtype=np.uint32), K, 400, M, trace_cap=256)
+        X, Y = learned.samples_from_trace(feat, hsh, niter)
+        P, _ = learned.train(idx, X, Y, epochs=3, batch=2048)
+    finally:
+        idx.set_queries(c1["queries"])
+    idx.set_stop_model(P)
+    q = np.arange(0, 10_000, 20, dtype=np.uint32)
+    feat, hsh, niter, _ = idx.search_trace(q, K, L, M, trace_cap=256)
+    assert int(niter.max()) <= 256
+    Pb = np.array(P, np.float64)
+    Pb[576:4672] = mo.bf16(P[576:4672])
+    min_iter, every, thr = 48, 4, 0.5
+    _case(test="c1_learned", L=L, threshold=thr)
+    res = idx.search(q, K, L, M, stop="learned", stop_threshold=thr, stop_min_iter=min_iter, stop_every=every)
+    judged = 0
+    for i in range(q.shape[0]):
+        n = int(niter[i])
+        _, p, _ = mo.forward_backward(Pb[:mo.N_PARAMS], mo.normalise(feat[i, :n], P), np.zeros(n))
+        evals = np.arange(min_iter, n + 1, every)  # 1-based iterations where the model runs
+        fire = [s for s in evals if p[s - 1] >= thr]
+        expect = int(fire[0]) if fire else n
+        if any(abs(p[s - 1] - thr) < 1e-3 for s in evals if s <= expect):
+            continue
+        judged += 1
+        assert int(res["niter"][i]) == expect, i
+        # the result is the fixed search's top-K at that iteration
+        assert _topk_hash(res["ids"][i]) == hsh[i, expect - 1], i
+    assert judged >= 0.9 * q.shape[0]
+    stopped = res["niter"] < niter
+    assert stopped.mean() > 0.2
