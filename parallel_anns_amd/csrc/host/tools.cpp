@@ -138,6 +138,7 @@ struct TopK {
 std::vector<uint64_t> new_bits(uint32_t n) { return std::vector<uint64_t>((n + 63) / 64, 0); }
 inline bool bget(const std::vector<uint64_t> &b, uint32_t i) { return (b[i >> 6] >> (i & 63)) & 1u; }
 inline void bset(std::vector<uint64_t> &b, uint32_t i) { b[i >> 6] |= 1ull << (i & 63); }
+inline void bclr(std::vector<uint64_t> &b, uint32_t i) { b[i >> 6] &= ~(1ull << (i & 63)); }
 
 }  // namespace
 
@@ -674,13 +675,22 @@ extern "C" int pt_build_graph(const float *base, uint32_t n, uint32_t dim, const
         for (uint32_t i = 0; i < deg[v]; ++i) adj[(size_t)v * R + i] = fin[v][i].id;
     }
     fin.clear();
-    // 4: DFS stitching from ep with the degree cap.
-    std::vector<uint64_t> reached = new_bits(n);
+    // 4: DFS stitching from ep with the degree cap (tree_grow,
+    // index_nsg.cpp:638-718, without its rand() and its degree overflow):
+    // the lowest-id vertex not reached yet gets one edge from a reached
+    // vertex with spare degree -- the first such one in its own k-NN list,
+    // else the lowest-id one -- and everything it reaches is reached.
+    // `spare` holds the reached vertices whose degree is below R.
+    std::vector<uint64_t> reached = new_bits(n), spare = new_bits(n);
     std::vector<uint32_t> stack;
+    auto mark = [&](uint32_t w) {
+        bset(reached, w);
+        if (deg[w] < R) bset(spare, w);
+    };
     auto dfs = [&](uint32_t root) {
         stack.clear();
         if (bget(reached, root)) return;
-        bset(reached, root);
+        mark(root);
         stack.push_back(root);
         while (!stack.empty()) {
             const uint32_t u = stack.back();
@@ -688,30 +698,33 @@ extern "C" int pt_build_graph(const float *base, uint32_t n, uint32_t dim, const
             for (uint32_t i = 0; i < deg[u]; ++i) {
                 const uint32_t w = adj[(size_t)u * R + i];
                 if (!bget(reached, w)) {
-                    bset(reached, w);
+                    mark(w);
                     stack.push_back(w);
                 }
             }
         }
     };
     dfs(ep);
-    uint32_t spare_scan = 0;
     for (uint32_t u = 0; u < n; ++u) {
         if (bget(reached, u)) continue;
         uint32_t root = 0xFFFFFFFFu;
         for (uint32_t i = 0; i < knn_k; ++i) {
             const uint32_t c = knn[(size_t)u * knn_k + i];
-            if (c != 0xFFFFFFFFu && bget(reached, c) && deg[c] < R) {
+            if (c != 0xFFFFFFFFu && bget(spare, c)) {
                 root = c;
                 break;
             }
         }
         if (root == 0xFFFFFFFFu) {
-            while (spare_scan < n && !(bget(reached, spare_scan) && deg[spare_scan] < R)) ++spare_scan;
-            if (spare_scan == n) return fail(PT_ERR_RANGE, "no reached vertex has spare degree");
-            root = spare_scan;
+            for (size_t wd = 0; wd < spare.size(); ++wd)
+                if (spare[wd]) {
+                    root = (uint32_t)(wd * 64 + (size_t)__builtin_ctzll(spare[wd]));
+                    break;
+                }
+            if (root == 0xFFFFFFFFu) return fail(PT_ERR_RANGE, "no reached vertex has spare degree");
         }
         adj[(size_t)root * R + deg[root]++] = u;
+        if (deg[root] == R) bclr(spare, root);
         dfs(u);
     }
     return PT_OK;
