@@ -96,20 +96,15 @@ def time_plan(plan, steps, warmup):
 
 
 def find_L(index, qids, K, M, gt, target, lo, hi, **kw):
-    from parallel_anns_amd import tools
+    """The reference's find-L bisection over [lo, hi] (operating_point.find_L);
+    (L, recall), or None when no L in the range reaches the target."""
+    from parallel_anns_amd import operating_point, tools
 
     def rec(L):
         return tools.recall(gt, index.search(qids, K, L, M, **kw)["ids"], L)
 
-    if rec(hi) < target:
-        return None
-    while lo < hi:
-        mid = (lo + hi) // 2
-        if rec(mid) >= target:
-            hi = mid
-        else:
-            lo = mid + 1
-    return lo
+    L, r, _ = operating_point.find_L(rec, lo, hi, target)
+    return (L, r) if r >= target else None
 
 
 def host_cores():
@@ -150,27 +145,31 @@ def cpu_baseline(ds, K, L, M, init_ids, cores, budget_s=12.0):
 
 
 def fixed_operating_point(index, qids, K, gt, target, ms, nq, op=None, hi=1024):
-    """Fastest (M, L) whose recall reaches target: L by bisection per M, each
-    candidate timed over 3 launches.  Returns (M, L, {M: L})."""
+    """Fastest (M, L) whose recall reaches target: L by the reference's
+    bisection per M, each candidate timed over 3 launches, the fastest at or
+    above the target kept (operating_point.fastest_above, the rule of
+    scripts/output_find_runtime_above_presicion.py:40-60).  Returns
+    (M, L, {M: L})."""
+    from parallel_anns_amd import operating_point, tools
     if op:
-        cands = [op]
+        r = tools.recall(gt, index.search(qids, K, op[1], op[0])["ids"], op[1])
+        cands = [(op[0], op[1], r)]
     else:
         cands = []
         for M in ms:
-            L = find_L(index, qids, K, M, gt, target, K, hi)
-            log(f"target {target}: find-L M={M}: L={L}")
-            if L is not None:
-                cands.append((M, L))
-    best = None
-    for M, L in cands:
+            found = find_L(index, qids, K, M, gt, target, K, hi)
+            log(f"target {target}: find-L M={M}: {found}")
+            if found is not None:
+                cands.append((M, found[0], found[1]))
+    rows = []
+    for M, L, r in cands:
         plan = index.plan(qids, K, L, M, want_queue=False)
         wall, _ = time_plan(plan, 3, 1)
         plan.close()
-        qps = 3 * nq / wall
-        log(f"target {target}: fixed candidate M={M} L={L}: {qps:,.0f} QPS")
-        if best is None or qps > best[0]:
-            best = (qps, M, L)
-    return None if best is None else (best[1], best[2], dict(cands))
+        rows.append({"M": M, "L": L, "recall": r, "runtime": wall})
+        log(f"target {target}: fixed candidate M={M} L={L}: {3 * nq / wall:,.0f} QPS")
+    best = operating_point.fastest_above(rows, target if not op else -1.0)
+    return None if best is None else (best["M"], best["L"], {M: L for M, L, _ in cands})
 
 
 def timed(index, qids, K, L, M, gt, steps, warmup, **kw):

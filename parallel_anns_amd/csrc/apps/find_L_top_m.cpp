@@ -10,7 +10,9 @@
 // Set PANNS_BATCH=1 to search all queries of one L in a single batched call
 // instead of one call per query, or PANNS_PSS=1 to call
 // para_search_PSS_v5_large_graph_dist_thresh_profiling per query (M unused;
-// the single-thread PSS_v5 search equals top-M with M = 1).  PANNS_OPT_GRAPH=1
+// the single-thread PSS_v5 search equals top-M with M = 1; with
+// <num_threads> above 1 it runs that many queues, PANNS_PSS_X sub-search
+// steps each, default 1).  PANNS_OPT_GRAPH=1
 // loads the graph with load_nsg_graph (the opt-graph family, distance
 // norm - 2 v.q) and calls that family's para_search_with_top_m_pure
 // (TOPM.cpp:1881-2070) or para_search_PSS_v5_profiling (PSS5.cpp:1211-1462).
@@ -43,7 +45,7 @@ void save_result(const char *filename, const std::vector<std::vector<unsigned>> 
 }
 
 Run search_one_time(PANNS::Searching &engine, unsigned M, unsigned L, unsigned K, bool batch, bool pss, bool opt,
-                    const std::vector<std::vector<PANNS::idi>> &true_nn_list,
+                    unsigned X, const std::vector<std::vector<PANNS::idi>> &true_nn_list,
                     std::vector<std::vector<unsigned>> &set_K_list)
 {
     const unsigned nq = engine.num_queries_;
@@ -65,12 +67,12 @@ Run search_one_time(PANNS::Searching &engine, unsigned M, unsigned L, unsigned K
     } else if (pss && opt) {
         for (unsigned q = 0; q < nq; ++q)
             engine.para_search_PSS_v5_profiling(q, K, L, set_L, init_ids, set_K_list[q], L, local_queues_starts,
-                                                local_queues_sizes, is_visited, 1);
+                                                local_queues_sizes, is_visited, X);
     } else if (pss) {
         for (unsigned q = 0; q < nq; ++q)
             engine.para_search_PSS_v5_large_graph_dist_thresh_profiling(q, K, L, set_L, init_ids, set_K_list[q], L,
                                                                         local_queues_starts, local_queues_sizes,
-                                                                        is_visited, 1);
+                                                                        is_visited, X);
     } else if (opt) {
         for (unsigned q = 0; q < nq; ++q)
             engine.para_search_with_top_m_pure(M, q, K, L, set_L, init_ids, set_K_list[q], L, local_queues_starts,
@@ -120,6 +122,8 @@ int main(int argc, char **argv)
     const bool pss = ps && ps[0] == '1';
     const char *og = getenv("PANNS_OPT_GRAPH");
     const bool opt = og && og[0] == '1';
+    const char *px = getenv("PANNS_PSS_X");
+    const unsigned X = px ? (unsigned)strtoul(px, nullptr, 0) : 1u;
 
     PANNS::Searching engine;
     engine.load_data_load(argv[1]);
@@ -145,7 +149,7 @@ int main(int argc, char **argv)
         bool have_last = false;
         while (L_lower <= L_upper) {
             printf("L: %u L_lower: %u L_upper: %u\n", L, L_lower, L_upper);
-            cur = search_one_time(engine, M, L, K, batch, pss, opt, true_nn_list, set_K_list);
+            cur = search_one_time(engine, M, L, K, batch, pss, opt, X, true_nn_list, set_K_list);
             if (cur.recall100 < P_dest) {
                 L_lower = L + 1;
             } else if (cur.recall100 > P_dest) {

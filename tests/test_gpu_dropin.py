@@ -98,3 +98,77 @@ def test_driver_rejects_mismatched_graph(tiny_set, tmp_path):
            p["truth"], "1", "64", "0.5"]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and "Error:" in r.stderr
+
+
+def test_find_L_bisection_against_a_full_scan(small_set, tmp_path):
+    """The driver's find-L (app/profile_find_L_para_single_query_top_m_search_
+    pure_large_graph.cpp:210-260) over L in [100, 400] on the 20k set: a full
+    oracle scan gives recall@100 at every L (a staircase: plateaus between
+    the L where new components' init ids arrive); for each target the
+    driver reports the L and P@100 that the reference's bisection reaches on
+    that curve -- the smallest L reaching the target where the curve never
+    decreases, an exact hit where the target is a plateau value, and the
+    upper end with its recall where nothing reaches it."""
+    from parallel_anns_amd import operating_point
+    ds = small_set
+    nq, K, M, lo, hi = ds["queries"].shape[0], 100, 4, 100, 400
+    p, gt = _write_inputs(str(tmp_path), ds, nq)
+    g = po.CsrGraph(ds["adj"], ds["deg"], ds["ep"])
+    print("case find-L full scan", lo, hi, flush=True)
+    init = po.prepare_init_ids(g, hi)
+    rec = {}
+    for L in range(lo, hi + 1):
+        ids, _, _, _ = po.search_batch(g, ds["base"], ds["queries"], K, L, M, init[:L], nthreads=16)
+        # the reference's count: hits / (100 * queries), in double
+        hits = sum(len(np.intersect1d(gt[q, :100], ids[q, :100])) for q in range(nq))
+        rec[L] = hits / (100.0 * nq)
+    vals = [rec[L] for L in range(lo, hi + 1)]
+    plateaus = sum(1 for a, b in zip(vals, vals[1:]) if a == b)
+    assert plateaus > (hi - lo) // 2  # mostly flat steps: a staircase
+    monotone = all(b >= a for a, b in zip(vals, vals[1:]))
+    mid = vals[(hi - lo) // 2]
+    targets = [0.5 * (vals[0] + vals[-1]), mid, max(vals) - 1e-4, vals[-1] + 0.01]
+    res_path = os.path.join(str(tmp_path), "result.bin")
+    env = dict(os.environ, PANNS_BATCH="1", PANNS_PSS="0", PANNS_OPT_GRAPH="0")
+    cmd = [DRIVER, p["base"], p["query"], p["graph"], str(lo), str(K), res_path, str(M), p["truth"], "1",
+           str(hi)] + [repr(t) for t in targets]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300, env=env)
+    assert r.returncode == 0, r.stderr
+    finals = [ln for ln in r.stdout.splitlines() if ln.startswith("P_dest")]
+    assert len(finals) == len(targets)
+    for t, line in zip(targets, finals):
+        L_drv = int(re.search(r" L: ([0-9]+)", line).group(1))
+        p_drv = float(re.search(r"P@100: ([0-9.]+)", line).group(1))
+        L_exp, r_exp, _ = operating_point.find_L(rec.__getitem__, lo, hi, t)
+        assert L_drv == L_exp, (t, L_drv, L_exp)
+        assert abs(p_drv - rec[L_drv]) < 2e-6
+        reach = [L for L in range(lo, hi + 1) if rec[L] >= t]
+        if not reach:
+            assert L_drv == hi
+        elif monotone and r_exp != t:
+            assert L_drv == reach[0]
+        else:
+            assert rec[L_drv] >= t
+
+
+@pytest.mark.parametrize("T,X", [(2, 1), (4, 4), (8, 2)])
+def test_driver_pss_v5_worker_queues(tiny_set, tmp_path, T, X):
+    """para_search_PSS_v5_large_graph_dist_thresh_profiling through the
+    drop-in with num_threads_ = T queues and local_queue_capacity = L (the
+    reference driver's setting): ids equal the oracle's lockstep PSS_v5."""
+    ds = tiny_set
+    nq, K, L = 32, 50, 96
+    print("case drop-in pss workers T", T, "X", X, flush=True)
+    p, _ = _write_inputs(str(tmp_path), ds, nq)
+    res_path = os.path.join(str(tmp_path), "result.bin")
+    env = dict(os.environ, PANNS_BATCH="0", PANNS_PSS="1", PANNS_OPT_GRAPH="0", PANNS_PSS_X=str(X))
+    cmd = [DRIVER, p["base"], p["query"], p["graph"], str(L), str(K), res_path, "1", p["truth"], str(T),
+           str(L), "0.5"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stderr
+    got = _read_result(res_path)
+    g = po.CsrGraph(ds["adj"], ds["deg"], ds["ep"])
+    init = po.prepare_init_ids(g, L)
+    for q in range(nq):
+        ref = po.search_pss_v5(g, ds["base"], ds["queries"][q], K, L, X, T, init)
+        assert (got[q] == ref["ids"]).all(), f"query {q}"
