@@ -193,6 +193,50 @@ def timed(index, qids, K, L, M, gt, steps, warmup, **kw):
                        "overflow_slots": int(opt.overflow_slots)}
 
 
+def stop_ceiling(index, qids, K, L, M, gt, target, fixed_ndist, max_iter=None):
+    """What any stop rule could reach at queue length L: the search stopped
+    after exactly s iterations (the learned-stop kernel with threshold 0 and
+    its first evaluation at s) for every s up to the longest query, giving
+    each query's recall and distance count after every iteration.  Two
+    ceilings: every query stopped at the first iteration from which its
+    top-K set no longer changes (the fixed search's recall, no loss), and
+    the per-query iteration choice of least mean distance count at mean
+    recall >= target (Lagrangian relaxation, learned.oracle_bound).  Needs
+    a stop model loaded (its output is ignored at threshold 0)."""
+    from parallel_anns_amd import learned, tools
+    final = index.search(qids, K, L, M)
+    n_iter = final["niter"].astype(np.int64)
+    S = int(n_iter.max()) if max_iter is None else min(int(n_iter.max()), max_iter)
+    fin_sets = np.sort(final["ids"], axis=1)
+    R, N, same = [], [], []
+    for s_ in range(1, S + 1):
+        r = index.search(qids, K, L, M, stop="learned", stop_threshold=0.0, stop_min_iter=s_, stop_every=1)
+        R.append(tools.recall_per_query(gt, r["ids"], L))
+        N.append(r["ndist"].astype(np.float64))
+        same.append((np.sort(r["ids"], axis=1) == fin_sets).all(axis=1))
+    R.append(tools.recall_per_query(gt, final["ids"], L))
+    N.append(final["ndist"].astype(np.float64))
+    Rm, Nm = np.stack(R, 1), np.stack(N, 1)
+    # first iteration of the unbroken run of final top-K sets that lasts to
+    # the end (a set can equal the final one and change again)
+    E = np.stack(same + [np.ones(qids.shape[0], bool)], 1)
+    tail = np.flip(np.cumprod(np.flip(E, 1), 1), 1).astype(bool)
+    first_final = np.argmax(tail, 1) + 1  # 1-based; column S means the final state
+    cols = first_final - 1
+    nd_ff = float(Nm[np.arange(qids.shape[0]), cols].mean())
+    rec_ff = float(Rm[np.arange(qids.shape[0]), cols].mean())
+    b = learned.oracle_bound(Rm, Nm, target)
+    out = {"L": L, "M": M, "iterations_scanned": S,
+           "first_final": {"ndist": round(nd_ff, 1), "recall": round(rec_ff, 4),
+                           "distance_ratio_vs_fixed": round(fixed_ndist / nd_ff, 3),
+                           "mean_first_final_iter": round(float(first_final.mean()), 2),
+                           "mean_iterations": round(float(n_iter.mean()), 2)},
+           "best_iteration_at_target": None if b is None else {
+               "ndist": round(b[0], 1), "recall": round(b[1], 4),
+               "distance_ratio_vs_fixed": round(fixed_ndist / b[0], 3)}}
+    return out
+
+
 def deficit_distribution(d):
     """Per-query recall deficit (fixed search at the same L minus learned
     stop), as quantiles and a histogram."""
@@ -301,7 +345,7 @@ def main():
     log(f"per-query bound: {oracle}")
 
     # ---- learned stop
-    lrn, lres, ldesc, lrn_other = None, None, None, {}
+    lrn, lres, ldesc, lrn_other, ceiling = None, None, None, {}, None
     if not args.no_learned:
         model_path = os.path.join(ds["dir"] or args.cache_dir, "stop_model.npy")
         lM = fM
@@ -376,6 +420,13 @@ def main():
                 picks[t] = best_l[1:]
             index.set_queries(ds["queries"])
         index.set_stop_model(model)
+        # the ceiling of any stop rule at the operating L and at a longer
+        # queue, from the search stopped after every iteration
+        ceiling = []
+        for Lc in sorted({fL, int(round(1.5 * fL))}):
+            c = stop_ceiling(index, qids, K, Lc, lM, gt, target, fixed["ndist"])
+            ceiling.append(c)
+            log(f"stop ceiling: {c}")
         if target in picks:
             lL, thr, skw = picks[target]
             stop = {} if learned.is_fallback(thr) else dict(stop="learned", stop_threshold=thr, **skw)
@@ -463,6 +514,7 @@ def main():
                    "fixed_L": fixed, "learned": lrn,
                    "learned_over_fixed": round(lrn["qps"] / fixed["qps"], 3) if use_lrn else None,
                    "per_query_stop_bound": oracle,
+                   "stop_ceiling": ceiling if not args.no_learned else None,
                    "fixed_L_curve": curve or None,
                    "host_inclusive": host_inclusive,
                    "other_targets": lrn_other or ({str(t): {"fixed_L": v} for t, v in fixed_other.items()}

@@ -778,22 +778,18 @@ __device__ __forceinline__ uint32_t claim_chunks(Visited &vis, uint32_t (&ids)[k
 }
 
 // The per-lane part of the stop model, kept in registers for the kernel's
-// life: lane j's row of W1, its biases, its output weight, and its row of
-// the 64x64 layer as 32 pairs of bf16 (round to nearest even; element 2i in
-// the low half of w2[i]); the feature means and inverse scales are the same
-// for all lanes.
+// life: lane j's row of W1, its biases and its output weight; the feature
+// means and inverse scales are the same for all lanes.  Lane j's row of the
+// 64x64 layer (fp32) is read at each evaluation (16 KB shared by every
+// workgroup, so it stays in the caches); the model runs a few times per
+// query, and holding the row in registers would take 64 of them for the
+// kernel's life.
 struct LaneModel {
     float w1[kFeat];
     float b1, b2, w3, b3;
     float mu[kFeat], isd[kFeat];
-    uint32_t w2[kHid / 2];
+    const float4 *w2row;
 };
-
-__device__ __forceinline__ uint32_t bf16_bits(float x)
-{
-    const uint32_t u = __float_as_uint(x);
-    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
-}
 
 __device__ __forceinline__ LaneModel load_lane_model(const StopModel &m, int lane)
 {
@@ -809,17 +805,11 @@ __device__ __forceinline__ LaneModel load_lane_model(const StopModel &m, int lan
         r.mu[i] = m.mu[i];
         r.isd[i] = 1.0f / m.sd[i];
     }
-    const float2 *row = reinterpret_cast<const float2 *>(m.w2 + lane * kHid);
-#pragma unroll
-    for (int i = 0; i < kHid / 2; ++i) {
-        const float2 v = row[i];
-        r.w2[i] = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
-    }
+    r.w2row = reinterpret_cast<const float4 *>(m.w2 + lane * kHid);
     return r;
 }
 
-// One-wave MLP: lane j computes hidden unit j of both layers, in fp32 with
-// the 64x64 layer's weights in bf16 (as the training kernels use them);
+// One-wave MLP in fp32: lane j computes hidden unit j of both layers;
 // features enter as (f - mu) * (1 / sd) (a multiply per feature instead of
 // an IEEE division), the sums run in index order.  Returns the sigmoid
 // output in every lane.  (Folding the scaling into the first layer would
@@ -827,6 +817,10 @@ __device__ __forceinline__ LaneModel load_lane_model(const StopModel &m, int lan
 __device__ __forceinline__ float mlp_eval(const LaneModel &lm, const float (&f)[kFeat], float *h1s, int lane,
                                           const XorSel &xs)
 {
+    // the 64x64 row's loads go out first and land while layer 1 runs
+    float4 w2[kHid / 4];
+#pragma unroll
+    for (int i = 0; i < kHid / 4; ++i) w2[i] = lm.w2row[i];
     float h = lm.b1;
 #pragma unroll
     for (int i = 0; i < kFeat; ++i) h = __builtin_fmaf(lm.w1[i], (f[i] - lm.mu[i]) * lm.isd[i], h);
@@ -838,11 +832,10 @@ __device__ __forceinline__ float mlp_eval(const LaneModel &lm, const float (&f)[
 #pragma unroll
     for (int i = 0; i < kHid / 4; ++i) {
         const float4 x = h1v[i];
-        const uint32_t p0 = lm.w2[2 * i], p1 = lm.w2[2 * i + 1];
-        g = __builtin_fmaf(__uint_as_float(p0 << 16), x.x, g);
-        g = __builtin_fmaf(__uint_as_float(p0 & 0xFFFF0000u), x.y, g);
-        g = __builtin_fmaf(__uint_as_float(p1 << 16), x.z, g);
-        g = __builtin_fmaf(__uint_as_float(p1 & 0xFFFF0000u), x.w, g);
+        g = __builtin_fmaf(w2[i].x, x.x, g);
+        g = __builtin_fmaf(w2[i].y, x.y, g);
+        g = __builtin_fmaf(w2[i].z, x.z, g);
+        g = __builtin_fmaf(w2[i].w, x.w, g);
     }
     g = g > 0.f ? g : 0.f;
     const float o0 = lm.w3 * g;

@@ -92,8 +92,8 @@ def test_learned_stop_semantics(small_set, gpu_index):
 
 def test_in_loop_model_matches_oracle(small_set, gpu_index):
     """Each query stops at the first iteration where the FP64 oracle of the
-    network (64x64 layer weights rounded to bf16, as the kernel holds them),
-    evaluated on the traced features, reaches the threshold; queries with a
+    network (the kernel evaluates it in fp32), evaluated on the traced
+    features, reaches the threshold; queries with a
     step within 1e-3 of the threshold before that are not judged (the kernel
     scales features by a precomputed reciprocal of sd in fp32 and sums in
     fp32, the oracle divides and sums in fp64)."""
@@ -106,7 +106,6 @@ def test_in_loop_model_matches_oracle(small_set, gpu_index):
     P, _ = learned.train(gpu_index, X, Y, epochs=2, batch=1024)
     gpu_index.set_stop_model(P)
     Pb = np.array(P, np.float64)
-    Pb[576:4672] = mo.bf16(P[576:4672])
     for thr in (0.3, 0.5, 0.7):
         print("case in-loop model threshold", thr, flush=True)
         res = gpu_index.search(q, K, L, M, stop="learned", stop_threshold=thr)

@@ -154,7 +154,6 @@ def test_c1_learned_stop(c1):
     feat, hsh, niter, _ = idx.search_trace(q, K, L, M, trace_cap=256)
     assert int(niter.max()) <= 256
     Pb = np.array(P, np.float64)
-    Pb[576:4672] = mo.bf16(P[576:4672])
     min_iter, every, thr = 48, 4, 0.5
     _case(test="c1_learned", L=L, threshold=thr)
     res = idx.search(q, K, L, M, stop="learned", stop_threshold=thr, stop_min_iter=min_iter, stop_every=every)
