@@ -322,6 +322,18 @@ int panns_exact_knn_ex(panns_index *h, int source, uint32_t k, int method, uint3
 int panns_search_trace(panns_index *h, const panns_params *p, const uint32_t *query_ids,
                        uint32_t nq, const uint32_t *init_ids, uint32_t n_init, uint32_t trace_cap,
                        float *feat, uint32_t *topk_hash, uint32_t *niter, uint32_t *out_ids);
+/* panns_search_trace that also records, after every iteration s, how many
+ * of the first min(K, 100) queue entries are among the first min(gt_k, 100)
+ * true neighbours of the query (recall@100's count, PSS5LG.cpp:309-348):
+ * hits[q*trace_cap + s], and (ndist_trace, nullable) the distance
+ * computations so far.  gt_ids holds gt_k ids per loaded query (indexed
+ * by query id).  A step whose count equals the last step's has reached its
+ * final recall, a label that also covers queries whose top-K keeps changing
+ * without gaining recall. */
+int panns_search_trace_hits(panns_index *h, const panns_params *p, const uint32_t *query_ids, uint32_t nq,
+                            const uint32_t *init_ids, uint32_t n_init, uint32_t trace_cap, const uint32_t *gt_ids,
+                            uint32_t gt_k, float *feat, uint32_t *topk_hash, uint16_t *hits,
+                            uint32_t *ndist_trace, uint32_t *niter, uint32_t *out_ids);
 int panns_set_stop_model(panns_index *h, const float *params, uint32_t n_floats);
 
 /* Training of the stop model on the GPU: BF16 MFMA forward and backward

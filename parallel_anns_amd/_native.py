@@ -110,6 +110,8 @@ AMD_API = {
     "panns_exact_knn_ex": (c_int, [vp, c_int, c_u32, c_int, vp, vp, P(c_u32)]),
     "panns_mfma_f32_probe": (c_int, [vp, vp, vp, vp]),
     "panns_search_trace": (c_int, [vp, P(PannsParams), vp, c_u32, vp, c_u32, c_u32, vp, vp, vp, vp]),
+    "panns_search_trace_hits": (c_int, [vp, P(PannsParams), vp, c_u32, vp, c_u32, c_u32, vp, c_u32, vp, vp, vp,
+                                        vp, vp, vp]),
     "panns_set_stop_model": (c_int, [vp, vp, c_u32]),
     "panns_mlp_train": (c_int, [vp, vp, vp, c_u64, vp, c_u32, c_u32, c_float, c_u64, vp]),
     "panns_mlp_grad": (c_int, [vp, vp, vp, vp, c_u32, vp, P(c_float)]),

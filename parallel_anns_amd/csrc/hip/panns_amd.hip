@@ -174,6 +174,10 @@ struct panns_plan {
     DevBuf<float> trace_feat;
     DevBuf<uint32_t> trace_topk;
     uint32_t trace_cap = 0;
+    DevBuf<uint32_t> trace_gt;   // true neighbours of the loaded queries (trace with hits)
+    DevBuf<uint16_t> trace_hits;
+    DevBuf<uint32_t> trace_ndist;
+    uint32_t trace_gt_k = 0;
     bool want_queue = false;
     uint32_t *shared_bitmap = nullptr;
     uint32_t grid = 0, lds = 0, hash_slots = 0, bucket_slots = 8;
@@ -724,6 +728,10 @@ int build_plan(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq, co
     if (pl->trace_cap) {
         HIP_TRY(pl->trace_feat.ensure((size_t)pl->nq * pl->trace_cap * panns::kFeat));
         HIP_TRY(pl->trace_topk.ensure((size_t)pl->nq * pl->trace_cap));
+        if (pl->trace_gt_k) {
+            HIP_TRY(pl->trace_hits.ensure((size_t)pl->nq * pl->trace_cap));
+            HIP_TRY(pl->trace_ndist.ensure((size_t)pl->nq * pl->trace_cap));
+        }
     }
 #ifdef PANNS_PHASE_TIMERS
     HIP_TRY(pl->phase_cycles.ensure((size_t)pl->grid * panns::kPhases));
@@ -827,6 +835,10 @@ void fill_args(panns_plan *pl, const Geometry &geo, const PlanBufs &b)
     a.trace_feat = pl->trace_cap ? pl->trace_feat.p : nullptr;
     a.trace_topk = pl->trace_cap ? pl->trace_topk.p : nullptr;
     a.trace_cap = pl->trace_cap;
+    a.trace_gt = pl->trace_gt_k ? pl->trace_gt.p : nullptr;
+    a.trace_gt_k = pl->trace_gt_k;
+    a.trace_hits = pl->trace_gt_k ? pl->trace_hits.p : nullptr;
+    a.trace_ndist = pl->trace_gt_k ? pl->trace_ndist.p : nullptr;
     a.phase_cycles = pl->phase_cycles.p;
     pl->input_gen = h->input_gen;
 }
@@ -1850,14 +1862,17 @@ int panns_subsearch_with_top_m(panns_index *h, uint32_t M_max, uint32_t query_id
     return PANNS_OK;
 }
 
-int panns_search_trace(panns_index *h, const panns_params *p, const uint32_t *query_ids, uint32_t nq,
-                       const uint32_t *init_ids, uint32_t n_init, uint32_t trace_cap, float *feat,
-                       uint32_t *topk_hash, uint32_t *niter, uint32_t *out_ids)
+namespace {
+int search_trace(panns_index *h, const panns_params *p, const uint32_t *query_ids, uint32_t nq,
+                 const uint32_t *init_ids, uint32_t n_init, uint32_t trace_cap, const uint32_t *gt_ids, uint32_t gt_k,
+                 float *feat, uint32_t *topk_hash, uint16_t *hits, uint32_t *ndist_trace, uint32_t *niter,
+                 uint32_t *out_ids)
 {
     int rc = check_ready(h);
     if (rc) return rc;
     if (!p || !query_ids || !init_ids || !feat || !topk_hash || !niter || nq == 0 || trace_cap == 0)
         return fail(PANNS_ERR_ARG, "bad trace arguments");
+    if (gt_ids && (gt_k == 0 || !hits)) return fail(PANNS_ERR_ARG, "ground truth needs gt_k > 0 and a hits array");
     if (p->stop != PANNS_STOP_FIXED) return fail(PANNS_ERR_ARG, "traces are taken from the fixed-L search");
     if (!h->nq) return fail(PANNS_ERR_STATE, "queries not loaded");
     HIP_TRY(hipSetDevice(h->device));
@@ -1866,6 +1881,11 @@ int panns_search_trace(panns_index *h, const panns_params *p, const uint32_t *qu
     pl->p = *p;
     pl->nq = nq;
     pl->trace_cap = trace_cap;
+    if (gt_ids) {
+        pl->trace_gt_k = gt_k;
+        HIP_TRY(pl->trace_gt.ensure((size_t)h->nq * gt_k));
+        HIP_TRY(hipMemcpy(pl->trace_gt.p, gt_ids, (size_t)h->nq * gt_k * 4, hipMemcpyHostToDevice));
+    }
     rc = build_plan(pl.get(), query_ids, nullptr, nullptr, init_ids, n_init);
     if (rc) return rc;
     float ms = 0.f;
@@ -1879,7 +1899,29 @@ int panns_search_trace(panns_index *h, const panns_params *p, const uint32_t *qu
     if (rc) return rc;
     HIP_TRY(hipMemcpy(feat, pl->trace_feat.p, (size_t)nq * trace_cap * panns::kFeat * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(topk_hash, pl->trace_topk.p, (size_t)nq * trace_cap * 4, hipMemcpyDeviceToHost));
+    if (gt_ids) HIP_TRY(hipMemcpy(hits, pl->trace_hits.p, (size_t)nq * trace_cap * 2, hipMemcpyDeviceToHost));
+    if (gt_ids && ndist_trace)
+        HIP_TRY(hipMemcpy(ndist_trace, pl->trace_ndist.p, (size_t)nq * trace_cap * 4, hipMemcpyDeviceToHost));
     return PANNS_OK;
+}
+}  // namespace
+
+int panns_search_trace(panns_index *h, const panns_params *p, const uint32_t *query_ids, uint32_t nq,
+                       const uint32_t *init_ids, uint32_t n_init, uint32_t trace_cap, float *feat,
+                       uint32_t *topk_hash, uint32_t *niter, uint32_t *out_ids)
+{
+    return search_trace(h, p, query_ids, nq, init_ids, n_init, trace_cap, nullptr, 0, feat, topk_hash, nullptr,
+                        nullptr, niter, out_ids);
+}
+
+int panns_search_trace_hits(panns_index *h, const panns_params *p, const uint32_t *query_ids, uint32_t nq,
+                            const uint32_t *init_ids, uint32_t n_init, uint32_t trace_cap, const uint32_t *gt_ids,
+                            uint32_t gt_k, float *feat, uint32_t *topk_hash, uint16_t *hits,
+                            uint32_t *ndist_trace, uint32_t *niter, uint32_t *out_ids)
+{
+    if (!gt_ids) return fail(PANNS_ERR_ARG, "null ground truth");
+    return search_trace(h, p, query_ids, nq, init_ids, n_init, trace_cap, gt_ids, gt_k, feat, topk_hash, hits,
+                        ndist_trace, niter, out_ids);
 }
 
 int panns_set_stop_model(panns_index *h, const float *params, uint32_t n_floats)

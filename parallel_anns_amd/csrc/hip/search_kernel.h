@@ -109,6 +109,10 @@ struct KArgs {
     float *trace_feat;      // [n_work][trace_cap][8] or null
     uint32_t *trace_topk;   // [n_work][trace_cap] order-free hash of the top-K id set
     uint32_t trace_cap;
+    const uint32_t *trace_gt;  // [num_queries][trace_gt_k] true neighbours by query id, or null
+    uint32_t trace_gt_k;
+    uint16_t *trace_hits;      // [n_work][trace_cap] top-min(K, 100) ids among the first min(gt_k, 100) true ones
+    uint32_t *trace_ndist;     // [n_work][trace_cap] distance computations so far, or null
     // per-workgroup cycle counts of the search phases (builds with
     // PANNS_PHASE_TIMERS only): [grid][kPhases]
     uint64_t *phase_cycles;
@@ -1184,6 +1188,21 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                     }
                     hsh = wave_sum(hsh, xs);
                     if (lane == 0) a.trace_topk[(size_t)w * a.trace_cap + (niter - 1)] = hsh;
+                    if (a.trace_gt) {
+                        // recall@100's count at this step (get_recall_for_all_queries,
+                        // PSS5LG.cpp:309-348): queue entries j < min(K, 100) found
+                        // among the first min(gt_k, 100) true neighbours
+                        const uint32_t *gq = a.trace_gt + (size_t)qid * a.trace_gt_k;
+                        const uint32_t gk = min(a.trace_gt_k, 100u);
+                        uint32_t hits = 0;
+                        for (uint32_t j = lane; j < K && j < S && j < 100u; j += 64) {
+                            const uint32_t id = (uint32_t)Q[j] & kIdMask;
+                            for (uint32_t t = 0; t < gk; ++t) hits += gq[t] == id ? 1u : 0u;
+                        }
+                        hits = wave_sum(hits, xs);
+                        if (lane == 0) a.trace_hits[(size_t)w * a.trace_cap + (niter - 1)] = (uint16_t)hits;
+                    }
+                    if (a.trace_ndist && lane == 0) a.trace_ndist[(size_t)w * a.trace_cap + (niter - 1)] = ndist;
                 }
                 if constexpr (kLearned) {
                     if (eval_stop) {

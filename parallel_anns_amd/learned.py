@@ -38,17 +38,20 @@ def init_model(seed=0):
     return P
 
 
-def samples_from_trace(feat, topk_hash, niter):
-    """(X [n, 8], Y [n]) with Y = 1 where the step's top-K set is the final one."""
+def samples_from_trace(feat, topk_hash, niter, hits=None):
+    """(X [n, 8], Y [n]) with Y = 1 where the step's top-K set is the final
+    one; with hits (recall@100's count per step, search_trace with gt_ids)
+    Y = 1 where the step's recall already equals the final recall."""
     cap = feat.shape[1]
+    key = topk_hash if hits is None else hits
     X, Y = [], []
     for q in range(feat.shape[0]):
         n = int(min(niter[q], cap))
         if n == 0:
             continue
-        final = topk_hash[q, n - 1]
+        final = key[q, n - 1]
         X.append(feat[q, :n])
-        Y.append((topk_hash[q, :n] == final).astype(np.float32))
+        Y.append((key[q, :n] == final).astype(np.float32))
     return np.concatenate(X), np.concatenate(Y)
 
 
@@ -67,7 +70,8 @@ def train(index, X, Y, epochs=4, batch=4096, lr=2e-3, seed=0):
 
 def first_final_iteration(topk_hash, niter, quantile=0.01):
     """The iteration (1-based) by which `quantile` of the traced queries have
-    reached their final top-K set: the stop model need not run earlier."""
+    reached their final top-K set (or, given per-step recall counts instead
+    of hashes, their final recall): the stop model need not run earlier."""
     firsts = []
     for q in range(topk_hash.shape[0]):
         n = int(min(niter[q], topk_hash.shape[1]))

@@ -283,10 +283,14 @@ class Index:
                                           ptr(init_ids), init_ids.shape[0], sample_stride, best, ms))
         return best, ms.value
 
-    def search_trace(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, trace_cap=256):
+    def search_trace(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, trace_cap=256, gt_ids=None):
         """Fixed-L search that also records the stop features and the top-K
         hash after every iteration; returns (feat [nq, cap, 8], topk_hash
-        [nq, cap], niter [nq], ids [nq, K])."""
+        [nq, cap], niter [nq], ids [nq, K]).  With gt_ids (the loaded
+        queries' true neighbours, one row per query id) it also returns
+        hits [nq, cap], recall@100's count after every iteration, and ndist
+        [nq, cap], the distance computations so far, as fifth and sixth
+        elements."""
         qids = np.ascontiguousarray(np.atleast_1d(query_ids), dtype=np.uint32)
         nq = qids.shape[0]
         if init_ids is None:
@@ -297,9 +301,19 @@ class Index:
         hsh = np.zeros((nq, trace_cap), np.uint32)
         niter = np.zeros(nq, np.uint32)
         ids = np.zeros((nq, K), np.uint32)
-        _chk(self.lib.panns_search_trace(self.h, p, ptr(qids), nq, ptr(init_ids), init_ids.shape[0],
-                                         trace_cap, ptr(feat), ptr(hsh), ptr(niter), ptr(ids)))
-        return feat, hsh, niter, ids
+        if gt_ids is None:
+            _chk(self.lib.panns_search_trace(self.h, p, ptr(qids), nq, ptr(init_ids), init_ids.shape[0],
+                                             trace_cap, ptr(feat), ptr(hsh), ptr(niter), ptr(ids)))
+            return feat, hsh, niter, ids
+        gt = np.ascontiguousarray(gt_ids, dtype=np.uint32)
+        if gt.shape[0] != self.info()["num_queries"]:
+            raise ValueError("gt_ids needs one row per loaded query")
+        hits = np.zeros((nq, trace_cap), np.uint16)
+        nd = np.zeros((nq, trace_cap), np.uint32)
+        _chk(self.lib.panns_search_trace_hits(self.h, p, ptr(qids), nq, ptr(init_ids), init_ids.shape[0],
+                                              trace_cap, ptr(gt), gt.shape[1], ptr(feat), ptr(hsh), ptr(hits),
+                                              ptr(nd), ptr(niter), ptr(ids)))
+        return feat, hsh, niter, ids, hits, nd
 
     def set_stop_model(self, params):
         params = np.ascontiguousarray(params, dtype=np.float32).ravel()

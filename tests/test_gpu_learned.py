@@ -190,3 +190,23 @@ def test_trained_network_gains_over_fixed_l(small_set, gpu_index):
             if f[1] / nd > best[0]:
                 best = (f[1] / nd, thr)
     assert best[0] >= 1.03, best
+
+
+def test_trace_with_hits(small_set, gpu_index):
+    """panns_search_trace_hits: the per-iteration recall count ends at the
+    final result's count against the same truth, and the per-iteration
+    distance count is non-decreasing up to the search's own count; the
+    features and hashes equal the plain trace's."""
+    ds = small_set
+    K, L, M = 100, 160, 4
+    q = np.arange(ds["queries"].shape[0], dtype=np.uint32)
+    gt = ds["gt_ids"]
+    feat, hsh, niter, ids, hits, nd = gpu_index.search_trace(q, K, L, M, trace_cap=256, gt_ids=gt)
+    f0, h0, n0, i0 = gpu_index.search_trace(q, K, L, M, trace_cap=256)
+    assert (feat == f0).all() and (hsh == h0).all() and (niter == n0).all() and (ids == i0).all()
+    fixed = gpu_index.search(q, K, L, M)
+    for i in range(q.shape[0]):
+        n = int(niter[i])
+        assert int(hits[i, n - 1]) == len(np.intersect1d(gt[i, :100], fixed["ids"][i, :100]))
+        assert (np.diff(nd[i, :n].astype(np.int64)) >= 0).all() and int(nd[i, n - 1]) == int(fixed["ndist"][i])
+        assert int(nd[i, 0]) >= L
