@@ -68,8 +68,9 @@ typedef struct {
     float stop_threshold;  /* LEARNED: stop when sigmoid output >= threshold */
     uint32_t hash_slots;   /* 0 = automatic; else a multiple of 64 in [64, 32768]: 16-bit slots of
                               the per-query LDS visited table (raised when the id space needs).
-                              Automatic: 8,192 up to L = 512, 10,240 above; without plan options
-                              and up to L = 400, 7,168 plus 256 overflow entries */
+                              Automatic: 8,192 up to L = 512, 10,240 above, scaled by the launch's
+                              largest L when no plan options give a size (see
+                              panns_plan_options.table_eighths = 0) */
     uint32_t waves;        /* wavefronts per query: 0 or 1, else 2 or 4 (TOP_M and SCALE_M with the
                               fixed stop): the picks of an iteration are expanded by several waves
                               and merged into the master queue, the deterministic form of the
@@ -238,8 +239,11 @@ typedef struct {
     uint32_t n_buckets;                        /* 1..PANNS_MAX_BUCKETS (0 means 1) */
     uint32_t bucket_max_L[PANNS_MAX_BUCKETS];  /* ascending upper bounds of L; all 0: split the
                                                   queries into buckets of equal summed L */
-    uint32_t table_eighths;                    /* visited tables at this many eighths (1-8; 0 means
-                                                  8) of the automatic size, for every bucket */
+    uint32_t table_eighths;                    /* visited tables at this many eighths (1-8) of the
+                                                  automatic size, for every bucket; 0: each bucket's
+                                                  own by its largest L (6/8 up to L = 200, 7/8 to 512,
+                                                  6/8 to 800, 3/8 above; 256 overflow entries up to
+                                                  L = 512, overflow_slots ignored) */
     uint32_t use_graph;                        /* capture and replay as one hipGraph */
     uint32_t concurrent;                       /* buckets side by side */
     uint32_t want_queue;                       /* keep the final queues for panns_plan_fetch */
@@ -250,7 +254,13 @@ typedef struct {
     uint32_t overflow_slots;                   /* LDS overflow entries (full ids) for visited ids whose
                                                   buckets are full, before the global bitmap: 0 or
                                                   a power of two in [8, 4096] */
-    uint32_t reserved[1];
+    uint32_t deferred_merge;                   /* 0: automatic (on for buckets whose largest L is above
+                                                  1,400); 1: merge every iteration's survivors into the
+                                                  queue; 2: always defer.  Deferred: survivors wait in a
+                                                  sorted buffer of 256 keys and are merged into the queue
+                                                  when it fills or a pick could rank outside the top L
+                                                  (fixed stop, one wave, M <= 64, no traces); results are
+                                                  the same */
 } panns_plan_options;
 
 int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *query_ids,
@@ -262,8 +272,9 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
 int panns_plan_describe_buckets(panns_plan *plan, uint32_t *n_buckets, uint32_t *max_L, uint32_t *nq,
                                 uint32_t *workgroups, uint32_t *lds_bytes, uint32_t *hash_slots);
 /* The autotuner: times candidate bucketings (1, 2, 3, 4, 6 buckets of equal
- * summed L; side by side or not; visited tables at 8/8 down to 4/8 of the
- * automatic size, each with 0, 256 or 512 overflow entries; 1, 2 or 4 waves
+ * summed L, each bucket's table by its own largest L; side by side or not;
+ * then one table size for all buckets, 8/8 down to 3/8 of the automatic
+ * size, each with 0, 256 or 512 overflow entries; 1, 2 or 4 waves
  * per query where the search allows it;
  * 96, 64 or 32 neighbour rows per wave and distance step;
  * always as a graph) on every `sample_stride`-th query of the batch (0: a stride that

@@ -60,6 +60,19 @@ struct DevBuf {
     }
 };
 
+// Visited-table shape by a launch's largest L, measured on the 1M and 10M
+// workloads (M = 4, every table size from 3/8 to 8/8 with 0, 256 or 1,024
+// overflow entries, per L range): up to L = 200 the table holds almost every
+// visited id at 6/8 with 256 overflow entries (8 workgroups per CU);
+// 7/8 with 256 up to L = 512; 6/8 up to L = 800; above that the visited
+// set outgrows any table that leaves room for occupancy, and 3/8 (more
+// workgroups, more ids in the global bitmap) is fastest.
+void table_by_L(uint32_t L, uint32_t *eighths, uint32_t *overflow)
+{
+    *eighths = L <= 200 ? 6u : L <= 512 ? 7u : L <= 800 ? 6u : 3u;
+    *overflow = L <= 512 ? 256u : 0u;
+}
+
 uint32_t next_pow2(uint32_t x)
 {
     uint32_t p = 1;
@@ -79,6 +92,8 @@ constexpr uint32_t kLogCap = 16384;
 constexpr uint32_t kRanges = 8;         // work ranges (one per XCD)
 constexpr uint32_t kPivots = 256;       // locality pivots
 constexpr uint32_t kLocalityMin = 512;  // smallest batch that is reordered
+constexpr uint32_t kPendCap = 256;      // pending keys of the deferred merge
+constexpr uint32_t kDeferMinL = 1400;   // automatic deferred merge above this L
 
 }  // namespace
 
@@ -184,6 +199,8 @@ struct panns_plan {
     uint32_t table_eighths = 8;
     uint32_t rows_per_wave = 0;  // neighbour rows per distance step (0: 96, as registers allow)
     uint32_t overflow_slots = 0;  // LDS overflow entries of the visited set (0: none)
+    uint32_t deferred_merge = 0;  // panns_plan_options.deferred_merge
+    uint32_t pend_cap = 0;        // pending-key buffer of the deferred merge (0: none)
     bool shape_given = false;     // launch shape from panns_plan_options (else automatic)
     uint32_t waves = 1;  // wavefronts per query
     bool pss = false;    // PSS_v5 with num_threads > 1 (search_kernel_pss)
@@ -426,7 +443,8 @@ int check_params(panns_index *h, const panns_params *p, uint32_t L, uint32_t K)
 }
 
 // Layout of one workgroup's LDS; must match search_kernel's carving:
-// Q[Lcap] cand[nb_cap] bb[64] (8 B each), query row (4*dim B),
+// Q[Lcap] cand[nb_cap] bb[64] P[pend_cap] (8 B each; the caller adds P's
+// 8 * pend_cap bytes), query row (4*dim B),
 // visited table (2 B per slot), newids[nb_cap] picks[Mcap] counts[4] h1[64]
 // (4 B each).
 uint32_t lds_bytes(uint32_t Lcap, uint32_t nb_cap, uint32_t tbl_slots, uint32_t Mcap, uint32_t dim)
@@ -610,14 +628,14 @@ int plan_geometry(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq,
     // marks an empty slot, bit 15 a second-bucket entry), so nb grows with N
     uint32_t hs = p.hash_slots;
     // automatic size: 8,192 slots up to L = 512, 10,240 above (measured:
-    // larger tables lose more to occupancy than they save in spills).  A
-    // plan built without launch options or a table size takes 7/8 of that
-    // with 256 overflow entries up to L = 400 (one more workgroup per CU; measured on 1M x 96
-    // at L = 100 to 400: 2 to 13 % faster), the full table above.
-    if (!pl->shape_given && !p.hash_slots && pl->maxL <= 400) {
-        pl->table_eighths = 7;
-        pl->overflow_slots = 256;
-    }
+    // larger tables lose more to occupancy than they save in spills), taken
+    // whole or in eighths.  A plan built without launch options or a table
+    // size, or with table_eighths = 0, sizes it by its largest L
+    // (table_by_L).
+    if ((!pl->shape_given || pl->table_eighths == 0) && !p.hash_slots)
+        table_by_L(pl->maxL, &pl->table_eighths, &pl->overflow_slots);
+    else if (pl->table_eighths == 0)
+        pl->table_eighths = 8;
     if (!hs) hs = ((pl->maxL <= 512 ? 8192u : 10240u) * pl->table_eighths / 8u + 63u) & ~63u;
     const uint32_t id_bits = std::max<uint32_t>(1u, log2u(h->n));
     const uint64_t min_nb = (((1ull << id_bits) - 1ull) / 0x7FFEull) + 1ull;
@@ -639,9 +657,15 @@ int plan_geometry(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq,
     const uint32_t Lcap = (pl->maxL + 7u) & ~7u;
     const uint32_t ovf = pl->shared_bitmap ? 0u : pl->overflow_slots;
     // the overflow entries are 32-bit, i.e. two 16-bit slots each in the carving
+    // deferred merge: the fixed-stop one-wave search without traces or a
+    // caller-held queue, at most 64 picks per iteration
+    const bool can_defer = !pl->pss && W == 1 && p.stop == PANNS_STOP_FIXED && !pl->trace_cap && !pl->init_queue.p &&
+                           !pl->shared_bitmap && p.M <= 64;
+    pl->pend_cap = can_defer && (pl->deferred_merge == 2 || (pl->deferred_merge == 0 && pl->maxL > kDeferMinL))
+                       ? kPendCap : 0u;
     pl->lds = pl->pss ? lds_bytes_pss(Lcap, nb_cap, hs + 2u * ovf, h->dim, p.num_threads)
               : W > 1 ? lds_bytes_mw(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim, W)
-                      : lds_bytes(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim);
+                      : lds_bytes(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim) + 8u * pl->pend_cap;
     if (pl->lds > 65536) {
         HIP_TRY(hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->lds));
     }
@@ -815,6 +839,7 @@ void fill_args(panns_plan *pl, const Geometry &geo, const PlanBufs &b)
     a.group_picks = std::max<uint32_t>(1u, geo.nb_cap >> h->stride_log2);
     a.group_rows = pl->rows_per_wave ? pl->rows_per_wave : 96u;
     a.num_threads = p.num_threads;
+    a.pend_cap = pl->pend_cap;
     a.init_queue = pl->init_queue.p;
     a.shared_bitmap = pl->shared_bitmap;
     a.stop_learned = p.stop == PANNS_STOP_LEARNED ? 1u : 0u;
@@ -1368,13 +1393,13 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
     if (opt) o = *opt;
     if (o.n_buckets == 0) o.n_buckets = 1;
     if (o.n_buckets > PANNS_MAX_BUCKETS) return fail(PANNS_ERR_ARG, "at most 8 buckets");
-    if (o.table_eighths == 0) o.table_eighths = 8;
-    if (o.table_eighths > 8) return fail(PANNS_ERR_ARG, "table_eighths must be 1..8");
+    if (o.table_eighths > 8) return fail(PANNS_ERR_ARG, "table_eighths must be 0..8");
     if (o.rows_per_wave && o.rows_per_wave != 32 && o.rows_per_wave != 64 && o.rows_per_wave != 96)
         return fail(PANNS_ERR_ARG, "rows_per_wave must be 0, 32, 64 or 96");
     if (o.overflow_slots &&
         ((o.overflow_slots & (o.overflow_slots - 1u)) || o.overflow_slots < 8u || o.overflow_slots > 4096u))
         return fail(PANNS_ERR_ARG, "overflow_slots must be 0 or a power of two in [8, 4096]");
+    if (o.deferred_merge > 2) return fail(PANNS_ERR_ARG, "deferred_merge must be 0, 1 or 2");
     HIP_TRY(hipSetDevice(h->device));
     std::unique_ptr<panns_plan> pl(new panns_plan());
     pl->h = h;
@@ -1387,6 +1412,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
     pl->shape_given = opt != nullptr;
     pl->rows_per_wave = o.rows_per_wave;
     pl->overflow_slots = o.overflow_slots;
+    pl->deferred_merge = o.deferred_merge;
     if (o.n_buckets == 1 && !o.use_graph && !o.concurrent) {
         rc = build_plan(pl.get(), query_ids, L_per_query, K_per_query, init_ids, n_init);
         if (rc) return rc;
@@ -1453,6 +1479,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
         part->table_eighths = o.table_eighths;
         part->rows_per_wave = o.rows_per_wave;
         part->overflow_slots = o.overflow_slots;
+        part->deferred_merge = o.deferred_merge;
         part->shape_given = pl->shape_given;
         part->slot0 = o.concurrent ? slot : 0u;
         part->counter_idx = counter++;
@@ -1571,7 +1598,7 @@ int panns_plan_autotune(panns_index *h, const panns_params *p, const uint32_t *q
         if (nb > 1 && k < 2) continue;
         o.n_buckets = k;
         o.use_graph = 1;
-        o.table_eighths = 8;
+        o.table_eighths = 0;  // each bucket's table by its own largest L
         o.waves = 1;
         for (uint32_t conc = 0; conc < (k > 1 ? 2u : 1u); ++conc) {
             o.concurrent = conc;
@@ -1579,12 +1606,12 @@ int panns_plan_autotune(panns_index *h, const panns_params *p, const uint32_t *q
             if (rc) return rc;
         }
     }
-    // stage 2: smaller visited tables (more workgroups per CU, more ids
-    // past their buckets), with or without an LDS overflow area for those
+    // stage 2: one table size for every bucket (more workgroups per CU and
+    // more ids past their buckets as it shrinks), with or without an LDS
+    // overflow area for those
     const panns_plan_options s1 = *best;
-    for (uint32_t eighths : {8u, 7u, 6u, 5u, 4u}) {
+    for (uint32_t eighths : {8u, 7u, 6u, 5u, 4u, 3u}) {
         for (uint32_t ovf : {0u, 256u, 512u}) {
-            if (eighths == 8u && ovf == 0u) continue;  // stage 1's shape
             panns_plan_options o = s1;
             o.table_eighths = eighths;
             o.overflow_slots = ovf;

@@ -97,6 +97,7 @@ struct KArgs {
     uint32_t Mcap, nb_cap, group_picks;
     uint32_t group_rows;  // neighbour rows a wave gathers per distance step: 32, 64 or 96
     uint32_t num_threads; // PSS_v5 queues (the reference's num_threads_), search_kernel_pss only
+    uint32_t pend_cap;    // pending-key buffer of the deferred merge (multiple of 64, <= 256), 0: merge every step
     // subsearch on a caller-held queue and visited bitmap
     const uint64_t *init_queue;  // [n_work][L] keys, or null
     uint32_t *shared_bitmap;     // used instead of bitmaps when non-null; never cleared
@@ -512,6 +513,62 @@ __device__ __forceinline__ uint32_t merge_batch(uint64_t *Q, uint32_t &S, uint32
     if (valid) Q[pos] = x;
     S = min(L, S + nv);
     merge_sync<kWave>();
+    return p0;
+}
+
+// Merges the sorted run R[0, nr) (nr <= 256; keys may carry checked flags)
+// into Q[0, S), capacity L: Q becomes the L smallest of both, flags kept.
+// Run key i lands at i + (old keys below it), old key j moves right by
+// (run keys below it); both are computed on the queue before it changes.
+// Returns the queue position of the run's first key, or kSent when no run
+// key lands.
+__device__ __forceinline__ uint32_t merge_run(uint64_t *Q, uint32_t &S, uint32_t L, const uint64_t *R, uint32_t nr,
+                                              int lane)
+{
+    if (nr == 0) return kSent;
+    uint32_t rp[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t i = u * 64u + (uint32_t)lane;
+        rp[u] = i < nr ? lb_queue(Q, S, km(R[i])) + i : kSent;
+    }
+    const uint32_t p0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)rp[0]);
+    if (p0 >= L) return kSent;
+    __syncthreads();
+    uint32_t hi = S;
+    while (hi > p0) {
+        const uint32_t lo = (hi - p0 > 64u) ? hi - 64u : p0;
+        const uint32_t j = lo + (uint32_t)lane;
+        uint64_t e = 0;
+        uint32_t np = kSent;
+        if (j < hi) {
+            e = Q[j];
+            const uint64_t ek = km(e);
+            uint32_t c = 0, cnt = nr;
+            while (cnt > 0) {
+                const uint32_t step = cnt >> 1;
+                const uint32_t mid = c + step;
+                if (km(R[mid]) < ek) {
+                    c = mid + 1;
+                    cnt -= step + 1;
+                } else {
+                    cnt = step;
+                }
+            }
+            np = j + c;
+        }
+        __syncthreads();
+        if (np < L) Q[np] = e;
+        __syncthreads();
+        hi = lo;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t i = u * 64u + (uint32_t)lane;
+        if (rp[u] < L) Q[rp[u]] = R[i];
+    }
+    S = min(L, S + nr);
+    __syncthreads();
     return p0;
 }
 
@@ -957,7 +1014,10 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
     uint64_t *Q = smem;
     uint64_t *cand = Q + a.Lcap;
     uint64_t *bb = cand + a.nb_cap;
-    float4 *qs = reinterpret_cast<float4 *>(bb + 64);
+    // deferred merge (fixed-stop search without traces): survivors wait in
+    // P, sorted, and are merged into Q when P fills or a pick needs it
+    uint64_t *P = bb + 64;
+    float4 *qs = reinterpret_cast<float4 *>(P + a.pend_cap);
     uint32_t *tbl = reinterpret_cast<uint32_t *>(qs + D / 4);
     uint32_t *newids = tbl + a.tbl_words;
     uint32_t *picks = newids + a.nb_cap;
@@ -1055,9 +1115,100 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
 
         uint32_t k = 0;
         uint32_t Mcur = (a.mode == kModeScaleM) ? 1u : (a.mode == kModePssV5 ? 1u : a.M);
+        // deferred merge: the fixed-stop search without traces, picks of at
+        // most 64, a full queue (the stop features need the exact queue)
+        const bool defer = !kLearned && !kTrace && a.pend_cap != 0 && S == L;
+        uint32_t SP = 0;
         for (;;) {
-            // pick_top_m_unchecked (TOPMLG.cpp:1172-1198)
             uint32_t m = 0, last_k = 0, pos = k;
+            uint32_t nk = kSent, k_next = 0;
+            bool picked = false;
+            if (defer && SP > 0) {
+                // The exact queue is the L smallest of Q and P.  Its first
+                // Mcur unchecked entries are the Mcur smallest of Q's first
+                // Mcur unchecked ones (from k) and P's, provided those Q
+                // entries rank below L in the union (position + SP < L);
+                // otherwise P is merged first and the plain pick runs.
+                uint64_t *ck = cand, *pk = cand + 64;
+                uint32_t *cp = reinterpret_cast<uint32_t *>(cand + 128), *pi = cp + 64;
+                uint32_t mq = 0, q_last = 0;
+                while (mq < Mcur && pos < S) {
+                    const uint32_t j = pos + (uint32_t)lane;
+                    const uint64_t e = j < S ? Q[j] : kChk;
+                    const bool unc = j < S && !(e & kChk);
+                    const uint64_t msk = __ballot(unc);
+                    const uint32_t rank = (uint32_t)__popcll(msk & lt);
+                    const bool take = unc && rank < Mcur - mq;
+                    if (take) {
+                        ck[mq + rank] = e;
+                        cp[mq + rank] = j;
+                    }
+                    const uint64_t tm = __ballot(take);
+                    if (tm) {
+                        q_last = pos + 63u - (uint32_t)__clzll(tm);
+                        mq += (uint32_t)__popcll(tm);
+                    }
+                    pos += 64;
+                }
+                if (mq == Mcur && q_last + SP < L) {
+                    uint32_t mp = 0;
+                    for (uint32_t p = 0; mp < Mcur && p < SP; p += 64) {
+                        const uint32_t j = p + (uint32_t)lane;
+                        const uint64_t e = j < SP ? P[j] : kChk;
+                        const bool unc = j < SP && !(e & kChk);
+                        const uint64_t msk = __ballot(unc);
+                        const uint32_t rank = (uint32_t)__popcll(msk & lt);
+                        const bool take = unc && rank < Mcur - mp;
+                        if (take) {
+                            pk[mp + rank] = e;
+                            pi[mp + rank] = j;
+                        }
+                        mp += (uint32_t)__popcll(__ballot(take));
+                    }
+                    __syncthreads();
+                    // each candidate's rank among both lists (they are
+                    // sorted, and no key is in both)
+                    bool tq = false, tp = false;
+                    uint32_t rq = kSent, rpk = kSent;
+                    if ((uint32_t)lane < mq) {
+                        const uint64_t key = km(ck[lane]);
+                        uint32_t r = 0;
+                        for (uint32_t t = 0; t < mp; ++t) r += km(pk[t]) < key ? 1u : 0u;
+                        rq = (uint32_t)lane + r;
+                        tq = rq < Mcur;
+                    }
+                    if ((uint32_t)lane < mp) {
+                        const uint64_t key = km(pk[lane]);
+                        uint32_t r = 0;
+                        for (uint32_t t = 0; t < mq; ++t) r += km(ck[t]) < key ? 1u : 0u;
+                        rpk = (uint32_t)lane + r;
+                        tp = rpk < Mcur;
+                    }
+                    __syncthreads();
+                    if (tq) {
+                        Q[cp[lane]] |= kChk;
+                        picks[rq] = (uint32_t)ck[lane] & kIdMask;
+                    }
+                    if (tp) {
+                        P[pi[lane]] |= kChk;
+                        picks[rpk] = (uint32_t)pk[lane] & kIdMask;
+                    }
+                    // Q's picks are a prefix of its unchecked entries: the
+                    // scan restarts after the last of them (or where it was)
+                    const uint64_t qm = __ballot(tq);
+                    k_next = qm ? cp[63 - __clzll(qm)] + 1u : k;
+                    m = Mcur;
+                    picked = true;
+                } else {
+                    const uint32_t p0 = merge_run(Q, S, L, P, SP, lane);
+                    SP = 0;
+                    if (p0 != kSent) k = min(k, p0);
+                    pos = k;
+                }
+                __syncthreads();
+            }
+            if (!picked) {
+            // pick_top_m_unchecked (TOPMLG.cpp:1172-1198)
             while (m < Mcur && pos < S) {
                 const uint32_t j = pos + (uint32_t)lane;
                 const uint64_t e = j < S ? Q[j] : kChk;
@@ -1077,12 +1228,12 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                 }
                 pos += 64;
             }
+            }
             __syncthreads();
             PANNS_PT(kPhPick);
             if (m == 0) break;
             ++niter;
             nexp += m;
-            uint32_t nk = kSent;
             uint32_t ins = 0;
             for (uint32_t g0 = 0; g0 < m; g0 += a.group_picks) {
                 const uint32_t gs = min(a.group_picks, m - g0);
@@ -1148,12 +1299,24 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                     const uint32_t cnt = min(64u, ncand - b0);
                     uint64_t x = (uint32_t)lane < cnt ? cand[b0 + lane] : kKeyMax;
                     x = sort64_first(x, cnt, lane, xs);
-                    const uint32_t p0 = merge_batch(Q, S, L, x, cnt, bb, lane);
-                    if (p0 != kSent) nk = min(nk, p0);
+                    if (defer) {
+                        // into P, which is merged into Q first when full
+                        if (SP + cnt > a.pend_cap) {
+                            const uint32_t p0 = merge_run(Q, S, L, P, SP, lane);
+                            SP = 0;
+                            if (p0 != kSent) nk = min(nk, p0);
+                        }
+                        merge_batch(P, SP, a.pend_cap, x, cnt, bb, lane);
+                    } else {
+                        const uint32_t p0 = merge_batch(Q, S, L, x, cnt, bb, lane);
+                        if (p0 != kSent) nk = min(nk, p0);
+                    }
                 }
                 PANNS_PT(kPhMerge);
             }
-            k = (nk <= last_k) ? nk : last_k + 1u;
+            // k = nk <= last_k ? nk : last_k + 1 (TOPMLG.cpp:2168-2172)
+            if (!picked) k_next = last_k + 1u;
+            k = min(nk, k_next);
             ins_last = ins;
             if (a.mode == kModeScaleM && Mcur < a.M) Mcur <<= 1;
 
