@@ -220,8 +220,8 @@ int panns_plan_run(panns_plan *plan, int sync, float *kernel_ms);
  * plan's max K: row q holds K_q valid ids followed by 0xFFFFFFFF. */
 int panns_plan_fetch(panns_plan *plan, panns_outputs *out, panns_counters *cnt);
 int panns_plan_destroy(panns_plan *plan);
-/* Launch geometry chosen for the plan: persistent workgroups and the LDS
- * bytes each one uses. */
+/* Launch geometry chosen for the plan: persistent workgroups, the LDS
+ * bytes each one uses and its visited-table slots (0: no table). */
 int panns_plan_describe(panns_plan *plan, uint32_t *workgroups, uint32_t *lds_bytes,
                         uint32_t *hash_slots, uint32_t *max_L);
 
@@ -261,6 +261,11 @@ typedef struct {
                                                   when it fills or a pick could rank outside the top L
                                                   (fixed stop, one wave, M <= 64, no traces); results are
                                                   the same */
+    uint32_t visited_table;                    /* 0: automatic (none for one-wave buckets above L = 800
+                                                  whose id space needs half again the buckets of the
+                                                  planned table, e.g. 50M ids); 1: always a table;
+                                                  2: none, every visited id claimed in the bitmap.
+                                                  Results are the same */
 } panns_plan_options;
 
 int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *query_ids,
@@ -268,7 +273,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
                          const uint32_t *init_ids, uint32_t n_init, const panns_plan_options *opt,
                          panns_plan **out);
 /* Per bucket b < *n_buckets: largest L, queries, workgroups, LDS bytes,
- * visited-table slots (arrays of PANNS_MAX_BUCKETS, each nullable). */
+ * visited-table slots, 0 for none (arrays of PANNS_MAX_BUCKETS, each nullable). */
 int panns_plan_describe_buckets(panns_plan *plan, uint32_t *n_buckets, uint32_t *max_L, uint32_t *nq,
                                 uint32_t *workgroups, uint32_t *lds_bytes, uint32_t *hash_slots);
 /* The autotuner: times candidate bucketings (1, 2, 3, 4, 6 buckets of equal

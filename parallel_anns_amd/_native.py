@@ -48,7 +48,8 @@ MAX_BUCKETS = 8
 class PannsPlanOptions(ctypes.Structure):
     _fields_ = [("n_buckets", c_u32), ("bucket_max_L", c_u32 * MAX_BUCKETS), ("table_eighths", c_u32),
                 ("use_graph", c_u32), ("concurrent", c_u32), ("want_queue", c_u32), ("waves", c_u32),
-                ("rows_per_wave", c_u32), ("overflow_slots", c_u32), ("deferred_merge", c_u32)]
+                ("rows_per_wave", c_u32), ("overflow_slots", c_u32), ("deferred_merge", c_u32),
+                ("visited_table", c_u32)]
 
 
 # name -> (restype, argtypes) for include/panns_tools.h

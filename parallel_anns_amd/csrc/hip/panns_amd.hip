@@ -88,7 +88,7 @@ uint32_t log2u(uint32_t x)
 
 constexpr uint32_t kMaxL = 4096;
 constexpr uint32_t kMaxM = 1024;
-constexpr uint32_t kLogCap = 16384;
+constexpr uint32_t kLogCap = 32768;
 constexpr uint32_t kRanges = 8;         // work ranges (one per XCD)
 constexpr uint32_t kPivots = 256;       // locality pivots
 constexpr uint32_t kLocalityMin = 512;  // smallest batch that is reordered
@@ -200,6 +200,8 @@ struct panns_plan {
     uint32_t rows_per_wave = 0;  // neighbour rows per distance step (0: 96, as registers allow)
     uint32_t overflow_slots = 0;  // LDS overflow entries of the visited set (0: none)
     uint32_t deferred_merge = 0;  // panns_plan_options.deferred_merge
+    uint32_t visited_table = 0;   // panns_plan_options.visited_table
+    bool no_table = false;        // every visited id claimed in the bitmap
     uint32_t pend_cap = 0;        // pending-key buffer of the deferred merge (0: none)
     bool shape_given = false;     // launch shape from panns_plan_options (else automatic)
     uint32_t waves = 1;  // wavefronts per query
@@ -632,11 +634,13 @@ int plan_geometry(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq,
     // whole or in eighths.  A plan built without launch options or a table
     // size, or with table_eighths = 0, sizes it by its largest L
     // (table_by_L).
-    if ((!pl->shape_given || pl->table_eighths == 0) && !p.hash_slots)
+    const bool auto_size = (!pl->shape_given || pl->table_eighths == 0) && !p.hash_slots;
+    if (auto_size)
         table_by_L(pl->maxL, &pl->table_eighths, &pl->overflow_slots);
     else if (pl->table_eighths == 0)
         pl->table_eighths = 8;
     if (!hs) hs = ((pl->maxL <= 512 ? 8192u : 10240u) * pl->table_eighths / 8u + 63u) & ~63u;
+    const uint32_t planned = hs;
     const uint32_t id_bits = std::max<uint32_t>(1u, log2u(h->n));
     const uint64_t min_nb = (((1ull << id_bits) - 1ull) / 0x7FFEull) + 1ull;
     // buckets of 8 slots, or of 4 when an 8-slot table of this size has
@@ -646,6 +650,21 @@ int plan_geometry(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq,
     uint32_t nb = std::max<uint32_t>(hs / bslots, (uint32_t)((min_nb + 7ull) & ~7ull));
     nb = std::max<uint32_t>(nb, 8u);
     hs = nb * bslots;
+    // No table: every visited id is claimed in the bitmap.  Automatic for a
+    // one-wave plan above L = 800 whose id space needs half again the
+    // buckets of its planned table (16-bit tags hold 15 bits of the
+    // scrambled id, so 50M ids need 8,224 slots where 3,840 were planned):
+    // measured on 50M x 96, such a table costs more in occupancy than it
+    // saves in bitmap claims (no table: 8 % faster over L 100-2000).
+    pl->no_table = !pl->pss && !pl->shared_bitmap &&
+                   (pl->visited_table == 2 ||
+                    (pl->visited_table == 0 && auto_size && W == 1 && pl->maxL > 800 && hs > planned + planned / 2));
+    if (pl->no_table) {
+        hs = 64;
+        nb = 8;
+        bslots = 8;
+        pl->overflow_slots = 0;
+    }
     if (pl->shared_bitmap) {
         hs = 64;
         nb = 8;
@@ -833,7 +852,7 @@ void fill_args(panns_plan *pl, const Geometry &geo, const PlanBufs &b)
     a.tbl_bshift = geo.bslots == 8u ? 2u : 1u;
     a.tbl_magic = (uint32_t)((1ull << 32) / geo.nb);
     a.id_bits = geo.id_bits;
-    a.use_table = pl->shared_bitmap ? 0u : 1u;
+    a.use_table = pl->shared_bitmap || pl->no_table ? 0u : 1u;
     a.Mcap = geo.Mcap;
     a.nb_cap = geo.nb_cap;
     a.group_picks = std::max<uint32_t>(1u, geo.nb_cap >> h->stride_log2);
@@ -1400,6 +1419,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
         ((o.overflow_slots & (o.overflow_slots - 1u)) || o.overflow_slots < 8u || o.overflow_slots > 4096u))
         return fail(PANNS_ERR_ARG, "overflow_slots must be 0 or a power of two in [8, 4096]");
     if (o.deferred_merge > 2) return fail(PANNS_ERR_ARG, "deferred_merge must be 0, 1 or 2");
+    if (o.visited_table > 2) return fail(PANNS_ERR_ARG, "visited_table must be 0, 1 or 2");
     HIP_TRY(hipSetDevice(h->device));
     std::unique_ptr<panns_plan> pl(new panns_plan());
     pl->h = h;
@@ -1413,6 +1433,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
     pl->rows_per_wave = o.rows_per_wave;
     pl->overflow_slots = o.overflow_slots;
     pl->deferred_merge = o.deferred_merge;
+    pl->visited_table = o.visited_table;
     if (o.n_buckets == 1 && !o.use_graph && !o.concurrent) {
         rc = build_plan(pl.get(), query_ids, L_per_query, K_per_query, init_ids, n_init);
         if (rc) return rc;
@@ -1480,6 +1501,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
         part->rows_per_wave = o.rows_per_wave;
         part->overflow_slots = o.overflow_slots;
         part->deferred_merge = o.deferred_merge;
+        part->visited_table = o.visited_table;
         part->shape_given = pl->shape_given;
         part->slot0 = o.concurrent ? slot : 0u;
         part->counter_idx = counter++;
@@ -1511,7 +1533,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
     for (auto &part : pl->parts) {
         pl->grid += part->grid;
         pl->lds = std::max(pl->lds, part->lds);
-        pl->hash_slots = std::max(pl->hash_slots, part->hash_slots);
+        pl->hash_slots = std::max(pl->hash_slots, part->no_table ? 0u : part->hash_slots);
     }
     *out = pl.release();
     return PANNS_OK;
@@ -1530,7 +1552,7 @@ int panns_plan_describe_buckets(panns_plan *plan, uint32_t *n_buckets, uint32_t 
         if (nq) nq[b] = ps[b]->nq;
         if (workgroups) workgroups[b] = ps[b]->grid;
         if (lds_bytes) lds_bytes[b] = ps[b]->lds;
-        if (hash_slots) hash_slots[b] = ps[b]->hash_slots;
+        if (hash_slots) hash_slots[b] = ps[b]->no_table ? 0u : ps[b]->hash_slots;
     }
     return PANNS_OK;
 }
@@ -1679,7 +1701,7 @@ int panns_plan_describe(panns_plan *plan, uint32_t *workgroups, uint32_t *lds, u
     if (!plan) return fail(PANNS_ERR_ARG, "null plan");
     if (workgroups) *workgroups = plan->grid;
     if (lds) *lds = plan->lds;
-    if (hash_slots) *hash_slots = plan->hash_slots;
+    if (hash_slots) *hash_slots = plan->no_table ? 0u : plan->hash_slots;
     if (max_L) *max_L = plan->maxL;
     return PANNS_OK;
 }

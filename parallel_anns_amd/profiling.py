@@ -3,7 +3,7 @@
   python -m parallel_anns_amd.profiling trace   <results.db> [kernel-substring] [--last N]
   python -m parallel_anns_amd.profiling counter <results.db> <COUNTER> [kernel-substring] [--last N]
   python -m parallel_anns_amd.profiling scale   <copy_fetch.db> <copy_write.db> <bytes>
-  python -m parallel_anns_amd.profiling report  --trace T.db --fetch F.db --write W.db --cal-fetch CF.db
+  python -m parallel_anns_amd.profiling report [uneven] --trace T.db --fetch F.db --write W.db --cal-fetch CF.db
                                                 --cal-write CW.db --copy-bytes B --run run.json
                                                 --command "..." --md out.md --records out.json
 
@@ -14,7 +14,11 @@ steps of a benchmark); `counter` prints a hardware counter per dispatch;
 known size into bytes-per-counted-KiB factors.  `report` combines a kernel
 trace and the two counter passes of a profile_run command (the last `steps`
 dispatches of each search-kernel variant) with the copy-kernel factors into a
-markdown summary and the HBM-traffic records bench.py reports.
+markdown summary and the HBM-traffic records bench.py reports.  `report
+uneven` does the same for a profile_run --uneven command: the last
+`steps` runs of all its buckets' launches, each dispatch matched to its
+bucket by its LDS size and grid (in launch order where those do not tell
+buckets apart).
 """
 import argparse
 import sqlite3
@@ -36,6 +40,115 @@ def counter_values(db, counter, match=None):
     if match:
         rows = [r for r in rows if match in r[1]]
     return rows
+
+
+def _columns(c, view):
+    return [d[0] for d in c.execute(f"select * from {view} limit 1").description]
+
+
+def _dispatches(db, view, match, extra=""):
+    """(duration ns, key, value) per dispatch of kernels matching `match`, in
+    dispatch order; key = (LDS bytes, grid size) where the view has them."""
+    c = sqlite3.connect(db)
+    cols = _columns(c, view)
+    pick = lambda names: next((n for n in names if n in cols), None)
+    lds = pick(("lds_size", "lds_block_size", "group_segment_size"))
+    grid = pick(("grid_size", "grid_size_x"))
+    name = "name" if view == "kernels" else "kernel_name"
+    order = "start" if view == "kernels" else "dispatch_id"
+    key = ", ".join(x for x in (lds, grid) if x) or "0"
+    val = "value" if view != "kernels" else "0"
+    q = f"select {name}, duration, {val}, {key} from {view} {extra} order by {order}"
+    return [(r[1], tuple(r[3:]), r[2]) for r in c.execute(q) if match in r[0]]
+
+
+def _by_run(rows, nb, steps, buckets, lds_col=True):
+    """The last `steps` runs of nb dispatches each, as lists indexed by bucket."""
+    rows = rows[-steps * nb:]
+    assert len(rows) == steps * nb, (len(rows), steps, nb)
+    runs = []
+    for s in range(steps):
+        grp = rows[s * nb:(s + 1) * nb]
+        keys = [g[1] for g in grp]
+        want = [(b["lds_bytes"],) for b in buckets]
+        out = [None] * nb
+        if len(set(keys)) == nb and keys[0] != (0,):
+            # LDS bytes first; the grid (in work-items or workgroups) splits ties
+            free = list(range(nb))
+            for g in grp:
+                cand = [i for i in free if want[i][0] == g[1][0]] or free
+                if len(cand) > 1 and len(g[1]) > 1:
+                    gs = g[1][1]
+                    cand = [i for i in cand if gs in (buckets[i]["workgroups"], 64 * buckets[i]["workgroups"])] or cand
+                out[cand[0]] = g
+                free.remove(cand[0])
+        else:
+            out = grp
+        runs.append(out)
+    return runs
+
+
+def report_uneven(a):
+    import json
+    run = json.loads(open(a.run).read().strip().splitlines()[-1])
+    steps, bk = run["steps"], run["buckets"]
+    nb = len(bk)
+    cf = [r[2] for r in counter_values(a.cal_fetch, "FETCH_SIZE", "copy_kernel")]
+    cw = [r[2] for r in counter_values(a.cal_write, "WRITE_SIZE", "copy_kernel")]
+    ff = a.copy_bytes / (sum(cf) / len(cf) * 1024)
+    wf = a.copy_bytes / (sum(cw) / len(cw) * 1024)
+    match = f"search_kernel<{a.dim}, false, false, false>"
+    tr = _by_run(_dispatches(a.trace, "kernels", match), nb, steps, bk)
+    fe = _by_run(_dispatches(a.fetch, "counters_collection", match, "where counter_name='FETCH_SIZE'"), nb, steps, bk)
+    wr = _by_run(_dispatches(a.write, "counters_collection", match, "where counter_name='WRITE_SIZE'"), nb, steps, bk)
+    o = run["options"]
+    lines = ["# " + a.title, "", "Command (each pass):", "", "```", a.command, "```", "",
+             f"Launch: {nb} queue-size buckets (largest L {o['bucket_max_L']}), "
+             f"{'side by side' if o['concurrent'] else 'one after another'}, one hipGraph per run; "
+             f"visited tables {'by each bucket' + chr(39) + 's largest L' if o['table_eighths'] == 0 else str(o['table_eighths']) + '/8'}, "
+             f"{o['rows_per_wave'] or 96} rows per distance step, {o['overflow_slots']} overflow entries; "
+             f"M = {run['M']}, mean L {run['mean_L']}.", "",
+             f"Copy-kernel calibration ({a.copy_bytes} bytes copied): FETCH_SIZE factor {ff:.4f}, "
+             f"WRITE_SIZE factor {wf:.4f} bytes per counted byte.", "",
+             f"Per bucket, mean of the last {steps} runs (algorithmic bytes: distance rows and results exact, "
+             "adjacency rows at the batch's mean read slots per expansion):", "",
+             "| bucket | largest L | queries | workgroups | LDS B | visited slots | trace mean us | FETCH KiB | "
+             "WRITE KiB | traffic GB | algorithmic GB | traffic / alg. | achieved GB/s |",
+             "|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    mean = lambda xs: sum(xs) / len(xs)
+    tot_us = tot_t = 0.0
+    records = []
+    for i, b in enumerate(bk):
+        us = mean([r[i][0] for r in tr]) / 1e3
+        fk = mean([r[i][2] for r in fe])
+        wk = mean([r[i][2] for r in wr])
+        traffic = fk * 1024 * ff + wk * 1024 * wf
+        alg = b["algorithmic_bytes"]
+        tot_us += us
+        tot_t += traffic
+        lines.append(f"| {i} | {b['max_L']} | {b['nq']:,} | {b['workgroups']:,} | {b['lds_bytes']:,} | "
+                     f"{b['hash_slots']:,} | {us:,.1f} | {fk:,.0f} | {wk:,.0f} | {traffic / 1e9:.3f} | "
+                     f"{alg / 1e9:.3f} | {traffic / alg:.3f} | {alg / (us * 1e-6) / 1e9:,.1f} |")
+    alg = run["algorithmic_bytes_per_run"]
+    lines += ["", f"Whole run: {run['run_ms']} ms device time per run (HIP events around the graph; "
+              f"sum of bucket kernel durations {tot_us / 1e3:,.3f} ms), algorithmic {alg / 1e9:.3f} GB, "
+              f"HBM traffic {tot_t / 1e9:.3f} GB (traffic / alg. {tot_t / alg:.3f}); achieved "
+              f"{alg / (run['run_ms'] * 1e-3) / 1e9:,.1f} GB/s algorithmic = "
+              f"{alg / (run['run_ms'] * 1e-3) / 8e12:.3f} of 8 TB/s.", ""]
+    records.append({"config": run["config"], "kernel": f"search_kernel<{a.dim}, false>", "M": run["M"],
+                    "uneven": True, "buckets": [b["max_L"] for b in bk], "traffic_bytes": int(tot_t),
+                    "algorithmic_bytes": int(alg), "run_ms": run["run_ms"], "fetch_factor": round(ff, 4),
+                    "write_factor": round(wf, 4), "command": a.command})
+    lines += ["Kernel trace, all kernels of the traced run:", "", "```"]
+    rows = kernel_durations(a.trace)
+    for n in sorted({x[0] for x in rows}):
+        dd = [x[1] / 1e3 for x in rows if x[0] == n]
+        lines.append(f"{n[:60]:60s} calls {len(dd):4d} mean {sum(dd) / len(dd):10.1f} us "
+                     f"min {min(dd):10.1f} max {max(dd):10.1f}")
+    lines += ["```", ""]
+    open(a.md, "w").write("\n".join(lines))
+    json.dump({"records": records}, open(a.records, "w"), indent=1)
+    print("\n".join(lines))
 
 
 def report(a):
@@ -94,13 +207,15 @@ def main():
     if len(sys.argv) > 1 and sys.argv[1] == "report":
         ap = argparse.ArgumentParser()
         ap.add_argument("mode")
+        ap.add_argument("kind", nargs="?", default="fixed", choices=["fixed", "uneven"])
         for k in ("--trace", "--fetch", "--write", "--cal-fetch", "--cal-write", "--run", "--command", "--md",
                   "--records"):
             ap.add_argument(k, required=True)
         ap.add_argument("--copy-bytes", type=int, required=True)
         ap.add_argument("--dim", type=int, default=96)
         ap.add_argument("--title", default="search_kernel, 1M x 96, 10k queries")
-        return report(ap.parse_args())
+        a = ap.parse_args()
+        return report_uneven(a) if a.kind == "uneven" else report(a)
     ap = argparse.ArgumentParser()
     ap.add_argument("mode", choices=["trace", "counter", "scale"])
     ap.add_argument("args", nargs="+")

@@ -235,7 +235,8 @@ class Index:
 
     @staticmethod
     def plan_options(buckets=1, bucket_max_L=None, table_eighths=8, use_graph=False, concurrent=False,
-                     want_queue=False, waves=0, rows_per_wave=0, overflow_slots=0, deferred_merge=0):
+                     want_queue=False, waves=0, rows_per_wave=0, overflow_slots=0, deferred_merge=0,
+                     visited_table=0):
         o = _native.PannsPlanOptions()
         o.n_buckets = len(bucket_max_L) if bucket_max_L else buckets
         for i, v in enumerate(bucket_max_L or ()):
@@ -246,6 +247,7 @@ class Index:
         o.rows_per_wave = rows_per_wave
         o.overflow_slots = overflow_slots
         o.deferred_merge = deferred_merge
+        o.visited_table = visited_table
         return o
 
     def plan(self, query_ids, K, L, M=1, mode="top_m", init_ids=None, L_per_query=None,
