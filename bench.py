@@ -115,33 +115,143 @@ def host_cores():
     return max(1, min(n, 16))
 
 
-def cpu_baseline(ds, K, L, M, init_ids, cores, budget_s=12.0):
-    """The oracle restatement of top-M search, inter-query parallel on the
-    host cores (num_t = 1 per query), timed on a bounded sample."""
+def start_cpu_child(cores):
+    """The CPU baseline runs in a child process started before this process
+    touches the GPU, with the reference's pinning in its environment only
+    (OMP_PROC_BIND=close OMP_PLACES=cores, the GOMP form of the reference
+    scripts' KMP_AFFINITY=granularity=fine,compact,1,0) and one OpenMP
+    thread per host core.  It waits for one request on stdin."""
+    import subprocess
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores", OMP_NUM_THREADS=str(cores))
+    return subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-child"], stdin=subprocess.PIPE,
+                            stdout=subprocess.PIPE, env=env, text=True, cwd=ROOT)
+
+
+def _host_info(threads):
+    import platform
+    import subprocess
+    info = {"threads": threads, "cpu_count": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""), "omp_places": os.environ.get("OMP_PLACES", ""),
+            "machine": platform.machine()}
+    # nproc without the OpenMP variables (GNU nproc honours OMP_NUM_THREADS)
+    clean = {k: v for k, v in os.environ.items() if k not in ("OMP_NUM_THREADS", "OMP_THREAD_LIMIT")}
+    for cmd, key in ((["nproc"], "nproc"), (["lscpu"], "lscpu")):
+        try:
+            out = subprocess.run(cmd, capture_output=True, text=True, timeout=10, env=clean).stdout
+        except (OSError, subprocess.SubprocessError):
+            continue
+        if key == "nproc":
+            info["nproc"] = int(out.strip() or 0)
+        else:
+            want = ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU max MHz",
+                    "L3 cache", "NUMA node(s)")
+            for ln in out.splitlines():
+                k, _, v = ln.partition(":")
+                if k.strip() in want:
+                    info["lscpu_" + k.strip().lower().replace("(s)", "s").replace(" ", "_")] = v.strip()
+    try:
+        gcc = subprocess.run(["gcc", "--version"], capture_output=True, text=True, timeout=10).stdout
+        info["gcc"] = gcc.splitlines()[0] if gcc else ""
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_child_main():
+    """--cpu-child: read one JSON request, time the two CPU modes on the
+    oracle restatement, print one JSON line.  Never touches the GPU.
+
+    throughput   inter-query parallel top-M at the GPU's fixed operating
+                 point (K, L, M): one query per OpenMP thread at num_t = 1,
+                 each with its own queue and visited set (the pattern of
+                 app/PSS_v5_multi_queries.cpp:89-96,140-154 with inner = 1);
+                 ids identical to the GPU's fixed-L search.
+    reference    the reference design: PSS_v5 with T = all threads on one
+                 query at a time (PSS5LG.cpp:1376-1724), X swept as
+                 scripts/bg7/sh70.PSS_v5_dt_profiling_deep1b.sh:48-60 does
+                 (L/T to L/T + 64, step 32), at the same L.
+    QPS is queries / wall time of the whole query loop with
+    prepare_init_ids inside (app/PSS_v5_LG_distance_threshold_profiling.cpp:
+    144-166), over a bounded sample of the test queries."""
+    line = sys.stdin.readline()
+    if not line.strip():
+        return  # the parent ended without a request
+    req = json.loads(line)
+    # the host as this process sees it before any OpenMP region binds it
+    host = _host_info(req["threads"])
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
-    g = po.CsrGraph(ds["adj"], ds["deg"], ds["ep"])
-    q = ds["queries"]
-    n_probe = min(200, q.shape[0])
+    from parallel_anns_amd import tools
+    d = req["dir"]
+    base = tools.fvecs_read(os.path.join(d, "base.fvecs"))
+    queries = tools.fvecs_read(os.path.join(d, "query.fvecs"))
+    adj, deg, ep, _ = tools.nsg_read(os.path.join(d, "graph.nsg"))
+    gt, _ = tools.truth_read(os.path.join(d, "truth.bin"))
+    g = po.CsrGraph(adj, deg, ep)
+    del adj
+    K, L, M, T = req["K"], req["L"], req["M"], req["threads"]
+    budget = req["budget_s"]
+    nq = queries.shape[0]
+
+    def sample_size(per_q, share):
+        return int(max(min(nq, 64), min(nq, share / max(per_q, 1e-9))))
+
+    # throughput mode
     t0 = time.perf_counter()
-    po.search_batch(g, ds["base"], q[:n_probe], K, L, M, init_ids, nthreads=cores)
-    per_q = (time.perf_counter() - t0) / n_probe
-    n_s = int(max(n_probe, min(q.shape[0], budget_s / max(per_q, 1e-9))))
+    po.search_batch(g, base, queries[:200], K, L, M, po.prepare_init_ids(g, L), nthreads=T)
+    n = sample_size((time.perf_counter() - t0) / 200, budget * 0.5)
     t0 = time.perf_counter()
-    ids, _, _, _ = po.search_batch(g, ds["base"], q[:n_s], K, L, M, init_ids, nthreads=cores)
+    ids, _, nd, _ = po.search_batch(g, base, queries[:n], K, L, M, po.prepare_init_ids(g, L), nthreads=T)
     el = time.perf_counter() - t0
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
-    except OSError:
-        pass
-    return {"value": round(n_s / el, 1), "unit": "QPS", "cores": cores, "kind": "port",
-            "cpu_model": model, "compiler": "gcc -O3 -march=x86-64-v3 -fopenmp (oracle/Makefile)",
-            "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""), "omp_places": os.environ.get("OMP_PLACES", ""),
-            "sample": f"first {n_s} of {q.shape[0]} test queries, fixed-L top-M M={M} L={L} K={K}, "
-                      f"OpenMP inter-query over {cores} threads (oracle restatement; ids identical "
-                      f"to the GPU fixed-L search)"}, ids
+    ids_path = os.path.join("/tmp", f"panns_cpu_ids_{os.getpid()}.npy")
+    np.save(ids_path, ids)
+    thr = {"qps": round(n / el, 1), "queries": n, "seconds": round(el, 3), "K": K, "L": L, "M": M,
+           "recall": round(float(tools.recall(gt[:n], ids, K)), 4), "ndist_mean": round(float(nd.mean()), 1)}
+    # reference design: X from L/T to L/T + 64 in steps of 32
+    rows = []
+    xs = list(range(max(1, L // T), max(1, L // T) + 65, 32))
+    for X in xs:
+        t0 = time.perf_counter()
+        po.search_pss_v5_par(g, base, queries[:16], K, L, X, T)
+        m = sample_size((time.perf_counter() - t0) / 16, budget * 0.5 / len(xs))
+        t0 = time.perf_counter()
+        rid, rnd = po.search_pss_v5_par(g, base, queries[:m], K, L, X, T)
+        el = time.perf_counter() - t0
+        rows.append({"X": X, "qps": round(m / el, 1), "queries": m, "seconds": round(el, 3),
+                     "recall": round(float(tools.recall(gt[:m], rid, K)), 4),
+                     "ndist_mean": round(float(rnd.mean()), 1)})
+    out = {"throughput": thr, "reference_design": {"T": T, "L": L, "K": K, "rows": rows}, "ids_path": ids_path,
+           "host": host, "oracle_compiler": "gcc " + po.build_info() +
+           " -O3 -march=x86-64-v3 -ffp-contract=off -fopenmp (oracle/Makefile)"}
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(child, threads, ds, K, L, M, budget_s=20.0):
+    """Send the request to the CPU child and collect its result: the
+    throughput mode is `value`, the reference design rides beside it."""
+    req = {"dir": ds["dir"], "K": int(K), "L": int(L), "M": int(M), "threads": int(threads), "budget_s": budget_s}
+    child.stdin.write(json.dumps(req) + "\n")
+    child.stdin.flush()
+    line = child.stdout.readline()
+    rc = child.wait(timeout=600)
+    if rc != 0 or not line:
+        raise RuntimeError(f"CPU baseline child failed (exit {rc})")
+    r = json.loads(line)
+    ids = np.load(r["ids_path"])
+    os.remove(r["ids_path"])
+    thr, rd, host = r["throughput"], r["reference_design"], r["host"]
+    best = max(rd["rows"], key=lambda x: x["qps"])
+    out = {"value": thr["qps"], "unit": "QPS", "cores": host["threads"], "kind": "port",
+           "sample": f"first {thr['queries']} of {ds['queries'].shape[0]} test queries, fixed-L top-M M={M} "
+                     f"L={L} K={K}, OpenMP inter-query over {host['threads']} threads at num_t = 1 (oracle "
+                     f"restatement, ids identical to the GPU fixed-L search)",
+           "recall": thr["recall"],
+           "reference_design": {"what": f"PSS_v5, T = {rd['T']} threads on one query at a time, X swept "
+                                        f"L/T..L/T+64 step 32, L = {rd['L']}",
+                                "rows": rd["rows"], "best_qps": best["qps"], "best_X": best["X"]},
+           "host": host, "compiler": r["oracle_compiler"], "process": "child started before the GPU was used"}
+    return out, ids
 
 
 def fixed_operating_point(index, qids, K, gt, target, ms, nq, op=None, hi=1024):
@@ -283,6 +393,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; queries shard across GPUs with no "
                                    "exchange step, so N>1 is not offered", "n_gpus": args.gpus}))
         sys.exit(2)
+    # the CPU baseline's process starts before this one touches the GPU
+    child = None if args.no_cpu else start_cpu_child(cores)
 
     from parallel_anns_amd import datasets, learned, tools
     from parallel_anns_amd.search import Index
@@ -485,10 +597,10 @@ def main():
     achieved = B / (main_ms * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu:
-        init = index.prepare_init_ids(fL)
-        cpu, cpu_ids = cpu_baseline(ds, K, fL, fM, init, cores)
+        cpu, cpu_ids = cpu_baseline(child, cores, ds, K, fL, fM)
         cpu["ids_identical_to_gpu"] = bool((cpu_ids == fres["ids"][:cpu_ids.shape[0]]).all())
-        log(f"cpu baseline {cpu['value']:,.0f} QPS on {cpu['cores']} cores; ids identical: {cpu['ids_identical_to_gpu']}")
+        log(f"cpu baseline {cpu['value']:,.0f} QPS on {cpu['cores']} cores; ids identical: "
+            f"{cpu['ids_identical_to_gpu']}; reference design {cpu['reference_design']['rows']}")
     kernel = f"search_kernel<{dim}, {'true' if use_lrn and not lrn['fallback_to_fixed'] else 'false'}>"
     traffic, traffic_src = measured_traffic(args.config, kernel, dict(lrn if use_lrn else fixed, algorithmic_bytes=B))
 
@@ -534,4 +646,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "--cpu-child":
+        cpu_child_main()
+    else:
+        main()

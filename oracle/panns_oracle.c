@@ -577,6 +577,162 @@ void po_search_pss_v5(const po_graph *g, const float *base, uint32_t dim, const 
     }
 }
 
+/* ---- the reference design for the CPU baseline (timed, not a parity
+ * function): PSS_v5 with its T loops running at once as OpenMP threads.
+ *
+ * Per query as PSS5LG.cpp:1376-1724: initialise the master queue (the L
+ * init distances in an OpenMP loop, :1055-1096), one sequential master
+ * step, then iterations of pick_top_m_to_workers, a parallel region in
+ * which thread w runs up to X expansions on its own queue (the master,
+ * w = T-1, on the master queue; :1584-1646), and merge_all_queues_to_master.
+ * The reference's shared state is made race-free without changing what is
+ * shared: the visited bitset is claimed with an atomic fetch-or (the
+ * reference's unsynchronised read-modify-write of bitset blocks,
+ * :1017-1020), and the master's L-th distance, which the workers read live
+ * through `const distf &last_dist` (:1412), is published by the master
+ * thread after each of its insertions and read with relaxed atomics.  At
+ * T > 1 the result depends on the schedule, as the reference's does. */
+static inline float load_bound(const uint32_t *bits)
+{
+    uint32_t b = __atomic_load_n(bits, __ATOMIC_RELAXED);
+    float f;
+    memcpy(&f, &b, sizeof(f));
+    return f;
+}
+
+static uint32_t expand_one_candidate_par(const po_graph *g, const float *base, uint32_t dim, uint32_t cand_id,
+                                         const float *query, int is_master, uint32_t *bound_bits,
+                                         float dist_thresh, po_cand *set_L, uint32_t queue_start,
+                                         uint32_t *queue_size, uint32_t queue_capacity, uint64_t *visited,
+                                         uint64_t *count_distance)
+{
+    const uint32_t *out_edges = g->deg_ngbrs + g->vertex_base[cand_id];
+    uint32_t out_degree = *out_edges++;
+    uint32_t nk = queue_capacity;
+    uint64_t cnt = 0;
+    for (uint32_t e = 0; e < out_degree; ++e) {
+        const uint32_t nb = out_edges[e];
+        const uint64_t m = (uint64_t)1 << (nb & 63);
+        if (__atomic_load_n(&visited[nb >> 6], __ATOMIC_RELAXED) & m) continue;
+        if (__atomic_fetch_or(&visited[nb >> 6], m, __ATOMIC_RELAXED) & m) continue;
+        ++cnt;
+        const float d = vdist(g->norms, base, dim, nb, query);
+        const float bound = is_master ? set_L[queue_start + *queue_size - 1].dist : load_bound(bound_bits);
+        if (d > bound || d > dist_thresh) continue;
+        po_cand c;
+        memset(&c, 0, sizeof(c));
+        c.id = nb;
+        c.dist = d;
+        c.checked = 0;
+        const uint32_t r = po_add_into_queue(set_L, queue_start, queue_size, queue_capacity, c);
+        if (r < nk) nk = r;
+        if (is_master && r < queue_capacity) {
+            uint32_t b;
+            memcpy(&b, &set_L[queue_start + *queue_size - 1].dist, sizeof(b));
+            __atomic_store_n(bound_bits, b, __ATOMIC_RELAXED);
+        }
+    }
+    *count_distance += cnt;
+    return nk;
+}
+
+uint64_t po_search_pss_v5_par(const po_graph *g, const float *base, uint32_t dim, const float *queries,
+                              uint32_t nq, uint32_t K, uint32_t L, uint32_t X, uint32_t T, uint32_t local_L,
+                              uint32_t *out_ids, uint32_t *out_ndist)
+{
+    const size_t words = (g->num_v + 63) / 64;
+    po_cand *set_L = (po_cand *)malloc(sizeof(po_cand) * ((size_t)(T - 1) * local_L + L));
+    uint64_t *visited = (uint64_t *)calloc(words, sizeof(uint64_t));
+    uint32_t *starts = (uint32_t *)malloc(sizeof(uint32_t) * T);
+    uint32_t *sizes = (uint32_t *)calloc(T, sizeof(uint32_t));
+    uint32_t *init_ids = (uint32_t *)malloc(sizeof(uint32_t) * L);
+    for (uint32_t w = 0; w < T; ++w) starts[w] = w * local_L;
+    const uint32_t ms = starts[T - 1];
+    uint64_t total = 0;
+    po_prepare_init_ids(g, L, init_ids);
+    for (uint32_t qi = 0; qi < nq; ++qi) {
+        const float *query = queries + (size_t)qi * dim;
+        uint64_t ndist = L;
+        for (uint32_t c = 0; c < L; ++c) bit_set(visited, init_ids[c]);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(T)
+#endif
+        for (int64_t i = 0; i < (int64_t)L; ++i) {
+            po_cand c;
+            memset(&c, 0, sizeof(c));
+            c.id = init_ids[i];
+            c.dist = vdist(g->norms, base, dim, c.id, query);
+            set_L[ms + i] = c;
+        }
+        qsort(set_L + ms, L, sizeof(po_cand), cmp_cand);
+        for (uint32_t w = 0; w + 1 < T; ++w) sizes[w] = 0;
+        sizes[T - 1] = L;
+        uint32_t bound_bits;
+        memcpy(&bound_bits, &set_L[ms + L - 1].dist, sizeof(bound_bits));
+        uint32_t k_master = 0;
+        {
+            po_cand *c = &set_L[ms];
+            c->checked = 1;
+            const uint32_t r = expand_one_candidate_par(g, base, dim, c->id, query, 1, &bound_bits,
+                                                        set_L[ms + L - 1].dist, set_L, ms, &sizes[T - 1], L,
+                                                        visited, &ndist);
+            k_master = r <= k_master ? r : k_master + 1;
+        }
+        while (pick_top_m_to_workers(set_L, starts, sizes, T, local_L, k_master)) {
+            const float dist_thresh = load_bound(&bound_bits);
+            uint32_t km = k_master;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(T) reduction(+ : ndist)
+#endif
+            {
+#ifdef _OPENMP
+                const uint32_t w = (uint32_t)omp_get_thread_num();
+#else
+                for (uint32_t w = 0; w < T; ++w)
+#endif
+                {
+                    const int master = w == T - 1;
+                    const uint32_t st = starts[w], cap = master ? L : local_L;
+                    uint32_t k_uc = master ? km : 0u, it = 0;
+                    uint64_t nd = 0;
+                    while (it < X && k_uc < sizes[w]) {
+                        po_cand *c = &set_L[st + k_uc];
+                        if (c->checked) {
+                            ++k_uc;
+                            continue;
+                        }
+                        c->checked = 1;
+                        ++it;
+                        const uint32_t r = expand_one_candidate_par(g, base, dim, c->id, query, master, &bound_bits,
+                                                                    dist_thresh, set_L, st, &sizes[w], cap, visited,
+                                                                    &nd);
+                        k_uc = r <= k_uc ? r : k_uc + 1;
+                    }
+                    if (master) km = k_uc;
+                    ndist += nd;
+                }
+            }
+            k_master = km;
+            const uint32_t r = merge_all_queues_to_master(set_L, starts, sizes, T, L);
+            if (r <= k_master) k_master = r;
+            memcpy(&bound_bits, &set_L[ms + L - 1].dist, sizeof(bound_bits));
+        }
+        for (uint32_t i = 0; i < K; ++i) out_ids[(size_t)qi * K + i] = set_L[ms + i].id;
+        if (out_ndist) out_ndist[qi] = (uint32_t)ndist;
+        total += ndist;
+        memset(visited, 0, sizeof(uint64_t) * words);
+    }
+    free(set_L);
+    free(visited);
+    free(starts);
+    free(sizes);
+    free(init_ids);
+    return total;
+}
+
+/* The compiler that built this library (its __VERSION__ string). */
+const char *po_build_info(void) { return __VERSION__; }
+
 /* TOPMLG.cpp:1077-1131: the bound passed to expand_one_candidate is the
  * element at index set_L_size-1 at the time of each call (a live reference). */
 static void subsearch_one_iteration(const po_graph *g, const float *base, uint32_t dim,

@@ -45,6 +45,12 @@
  *                          doubling M schedule.
  *   po_brute_force_knn     tiny inputs against numpy float64 ranking with ties
  *                          checked by id; equals L == N search.
+ *   po_search_pss_v5_par   the CPU baseline's timed reference design (its T
+ *                          threads race as the reference's do, so no parity
+ *                          claim at T > 1): T = 1 equals po_search_top_m
+ *                          with M = 1 for every X; T > 1 by properties (K
+ *                          distinct ids, #dist >= L, recall within 0.02 of
+ *                          po_search_pss_v5).
  * The reference itself cannot be built here (it needs Boost, which the image
  * lacks), so no golden vector comes from a compiled reference.
  */
@@ -153,6 +159,19 @@ uint32_t po_merge_two_queues_incr(po_cand *queue1, uint32_t queue1_start, uint32
 void po_search_pss_v5(const po_graph *g, const float *base, uint32_t dim, const float *query, uint32_t K,
                       uint32_t L, uint32_t X, uint32_t T, uint32_t local_L, const uint32_t *init_ids, po_cand *set_L,
                       uint32_t *set_K, uint64_t *visited, po_counters *cnt);
+
+/* The reference design for the CPU baseline (timing, not parity): queries
+ * one after another, each by PSS_v5 with T OpenMP threads expanding at once
+ * (PSS5LG.cpp:1376-1724, prepare_init_ids inside), the shared visited bits
+ * claimed with atomic fetch-or and the master's L-th distance read with
+ * relaxed atomics.  Results depend on the schedule at T > 1.  out_ids gets
+ * nq x K ids; returns the distance computations. */
+uint64_t po_search_pss_v5_par(const po_graph *g, const float *base, uint32_t dim, const float *queries,
+                              uint32_t nq, uint32_t K, uint32_t L, uint32_t X, uint32_t T, uint32_t local_L,
+                              uint32_t *out_ids, uint32_t *out_ndist);
+
+/* The compiler that built the library (__VERSION__). */
+const char *po_build_info(void);
 
 /* TOPMLG.cpp:1023-1071 subsearch_with_top_m + :1077-1131
  * subsearch_top_m_for_one_iteration.  Works in place on set_L[set_L_start ..

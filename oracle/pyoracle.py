@@ -53,6 +53,8 @@ _API = {
                                 ctypes.POINTER(Counters)]),
     "po_search_batch": (u64, [ctypes.POINTER(Graph), vp, u32, vp, u32, u32, u32, u32, cint, vp, vp,
                               vp, vp, cint]),
+    "po_search_pss_v5_par": (u64, [ctypes.POINTER(Graph), vp, u32, vp, u32, u32, u32, u32, u32, u32, vp, vp]),
+    "po_build_info": (ctypes.c_char_p, []),
     "po_brute_force_knn": (None, [vp, u32, u32, vp, u32, u32, vp, vp, cint]),
     "po_recall": (cd, [vp, u32, vp, u32, u32, u32, u32]),
 }
@@ -224,6 +226,25 @@ def search_batch(graph, base, queries, K, L, M, init_ids, mode="top_m", nthreads
                                   1 if mode == "pss_v5" else 0, _p(init_ids), _p(ids), _p(dists), _p(nd),
                                   nthreads)
     return ids, dists, nd, int(total)
+
+
+def search_pss_v5_par(graph, base, queries, K, L, X, T, local_L=None):
+    """The reference design for timing: PSS_v5 with T OpenMP threads per
+    query, queries one after another (po_search_pss_v5_par).  Returns
+    (ids [nq, K], distance counts [nq])."""
+    base = np.ascontiguousarray(base, np.float32)
+    queries = np.ascontiguousarray(queries, np.float32)
+    nq = queries.shape[0]
+    ids = np.empty((nq, K), np.uint32)
+    nd = np.empty(nq, np.uint32)
+    lib().po_search_pss_v5_par(ctypes.byref(graph.g), _p(base), base.shape[1], _p(queries), nq, K, L, X, T,
+                               L if local_L is None else local_L, _p(ids), _p(nd))
+    return ids, nd
+
+
+def build_info():
+    """The compiler string the oracle library was built with."""
+    return lib().po_build_info().decode()
 
 
 def brute_force_knn(base, queries, K, nthreads=0):

@@ -43,7 +43,7 @@ def main():
     from . import datasets
     from .search import Index
 
-    ds = datasets.prepare(args.config, cache_dir=args.cache_dir, gpu_knn=datasets.GpuKnn(0), verbose=False)
+    ds = datasets.prepare(args.config, cache_dir=args.cache_dir, gpu_knn=datasets.GpuKnn(0))
     K = ds["cfg"]["K"]
     nq = ds["queries"].shape[0]
     dim = ds["base"].shape[1]

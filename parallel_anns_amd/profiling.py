@@ -111,7 +111,9 @@ def report_uneven(a):
              f"Copy-kernel calibration ({a.copy_bytes} bytes copied): FETCH_SIZE factor {ff:.4f}, "
              f"WRITE_SIZE factor {wf:.4f} bytes per counted byte.", "",
              f"Per bucket, mean of the last {steps} runs (algorithmic bytes: distance rows and results exact, "
-             "adjacency rows at the batch's mean read slots per expansion):", "",
+             "adjacency rows at the batch's mean read slots per expansion)"
+             + (": the buckets run side by side, so their kernel durations overlap and each bucket's rate is "
+                "the one it gets while sharing the GPU with the others" if o["concurrent"] else "") + ":", "",
              "| bucket | largest L | queries | workgroups | LDS B | visited slots | trace mean us | FETCH KiB | "
              "WRITE KiB | traffic GB | algorithmic GB | traffic / alg. | achieved GB/s |",
              "|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
