@@ -240,10 +240,10 @@ struct panns_plan {
 
 namespace {
 
-template <int D, bool kLearned, bool kNorm, bool kTrace = false>
+template <int D, bool kLearned, bool kNorm, bool kTrace = false, bool kDefer = false>
 hipError_t launch_search(const panns::KArgs &a, uint32_t grid, uint32_t lds, hipStream_t s)
 {
-    hipLaunchKernelGGL((panns::search_kernel<D, kLearned, kNorm, kTrace>), dim3(grid), dim3(64), lds, s, a);
+    hipLaunchKernelGGL((panns::search_kernel<D, kLearned, kNorm, kTrace, kDefer>), dim3(grid), dim3(64), lds, s, a);
     return hipGetLastError();
 }
 
@@ -265,12 +265,19 @@ using LaunchFn = hipError_t (*)(const panns::KArgs &, uint32_t, uint32_t, hipStr
 using KernelPtr = const void *;
 
 // The search kernel for a dimension; `learned` selects the variant that
-// evaluates the stop model (squared L2 only), `norm` the opt-graph distance.
-bool kernel_for_dim(uint32_t dim, bool learned, bool norm, bool trace, LaunchFn *fn, KernelPtr *kp)
+// evaluates the stop model (squared L2 only), `norm` the opt-graph distance,
+// `defer` the fixed-L variant with the deferred merge.
+bool kernel_for_dim(uint32_t dim, bool learned, bool norm, bool trace, bool defer, LaunchFn *fn, KernelPtr *kp)
 {
     switch (dim) {
 #define PANNS_DIM(DD)                                                                                   \
     case DD:                                                                                            \
+        if (defer) {                                                                                    \
+            *fn = norm ? launch_search<DD, false, true, false, true> : launch_search<DD, false, false, false, true>; \
+            *kp = norm ? reinterpret_cast<const void *>(&panns::search_kernel<DD, false, true, false, true>)         \
+                       : reinterpret_cast<const void *>(&panns::search_kernel<DD, false, false, false, true>);       \
+            return true;                                                                                \
+        }                                                                                               \
         if (trace) {                                                                                    \
             *fn = launch_search<DD, false, false, true>;                                                \
             *kp = reinterpret_cast<const void *>(&panns::search_kernel<DD, false, false, true>);        \
@@ -344,12 +351,12 @@ bool kernel_for_dim_pss(uint32_t dim, bool norm, LaunchFn *fn, KernelPtr *kp)
     }
 }
 
-bool select_kernel(uint32_t dim, bool learned, bool norm, bool trace, uint32_t waves, bool pss, LaunchFn *fn,
-                   KernelPtr *kp)
+bool select_kernel(uint32_t dim, bool learned, bool norm, bool trace, uint32_t waves, bool pss, bool defer,
+                   LaunchFn *fn, KernelPtr *kp)
 {
     if (pss) return kernel_for_dim_pss(dim, norm, fn, kp);
     return waves > 1 ? kernel_for_dim_mw(dim, waves, norm, fn, kp)
-                     : kernel_for_dim(dim, learned, norm, trace, fn, kp);
+                     : kernel_for_dim(dim, learned, norm, trace, defer, fn, kp);
 }
 
 using KnnLaunch = void (*)(const float *, uint32_t, const float *, uint32_t, uint32_t, int, uint32_t,
@@ -591,7 +598,8 @@ int plan_geometry(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq,
     if (norm && pl->trace_cap) return fail(PANNS_ERR_UNSUPPORTED, "traces are taken with squared-L2 distances");
     LaunchFn fn;
     KernelPtr kp;
-    if (!select_kernel(h->dim, p.stop == PANNS_STOP_LEARNED, norm, pl->trace_cap != 0, pl->waves, pl->pss, &fn, &kp))
+    if (!select_kernel(h->dim, p.stop == PANNS_STOP_LEARNED, norm, pl->trace_cap != 0, pl->waves, pl->pss, false,
+                       &fn, &kp))
         return fail(PANNS_ERR_DIM, "dimension " + std::to_string(h->dim) + " has no kernel (16, 32, 40, 64, 96, 128)");
     pl->maxL = p.L;
     pl->maxK = p.K;
@@ -685,6 +693,8 @@ int plan_geometry(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq,
     pl->lds = pl->pss ? lds_bytes_pss(Lcap, nb_cap, hs + 2u * ovf, h->dim, p.num_threads)
               : W > 1 ? lds_bytes_mw(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim, W)
                       : lds_bytes(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim) + 8u * pl->pend_cap;
+    if (pl->pend_cap)
+        select_kernel(h->dim, false, norm, false, 1u, false, true, &fn, &kp);  // the deferred-merge variant
     if (pl->lds > 65536) {
         HIP_TRY(hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->lds));
     }
@@ -906,7 +916,7 @@ hipError_t launch_one(const panns_plan *pl, hipStream_t st)
     LaunchFn fn;
     KernelPtr kp;
     select_kernel(pl->h->dim, pl->args.stop_learned != 0, pl->args.norms != nullptr, pl->args.trace_feat != nullptr,
-                  pl->waves, pl->pss, &fn, &kp);
+                  pl->waves, pl->pss, pl->pend_cap != 0, &fn, &kp);
     return fn(pl->args, pl->grid, pl->lds, st);
 }
 

@@ -1002,8 +1002,11 @@ __device__ __forceinline__ uint32_t next_work(const KArgs &a, uint32_t &r, uint3
 // (a.norms) instead of squared L2.  kTrace: the fixed-L search that also
 // records the stop features and top-K hashes (a.trace_feat, a.trace_topk);
 // the search variants carry none of that code, which keeps their uniform
-// state within the scalar registers.
-template <int D, bool kLearned, bool kNorm, bool kTrace = false>
+// state within the scalar registers.  kDefer: the fixed-L search with the
+// deferred merge (a.pend_cap > 0; not with kLearned or kTrace); the other
+// variants carry none of its code (measured: its branches alone cost the
+// plain search 4 %).
+template <int D, bool kLearned, bool kNorm, bool kTrace = false, bool kDefer = false>
 __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
 {
     extern __shared__ uint64_t smem[];
@@ -1117,7 +1120,8 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
         uint32_t Mcur = (a.mode == kModeScaleM) ? 1u : (a.mode == kModePssV5 ? 1u : a.M);
         // deferred merge: the fixed-stop search without traces, picks of at
         // most 64, a full queue (the stop features need the exact queue)
-        const bool defer = !kLearned && !kTrace && a.pend_cap != 0 && S == L;
+        static_assert(!kDefer || (!kLearned && !kTrace), "the deferred merge is a fixed-L search variant");
+        const bool defer = kDefer && S == L;
         uint32_t SP = 0;
         for (;;) {
             uint32_t m = 0, last_k = 0, pos = k;
