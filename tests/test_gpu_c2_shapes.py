@@ -1,0 +1,67 @@
+"""GPU parity at the 10M uneven workload's own size: 10,000,000 x 96 from the
+256-component mixture, graph R <= 32 from exact 64-NN per component
+(prepared on the GPU in about 40 seconds), all 100,000 queries with per-query
+L log-uniform in [100, 2000] and K in {10, 100}.
+
+The batch runs in the autotuner's kind of launch (six queue-size buckets side
+by side, each bucket's visited table by its own largest L, the deferred merge
+above L = 1,400, one hipGraph), and again with no visited table (every id
+claimed in the bitmap, the form 50M ids take above L = 800).  A sample is
+checked against the oracle at each query's own L and K (the 30 largest L and
+every 2,500th query: ids, distance bits, distance counts, iterations); the
+two launches must agree on every query.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from parallel_anns_amd import datasets
+from parallel_anns_amd.search import Index
+
+pytestmark = pytest.mark.gpu
+
+CACHE = os.environ.get("PANNS_CACHE", "/tmp/panns_cache")
+
+
+def test_c2_uneven_batch_against_oracle():
+    print("case prepare c2 10M x 96", flush=True)
+    ds = datasets.prepare("c2", cache_dir=CACHE, gpu_knn=datasets.GpuKnn(0), verbose=False)
+    nq = ds["queries"].shape[0]
+    Ls, Ks = datasets.uneven_L_K(ds["cfg"], nq)
+    q = np.arange(nq, dtype=np.uint32)
+    idx = Index(0)
+    try:
+        idx.set_base(ds["base"])
+        idx.set_queries(ds["queries"])
+        idx.set_graph(ds["adj"], ds["deg"], ds["ep"])
+        maxL = int(Ls.max())
+        init = idx.prepare_init_ids(maxL)
+        res = []
+        for table in (0, 2):
+            print("case c2 uneven visited_table", table, flush=True)
+            opt = Index.plan_options(bucket_max_L=[417, 734, 1049, 1367, 1683, maxL], table_eighths=0,
+                                     use_graph=True, concurrent=True, rows_per_wave=64, visited_table=table)
+            pl = idx.plan(q, 100, maxL, 4, init_ids=init, L_per_query=Ls, K_per_query=Ks, options=opt,
+                          want_queue=False)
+            assert all((b["hash_slots"] == 0) == (table == 2) for b in pl.buckets())
+            pl.run()
+            pl.run()
+            res.append(pl.fetch())
+            pl.close()
+    finally:
+        idx.close()
+    a, b = res
+    for f in ("ids", "ndist", "niter", "nexp"):
+        assert (a[f] == b[f]).all(), f
+    assert (a["dists"].view(np.uint32) == b["dists"].view(np.uint32)).all()
+    g = po.CsrGraph(ds["adj"], ds["deg"], ds["ep"])
+    sample = np.concatenate([np.argsort(Ls, kind="stable")[-30:], np.arange(0, nq, 2_500)])
+    for i in sample:
+        L, K = int(Ls[i]), int(Ks[i])
+        r = po.search_one(g, ds["base"], ds["queries"][i], K, L, 4, init[:L])
+        assert (a["ids"][i, :K] == r["ids"]).all(), (i, L, K)
+        assert (a["dists"][i, :K].view(np.uint32) == r["queue"]["dist"][:K].view(np.uint32)).all(), i
+        assert a["ndist"][i] == r["ndist"] and a["niter"][i] == r["merges"], i
+        assert (a["ids"][i, K:] == 0xFFFFFFFF).all()
