@@ -769,6 +769,28 @@ struct Visited {
     }
 };
 
+// Zeroes the bitmap words a query set: the first n entries of its spill log
+// (one word index per id stored in the bitmap, duplicates allowed).  Eight
+// log entries per lane are read before any word is written, so a query with
+// thousands of spills waits for n / 512 log round trips instead of n / 64.
+// The log was written by other lanes of the wave with plain stores, so it is
+// read at agent scope (past the vector L1).
+__device__ __forceinline__ void clear_logged(uint32_t *bitmap, const uint32_t *log, uint32_t n, int lane)
+{
+    constexpr uint32_t kU = 8;
+    for (uint32_t i0 = 0; i0 < n; i0 += 64u * kU) {
+        uint32_t wd[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            const uint32_t i = i0 + u * 64u + (uint32_t)lane;
+            wd[u] = __hip_atomic_load(&log[i < n ? i : n - 1u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u)
+            if (i0 + u * 64u + (uint32_t)lane < n) bitmap[wd[u]] = 0u;
+    }
+}
+
 // Claims up to kClaimChunks ids per lane (kSent = empty slot): table probes
 // chunk by chunk, then one batch of bitmap atomics for the ids the table could
 // not take.  Appends the newly visited ids to newids[nnew...] in chunk order
@@ -1412,10 +1434,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
         // empty; every atomic above has returned, so plain stores suffice
         if (!a.shared_bitmap && nsp) {
             if (nsp <= a.log_cap) {
-                for (uint32_t i = lane; i < nsp; i += 64) {
-                    const uint32_t wd = __hip_atomic_load(&vis.log[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    vis.bitmap[wd] = 0u;
-                }
+                clear_logged(vis.bitmap, vis.log, nsp, lane);
             } else {
                 for (uint64_t i = lane; i < a.bitmap_words; i += 64) vis.bitmap[i] = 0u;
             }

@@ -255,10 +255,7 @@ __global__ __launch_bounds__(64 * W, 2) void search_kernel_mw(KArgs a)
         if (vis.spills) {
             __threadfence();
             if (vis.spills <= vis.log_cap) {
-                for (uint32_t i = lane; i < vis.spills; i += 64) {
-                    const uint32_t wd = __hip_atomic_load(&vis.log[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    vis.bitmap[wd] = 0u;
-                }
+                clear_logged(vis.bitmap, vis.log, vis.spills, lane);
             } else {
                 // log overflow: this wave clears the whole bitmap (another
                 // wave clearing its own words meanwhile writes zeros too)

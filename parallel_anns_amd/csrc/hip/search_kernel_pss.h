@@ -358,10 +358,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel_pss(KArgs a)
         }
         if (nsp) {
             if (nsp <= a.log_cap) {
-                for (uint32_t i = lane; i < nsp; i += 64) {
-                    const uint32_t wd = __hip_atomic_load(&vis.log[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    vis.bitmap[wd] = 0u;
-                }
+                clear_logged(vis.bitmap, vis.log, nsp, lane);
             } else {
                 for (uint64_t i = lane; i < a.bitmap_words; i += 64) vis.bitmap[i] = 0u;
             }
