@@ -32,6 +32,12 @@ CONFIGS = {
     # log-uniform in [100, 2000] and K = 100; exact 64-NN per component
     "c4": dict(n=50_000_000, nq=100_000, dim=96, comps=1024, sigma=0.3, knn=64, R=32, K=100, L=100,
                knn_by_component=True, L_range=(100, 2000), K_choices=(100,)),
+    # C1's base set and queries with the NSG-style construction (search-path
+    # candidate pool of 40 from the navigating node, occlusion over at most
+    # C = 500 candidates, reverse edges) instead of the configured k-NN ->
+    # RNG prune -> DFS stitch; a secondary workload, not a configuration
+    "c1nsg": dict(n=1_000_000, nq=10_000, dim=96, comps=64, sigma=0.3, knn=100, R=50, K=100, L=100,
+                  L_build=40, C=500, reverse=True),
     # a 1M-row rehearsal of C4's pipeline (per-component k-NN, uneven L)
     "c4s": dict(n=1_000_000, nq=10_000, dim=96, comps=64, sigma=0.3, knn=64, R=32, K=100, L=100,
                 knn_by_component=True, L_range=(100, 2000), K_choices=(100,)),
