@@ -255,15 +255,13 @@ typedef struct {
                                                   buckets are full, before the global bitmap: 0 or
                                                   a power of two in [8, 4096] */
     uint32_t deferred_merge;                   /* 0: automatic (on for buckets whose largest L is above
-                                                  1,400); 1: merge every iteration's survivors into the
+                                                  1,000); 1: merge every iteration's survivors into the
                                                   queue; 2: always defer.  Deferred: survivors wait in a
                                                   sorted buffer of 256 keys and are merged into the queue
                                                   when it fills or a pick could rank outside the top L
                                                   (fixed stop, one wave, M <= 64, no traces); results are
                                                   the same */
-    uint32_t visited_table;                    /* 0: automatic (none for one-wave buckets above L = 800
-                                                  whose id space needs half again the buckets of the
-                                                  planned table, e.g. 50M ids); 1: always a table;
+    uint32_t visited_table;                    /* 0 or 1: an LDS table with exact spill to the bitmap;
                                                   2: none, every visited id claimed in the bitmap.
                                                   Results are the same */
 } panns_plan_options;

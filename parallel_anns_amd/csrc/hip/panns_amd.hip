@@ -93,7 +93,11 @@ constexpr uint32_t kRanges = 8;         // work ranges (one per XCD)
 constexpr uint32_t kPivots = 256;       // locality pivots
 constexpr uint32_t kLocalityMin = 512;  // smallest batch that is reordered
 constexpr uint32_t kPendCap = 256;      // pending keys of the deferred merge
-constexpr uint32_t kDeferMinL = 1400;   // automatic deferred merge above this L
+// automatic deferred merge above this L: measured on 1M x 96 (M = 4) it
+// loses 8-19 % at L = 720 and 900, where its 2 KB of LDS cost a workgroup
+// per CU, and gains 2 % at L = 1100 and 7 % at 1400; on 50M x 96 it gains
+// 8 % for a bucket of L 579-1049
+constexpr uint32_t kDeferMinL = 1000;
 
 }  // namespace
 
@@ -648,7 +652,6 @@ int plan_geometry(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq,
     else if (pl->table_eighths == 0)
         pl->table_eighths = 8;
     if (!hs) hs = ((pl->maxL <= 512 ? 8192u : 10240u) * pl->table_eighths / 8u + 63u) & ~63u;
-    const uint32_t planned = hs;
     const uint32_t id_bits = std::max<uint32_t>(1u, log2u(h->n));
     const uint64_t min_nb = (((1ull << id_bits) - 1ull) / 0x7FFEull) + 1ull;
     // buckets of 8 slots, or of 4 when an 8-slot table of this size has
@@ -658,15 +661,13 @@ int plan_geometry(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq,
     uint32_t nb = std::max<uint32_t>(hs / bslots, (uint32_t)((min_nb + 7ull) & ~7ull));
     nb = std::max<uint32_t>(nb, 8u);
     hs = nb * bslots;
-    // No table: every visited id is claimed in the bitmap.  Automatic for a
-    // one-wave plan above L = 800 whose id space needs half again the
-    // buckets of its planned table (16-bit tags hold 15 bits of the
-    // scrambled id, so 50M ids need 8,224 slots where 3,840 were planned):
-    // measured on 50M x 96, such a table costs more in occupancy than it
-    // saves in bitmap claims (no table: 8 % faster over L 100-2000).
-    pl->no_table = !pl->pss && !pl->shared_bitmap &&
-                   (pl->visited_table == 2 ||
-                    (pl->visited_table == 0 && auto_size && W == 1 && pl->maxL > 800 && hs > planned + planned / 2));
+    // No table (visited_table = 2 only): every visited id is claimed in the
+    // bitmap.  Measured on 50M x 96 over L 100-2000, a table is faster at
+    // every L even where the id space forces it to 8,224 slots (16-bit tags
+    // hold 15 bits of the scrambled id) and the LDS allows 4 workgroups per
+    // CU: 4 to 48 % per L range, since a claim in the bitmap is a memory
+    // atomic on a random line and the bitmap must be cleared again.
+    pl->no_table = !pl->pss && !pl->shared_bitmap && pl->visited_table == 2;
     if (pl->no_table) {
         hs = 64;
         nb = 8;

@@ -5,7 +5,7 @@ L log-uniform in [100, 2000] and K in {10, 100}.
 
 The batch runs in the autotuner's kind of launch (six queue-size buckets side
 by side, each bucket's visited table by its own largest L, the deferred merge
-above L = 1,400, one hipGraph), and again with no visited table (every id
+above L = 1,000, one hipGraph), and again with no visited table (every id
 claimed in the bitmap, the form 50M ids take above L = 800).  A sample is
 checked against the oracle at each query's own L and K (the 30 largest L and
 every 2,500th query: ids, distance bits, distance counts, iterations); the

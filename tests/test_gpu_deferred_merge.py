@@ -56,7 +56,7 @@ def test_deferred_with_spills_modes_and_buckets(small_set, gpu_index):
     print("case deferred scale_m / pss_v5", flush=True)
     _same(_run(gpu_index, q, 50, 250, 32, 2, mode="scale_m"), _run(gpu_index, q, 50, 250, 32, 1, mode="scale_m"))
     _same(_run(gpu_index, q, 50, 250, 3, 2, mode="pss_v5"), _run(gpu_index, q, 50, 250, 3, 1, mode="pss_v5"))
-    # uneven L in buckets, automatic choice (deferred above L = 1,400)
+    # uneven L in buckets, automatic choice (deferred above L = 1,000)
     rng = np.random.default_rng(5)
     Ls = rng.integers(100, 2001, q.shape[0]).astype(np.uint32)
     init = gpu_index.prepare_init_ids(int(Ls.max()))
