@@ -258,9 +258,9 @@ typedef struct {
                                                   1,000); 1: merge every iteration's survivors into the
                                                   queue; 2: always defer.  Deferred: survivors wait in a
                                                   sorted buffer of 256 keys and are merged into the queue
-                                                  when it fills or a pick could rank outside the top L
-                                                  (fixed stop, one wave, M <= 64, no traces); results are
-                                                  the same */
+                                                  when it fills, a pick could rank outside the top L or
+                                                  the stop model runs (one wave, M <= 64, no traces);
+                                                  results are the same */
     uint32_t visited_table;                    /* 0 or 1: an LDS table with exact spill to the bitmap;
                                                   2: none, every visited id claimed in the bitmap.
                                                   Results are the same */
@@ -324,8 +324,9 @@ int panns_exact_knn_ex(panns_index *h, int source, uint32_t k, int method, uint3
  *   [iteration, top-1 dist, K-th dist, K-th/top-1, new keys that beat the
  *    queue's last key in the iteration, best initial dist, query norm, unchecked entries]
  * into feat[(q*trace_cap + s)*8 ...] and an order-free hash of the top-K id
- * set into topk_hash[q*trace_cap + s]; niter[q] receives the iteration count.
- * A step is labelled "final" when its hash equals the last step's.
+ * set into topk_hash[q*trace_cap + s]; niter[q] receives the iteration count,
+ * and the entries of steps s >= niter[q] are zero.  A step is labelled
+ * "final" when its hash equals the last step's.
  *
  * panns_set_stop_model: the 8-64-64-1 MLP (ReLU, ReLU, sigmoid) evaluated by
  * one wavefront after every iteration when panns_params.stop = LEARNED; the

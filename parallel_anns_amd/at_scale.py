@@ -106,6 +106,7 @@ def main():
                 "hbm_frac": round(B / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "launches": len(nb),
                 "table_eighths": opt.table_eighths, "concurrent": bool(opt.concurrent),
                 "rows_per_wave": opt.rows_per_wave, "overflow_slots": opt.overflow_slots,
+                "bucket_max_L": [int(opt.bucket_max_L[i]) for i in range(max(1, opt.n_buckets))],
                 "tune_and_time_s": round(time.time() - t0, 1)}
         log(f"{name}: {info}")
         return res, rec, info
@@ -154,8 +155,9 @@ def main():
     target = rf - args.target_drop
     best = (ms_f, None, {})
     # sparser model evaluation first; a denser one (or the model at all, over
-    # the fixed search) is kept only when 2 % faster, as in bench.py
-    for every in (4, 2, 1):
+    # the fixed search) is kept only when 2 % faster, as in bench.py.  Long
+    # queues run hundreds of iterations, so strides of 16 and 8 are tried too
+    for every in (16, 8, 4, 2, 1):
         kw = dict(stop_min_iter=min_iter, stop_every=every)
         thr = learned.calibrate(idx, vq, vgt, K, vmax, M, target, L_per_query=vL, **kw)
         if thr is None:

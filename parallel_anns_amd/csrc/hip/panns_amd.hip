@@ -277,9 +277,12 @@ bool kernel_for_dim(uint32_t dim, bool learned, bool norm, bool trace, bool defe
 #define PANNS_DIM(DD)                                                                                   \
     case DD:                                                                                            \
         if (defer) {                                                                                    \
-            *fn = norm ? launch_search<DD, false, true, false, true> : launch_search<DD, false, false, false, true>; \
-            *kp = norm ? reinterpret_cast<const void *>(&panns::search_kernel<DD, false, true, false, true>)         \
-                       : reinterpret_cast<const void *>(&panns::search_kernel<DD, false, false, false, true>);       \
+            *fn = norm      ? launch_search<DD, false, true, false, true>                               \
+                  : learned ? launch_search<DD, true, false, false, true>                               \
+                            : launch_search<DD, false, false, false, true>;                             \
+            *kp = norm      ? reinterpret_cast<const void *>(&panns::search_kernel<DD, false, true, false, true>) \
+                  : learned ? reinterpret_cast<const void *>(&panns::search_kernel<DD, true, false, false, true>) \
+                            : reinterpret_cast<const void *>(&panns::search_kernel<DD, false, false, false, true>); \
             return true;                                                                                \
         }                                                                                               \
         if (trace) {                                                                                    \
@@ -685,17 +688,18 @@ int plan_geometry(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq,
     const uint32_t Lcap = (pl->maxL + 7u) & ~7u;
     const uint32_t ovf = pl->shared_bitmap ? 0u : pl->overflow_slots;
     // the overflow entries are 32-bit, i.e. two 16-bit slots each in the carving
-    // deferred merge: the fixed-stop one-wave search without traces or a
-    // caller-held queue, at most 64 picks per iteration
-    const bool can_defer = !pl->pss && W == 1 && p.stop == PANNS_STOP_FIXED && !pl->trace_cap && !pl->init_queue.p &&
-                           !pl->shared_bitmap && p.M <= 64;
+    // deferred merge: the one-wave search (fixed or learned stop) without
+    // traces or a caller-held queue, at most 64 picks per iteration
+    const bool can_defer = !pl->pss && W == 1 && !pl->trace_cap && !pl->init_queue.p && !pl->shared_bitmap &&
+                           p.M <= 64;
     pl->pend_cap = can_defer && (pl->deferred_merge == 2 || (pl->deferred_merge == 0 && pl->maxL > kDeferMinL))
                        ? kPendCap : 0u;
     pl->lds = pl->pss ? lds_bytes_pss(Lcap, nb_cap, hs + 2u * ovf, h->dim, p.num_threads)
               : W > 1 ? lds_bytes_mw(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim, W)
                       : lds_bytes(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim) + 8u * pl->pend_cap;
     if (pl->pend_cap)
-        select_kernel(h->dim, false, norm, false, 1u, false, true, &fn, &kp);  // the deferred-merge variant
+        select_kernel(h->dim, p.stop == PANNS_STOP_LEARNED, norm, false, 1u, false, true, &fn,
+                      &kp);  // the deferred-merge variant
     if (pl->lds > 65536) {
         HIP_TRY(hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl->lds));
     }
@@ -1948,6 +1952,14 @@ int search_trace(panns_index *h, const panns_params *p, const uint32_t *query_id
     }
     rc = build_plan(pl.get(), query_ids, nullptr, nullptr, init_ids, n_init);
     if (rc) return rc;
+    // entries past a query's last iteration read as zeros
+    const size_t cells = (size_t)nq * trace_cap;
+    HIP_TRY(hipMemsetAsync(pl->trace_feat.p, 0, cells * panns::kFeat * 4, h->stream));
+    HIP_TRY(hipMemsetAsync(pl->trace_topk.p, 0, cells * 4, h->stream));
+    if (gt_ids) {
+        HIP_TRY(hipMemsetAsync(pl->trace_hits.p, 0, cells * 2, h->stream));
+        HIP_TRY(hipMemsetAsync(pl->trace_ndist.p, 0, cells * 4, h->stream));
+    }
     float ms = 0.f;
     rc = run_plan(pl.get(), 1, &ms);
     if (rc) return rc;

@@ -1143,12 +1143,23 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
 
         uint32_t k = 0;
         uint32_t Mcur = (a.mode == kModeScaleM) ? 1u : (a.mode == kModePssV5 ? 1u : a.M);
-        // deferred merge: the fixed-stop search without traces, picks of at
-        // most 64, a full queue (the stop features need the exact queue)
-        static_assert(!kDefer || (!kLearned && !kTrace), "the deferred merge is a fixed-L search variant");
+        // deferred merge: picks of at most 64, a full queue, no traces (they
+        // need the exact queue after every iteration).  The learned stop
+        // needs it only where the model runs: the pending keys are merged
+        // before such an iteration's pick and its survivors go straight into
+        // the queue, so the bound that counts the new keys, and the queue the
+        // features are read from, are those of the search without deferral.
+        static_assert(!kDefer || !kTrace, "traces need the exact queue after every iteration");
         const bool defer = kDefer && S == L;
         uint32_t SP = 0;
         for (;;) {
+            const bool eval_iter = kLearned && niter + 1u == next_eval;
+            if (kLearned && defer && eval_iter && SP > 0) {
+                const uint32_t p0 = merge_run(Q, S, L, P, SP, lane);
+                SP = 0;
+                if (p0 != kSent) k = min(k, p0);
+                __syncthreads();
+            }
             uint32_t m = 0, last_k = 0, pos = k;
             uint32_t nk = kSent, k_next = 0;
             bool picked = false;
@@ -1328,7 +1339,7 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                     const uint32_t cnt = min(64u, ncand - b0);
                     uint64_t x = (uint32_t)lane < cnt ? cand[b0 + lane] : kKeyMax;
                     x = sort64_first(x, cnt, lane, xs);
-                    if (defer) {
+                    if (defer && !eval_iter) {
                         // into P, which is merged into Q first when full
                         if (SP + cnt > a.pend_cap) {
                             const uint32_t p0 = merge_run(Q, S, L, P, SP, lane);

@@ -70,3 +70,27 @@ def test_deferred_with_spills_modes_and_buckets(small_set, gpu_index):
         pl.close()
     for f in ("ids", "ndist", "niter"):
         assert (res[0][f] == res[1][f]).all(), f
+
+
+def test_deferred_learned_stop_equals_every_step_merge(small_set, gpu_index):
+    """The learned-stop search with the deferred merge: the pending keys are
+    merged before every iteration where the model runs, so its features, its
+    stops and all results equal those of the search that merges every
+    iteration -- at every evaluation stride and with spills."""
+    from parallel_anns_amd import learned
+    ds = small_set
+    q = np.arange(ds["queries"].shape[0], dtype=np.uint32)
+    K, L, M = 100, 600, 4
+    feat, hsh, niter, _ = gpu_index.search_trace(q, K, L, M, trace_cap=256)
+    X, Y = learned.samples_from_trace(feat, hsh, niter)
+    P, _ = learned.train(gpu_index, X, Y, epochs=2, batch=1024)
+    gpu_index.set_stop_model(P)
+    stops = 0
+    for thr, every, mi, hs in ((0.3, 1, 0, 0), (0.5, 4, 20, 0), (0.5, 3, 7, 64), (0.7, 2, 0, 0)):
+        print("case deferred learned", thr, every, mi, hs, flush=True)
+        kw = dict(stop="learned", stop_threshold=thr, stop_every=every, stop_min_iter=mi, hash_slots=hs)
+        a = _run(gpu_index, q, K, L, M, 2, **kw)
+        b = _run(gpu_index, q, K, L, M, 1, **kw)
+        _same(a, b)
+        stops += int((a["niter"] < niter).sum())
+    assert stops > 0  # the model stopped some queries early
