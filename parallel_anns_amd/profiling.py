@@ -97,7 +97,9 @@ def report_uneven(a):
     cw = [r[2] for r in counter_values(a.cal_write, "WRITE_SIZE", "copy_kernel")]
     ff = a.copy_bytes / (sum(cf) / len(cf) * 1024)
     wf = a.copy_bytes / (sum(cw) / len(cw) * 1024)
-    match = f"search_kernel<{a.dim}, false, false, false>"
+    # the fixed-L squared-L2 instantiations, with and without the deferred
+    # merge: search_kernel<D, learned = false, norm = false, trace = false, defer>
+    match = f"search_kernel<{a.dim}, false, false, false, "
     tr = _by_run(_dispatches(a.trace, "kernels", match), nb, steps, bk)
     fe = _by_run(_dispatches(a.fetch, "counters_collection", match, "where counter_name='FETCH_SIZE'"), nb, steps, bk)
     wr = _by_run(_dispatches(a.write, "counters_collection", match, "where counter_name='WRITE_SIZE'"), nb, steps, bk)
@@ -173,16 +175,18 @@ def report(a):
         if name not in run:
             continue
         r = run[name]
-        # squared-L2 search instantiation: search_kernel<D, learned, norm = false, trace = false>
-        match = f"search_kernel<{dim}, {variant}, false, false>"
-        d = [x[1] / 1e3 for x in kernel_durations(a.trace, match)][-steps:]
+        # squared-L2 search instantiation: search_kernel<D, learned, norm = false, trace = false, defer>
+        match = f"search_kernel<{dim}, {variant}, false, false, "
+        kd = kernel_durations(a.trace, match)
+        shown = kd[-1][0].split("(")[0].replace("void panns::", "") if kd else match
+        d = [x[1] / 1e3 for x in kd][-steps:]
         fv = [x[2] for x in counter_values(a.fetch, "FETCH_SIZE", match)][-steps:]
         wv = [x[2] for x in counter_values(a.write, "WRITE_SIZE", match)][-steps:]
         mean_us = sum(d) / len(d)
         fk, wk = sum(fv) / len(fv), sum(wv) / len(wv)
         traffic = fk * 1024 * ff + wk * 1024 * wf
         alg = r["algorithmic_bytes_per_launch"]
-        lines.append(f"| {match} | {r['M']} | {r['L']} | {r['table_eighths']} | {mean_us:.1f} | {r['kernel_ms']} | "
+        lines.append(f"| {shown} | {r['M']} | {r['L']} | {r['table_eighths']} | {mean_us:.1f} | {r['kernel_ms']} | "
                      f"{fk:,.0f} | {wk:,.0f} | {traffic / 1e9:.3f} | {alg / 1e9:.3f} | {traffic / alg:.3f} | "
                      f"{alg / (mean_us * 1e-6) / 1e9:,.1f} |")
         rec = {"config": run["config"], "kernel": f"search_kernel<{dim}, {variant}>", "M": r["M"], "L": r["L"],
