@@ -398,7 +398,7 @@ def secondary_line(args, name, target, steps, warmup):
     best = (ms_fixed, fL, learned.FIXED_FALLBACK, {})
     for f in (float(x) for x in args.learned_L.split(",")):
         lL = min(4096, int(round(fL * f)))
-        for every in (4, 2, 1):
+        for every in (8, 4, 2, 1):
             kw = dict(stop_min_iter=min_iter, stop_every=every)
             thr = learned.calibrate(index, vq, vgt, K, lL, M, target + margin, **kw)
             if thr is None:
@@ -586,7 +586,7 @@ def main():
                 # candidates are compared by validation device time (the
                 # model's own cost included); the model runs from the
                 # iteration by which 1 % of training queries had their final
-                # top-K, at every fourth, every second or every iteration --
+                # top-K, at every eighth, fourth, second or every iteration --
                 # tried in that order, so a more frequent evaluation must pay
                 # for itself clearly (the 2 % rule below) to be chosen
                 _, _, ms_fixed = learned.recall_work(index, vq, vgt, K, Lf, lM, None, runs=3)
@@ -599,7 +599,7 @@ def main():
                 log(f"target {t}: calibration margin {margin:.4f}")
                 for f in (float(x) for x in args.learned_L.split(",")):
                     lL = min(4096, int(round(Lf * f)))
-                    for every in (4, 2, 1):
+                    for every in (8, 4, 2, 1):
                         kw = dict(stop_min_iter=min_iter, stop_every=every)
                         thr = learned.calibrate(index, vq, vgt, K, lL, lM, t + margin, **kw)
                         if thr is None:

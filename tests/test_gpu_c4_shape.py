@@ -1,0 +1,74 @@
+"""GPU parity at the 50M at-scale workload's shape: 50,000,000 x 96 rows of
+the 1,024-component mixture (19.2 GB), so ids need 26 bits -- the visited
+tables switch to 4-slot buckets at 8,224 slots, spills go to 50M-bit
+bitmaps -- and queries with L log-uniform in [100, 2000], K = 100, in six
+queue-size buckets side by side (deferred merge above L = 1,000, one
+hipGraph) and in one launch sized for L = 2,000.
+
+The graph is a seeded random one (out-degree 8 to 32, rows padded to 32
+slots) rather than the configured k-NN graph, whose construction takes
+minutes at this size: parity needs the kernels' id width, table shapes,
+spill path, queue lengths and merges, not a good graph.  The two launches
+must agree on every query, and a sample (the 20 largest L and every 100th
+query) is checked against the oracle at each query's own L: ids, distance
+bits, distance counts and iterations."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from parallel_anns_amd import datasets, tools
+from parallel_anns_amd.search import Index
+
+pytestmark = pytest.mark.gpu
+
+
+def test_c4_shape_uneven_buckets_against_oracle():
+    cfg = datasets.CONFIGS["c4"]
+    n, nq, R = cfg["n"], 2000, 32
+    print("case c4 shape: generate 50M x 96", flush=True)
+    base = tools.gen_mixture(n, cfg["dim"], cfg["comps"], cfg["sigma"], 0, 1)
+    queries = datasets.generate(cfg, "query", n=nq)
+    rng = np.random.default_rng(41)
+    deg = rng.integers(8, R + 1, n, dtype=np.uint32)
+    adj = rng.integers(0, n, (n, R), dtype=np.uint32)
+    adj[np.arange(R)[None, :] >= deg[:, None]] = 0xFFFFFFFF
+    ep = int(rng.integers(0, n))
+    Ls, Ks = datasets.uneven_L_K(cfg, nq)
+    q = np.arange(nq, dtype=np.uint32)
+    idx = Index(0)
+    try:
+        idx.set_base(base)
+        idx.set_queries(queries)
+        idx.set_graph(adj, deg, ep)
+        maxL = int(Ls.max())
+        init = idx.prepare_init_ids(maxL)
+        res = []
+        for opt in (Index.plan_options(bucket_max_L=[417, 734, 1049, 1367, 1683, maxL], table_eighths=0,
+                                       use_graph=True, concurrent=True, rows_per_wave=64),
+                    Index.plan_options(table_eighths=0)):
+            print("case c4 shape launch", opt.n_buckets, "buckets", flush=True)
+            pl = idx.plan(q, 100, maxL, 4, init_ids=init, L_per_query=Ls, K_per_query=Ks, options=opt,
+                          want_queue=False)
+            bk = pl.buckets()
+            assert all(b["hash_slots"] == 8224 for b in bk), bk
+            pl.run()
+            pl.run()
+            res.append(pl.fetch())
+            pl.close()
+    finally:
+        idx.close()
+    a, b = res
+    for f in ("ids", "ndist", "niter", "nexp"):
+        assert (a[f] == b[f]).all(), f
+    assert (a["dists"].view(np.uint32) == b["dists"].view(np.uint32)).all()
+    assert a["counters"]["visited_spills"] > 0  # the 50M-bit bitmaps were used
+    print("case c4 shape: oracle", flush=True)
+    g = po.CsrGraph(adj, deg, ep)
+    assert (po.prepare_init_ids(g, maxL) == init).all()
+    sample = np.concatenate([np.argsort(Ls, kind="stable")[-20:], np.arange(0, nq, 100)])
+    for i in sample:
+        L, K = int(Ls[i]), int(Ks[i])
+        r = po.search_one(g, base, queries[i], K, L, 4, init[:L])
+        assert (a["ids"][i, :K] == r["ids"]).all(), (i, L)
+        assert (a["dists"][i, :K].view(np.uint32) == r["queue"]["dist"][:K].view(np.uint32)).all(), i
+        assert a["ndist"][i] == r["ndist"] and a["niter"][i] == r["merges"], i
