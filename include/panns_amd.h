@@ -297,20 +297,20 @@ int panns_plan_autotune(panns_index *h, const panns_params *p, const uint32_t *q
 int panns_exact_knn(panns_index *h, int source, uint32_t k, uint32_t *out_ids, float *out_dists);
 
 typedef enum {
-    PANNS_KNN_AUTO = 0,        /* the faster of the two on this hardware: EXACT_VALU */
+    PANNS_KNN_AUTO = 0,        /* MFMA from 65,536 base rows, EXACT_VALU below */
     PANNS_KNN_EXACT_VALU = 1,  /* the exact distance chain for every pair */
-    PANNS_KNN_MFMA = 2         /* FP32 MFMA candidates, exact re-rank, exact recompute where unproven */
+    PANNS_KNN_MFMA = 2         /* FP32 MFMA filter with a proven bound, exact re-rank of the candidates */
 } panns_knn_method;
 
 /* panns_exact_knn with a choice of method; all give the same ids and
- * distances.  MFMA (for k <= 120) keeps k + 28 candidates per query by
- * |v|^2 - 2 q.v from FP32 matrix instructions, recomputes their distances
- * with the reference's chain, and proves the result with an fp32 error bound
- * on the candidate margin; queries the bound does not cover are recomputed
- * with the exact chain over every row (their count in *n_recomputed,
- * nullable).  Measured on MI355X it is about 1.25x slower than EXACT_VALU
- * at 1M x 96: the kernel time goes to tile staging, synchronisation and
- * selection, not to the arithmetic the matrix cores take over. */
+ * distances.  MFMA: the exact chain over a sample of the rows (the first
+ * max(4k + 64, min(32768, n / 8))) bounds each query's k-th distance; FP32
+ * matrix instructions then estimate |v|^2 - 2 q.v for every row and keep the
+ * rows an fp32 error bound cannot exclude from that distance; their
+ * distances are recomputed with the reference's chain and the k smallest
+ * kept.  Queries whose candidate list overflows (or where the sample is too
+ * small to use) are recomputed with the exact chain over every row (their
+ * count in *n_recomputed, nullable). */
 int panns_exact_knn_ex(panns_index *h, int source, uint32_t k, int method, uint32_t *out_ids, float *out_dists,
                        uint32_t *n_recomputed);
 

@@ -93,6 +93,9 @@ constexpr uint32_t kRanges = 8;         // work ranges (one per XCD)
 constexpr uint32_t kPivots = 256;       // locality pivots
 constexpr uint32_t kLocalityMin = 512;  // smallest batch that is reordered
 constexpr uint32_t kPendCap = 256;      // pending keys of the deferred merge
+// exact k-NN: the automatic method takes the matrix-instruction path from
+// this many base rows
+constexpr uint32_t kKnnMfmaMinRows = 65536;
 // automatic deferred merge above this L: measured on 1M x 96 (M = 4) it
 // loses 8-19 % at L = 720 and 900, where its 2 KB of LDS cost a workgroup
 // per CU, and gains 2 % at L = 1100 and 7 % at 1400; on 50M x 96 it gains
@@ -368,8 +371,16 @@ bool select_kernel(uint32_t dim, bool learned, bool norm, bool trace, uint32_t w
 
 using KnnLaunch = void (*)(const float *, uint32_t, const float *, uint32_t, uint32_t, int, uint32_t,
                           const uint32_t *, uint32_t *, float *, uint32_t, uint32_t, hipStream_t);
-using KnnMfmaLaunch = void (*)(const float *, const float *, uint32_t, const float *, uint32_t, uint32_t, uint32_t,
-                              int, uint32_t, float, uint32_t *, float *, uint32_t *, uint32_t, uint32_t, hipStream_t);
+// The three launches of the matrix-instruction k-NN (knn_kernel.h): per-query
+// bounds from a sample's exact k-th distance, the filter over every row, the
+// exact re-rank of the candidates.
+struct KnnMfmaLaunch {
+    void (*bound)(const float *, uint32_t, const float *, uint32_t, float, float *, hipStream_t);
+    void (*filter)(const float *, const float *, uint32_t, const float *, uint32_t, const float *, int, uint32_t,
+                   uint32_t, uint32_t *, uint32_t, uint32_t *, uint32_t, hipStream_t);
+    void (*rerank)(const float *, const float *, uint32_t, uint32_t, const uint32_t *, uint32_t, const uint32_t *,
+                   uint32_t *, float *, uint32_t *, hipStream_t);
+};
 
 template <int D>
 void launch_knn(const float *b, uint32_t n, const float *q, uint32_t nq, uint32_t k, int ex, uint32_t qoff,
@@ -379,12 +390,27 @@ void launch_knn(const float *b, uint32_t n, const float *q, uint32_t nq, uint32_
 }
 
 template <int D>
-void launch_knn_mfma(const float *b, const float *norms, uint32_t n, const float *q, uint32_t nq, uint32_t kc,
-                     uint32_t k, int ex, uint32_t qoff, float vmax2, uint32_t *oi, float *od, uint32_t *fl,
-                     uint32_t grid, uint32_t lds, hipStream_t s)
+void launch_knn_bound(const float *q, uint32_t nq, const float *sd, uint32_t k, float vmax2, float *bound,
+                      hipStream_t s)
 {
-    hipLaunchKernelGGL(panns::knn_mfma_kernel<D>, dim3(grid), dim3(256), lds, s, b, norms, n, q, nq, kc, k, ex, qoff,
-                       vmax2, oi, od, fl);
+    hipLaunchKernelGGL(panns::knn_bound_kernel<D>, dim3((nq + 255) / 256), dim3(256), 0, s, q, nq, sd, k, vmax2,
+                       bound);
+}
+
+template <int D>
+void launch_knn_filter(const float *b, const float *norms, uint32_t n, const float *q, uint32_t nq,
+                       const float *bound, int ex, uint32_t qoff, uint32_t rows_per_split, uint32_t *cand,
+                       uint32_t cap, uint32_t *cnt, uint32_t splits, hipStream_t s)
+{
+    hipLaunchKernelGGL(panns::knn_filter_kernel<D>, dim3((nq + 63) / 64, splits), dim3(256), 0, s, b, norms, n, q,
+                       nq, bound, ex, qoff, rows_per_split, cand, cap, cnt);
+}
+
+template <int D>
+void launch_knn_rerank(const float *b, const float *q, uint32_t nq, uint32_t k, const uint32_t *cand, uint32_t cap,
+                       const uint32_t *cnt, uint32_t *oi, float *od, uint32_t *fl, hipStream_t s)
+{
+    hipLaunchKernelGGL(panns::knn_rerank_kernel<D>, dim3(nq), dim3(64), 0, s, b, q, k, cand, cap, cnt, oi, od, fl);
 }
 
 // PSS5.cpp:293-334 compute_norm for every base row, once per base set.
@@ -2219,16 +2245,15 @@ int panns_exact_knn_ex(panns_index *h, int source, uint32_t k, int method, uint3
     const uint32_t nq = source == 0 ? h->n : h->nq;
     const int ex = source == 0 ? 1 : 0;
     if (n_recomputed) *n_recomputed = 0;
-    const void *kp = nullptr, *kpm = nullptr;
+    const void *kp = nullptr;
     KnnLaunch launch = nullptr;
-    KnnMfmaLaunch launch_m = nullptr;
+    KnnMfmaLaunch mf{};
     switch (h->dim) {
 #define PANNS_KNN_DIM(DD)                                                                                 \
     case DD:                                                                                              \
         kp = reinterpret_cast<const void *>(&panns::knn_kernel<DD>);                                      \
-        kpm = reinterpret_cast<const void *>(&panns::knn_mfma_kernel<DD>);                                \
         launch = launch_knn<DD>;                                                                          \
-        launch_m = launch_knn_mfma<DD>;                                                                   \
+        mf = {launch_knn_bound<DD>, launch_knn_filter<DD>, launch_knn_rerank<DD>};                        \
         break;
         PANNS_KNN_DIM(16)
         PANNS_KNN_DIM(32)
@@ -2243,15 +2268,15 @@ int panns_exact_knn_ex(panns_index *h, int source, uint32_t k, int method, uint3
     const uint32_t D = h->dim, RS = D + 4u;
     const uint32_t lds = 8u * (panns::kKnnQT * panns::kKnnKCap + panns::kKnnQT * 64u + 4u * 64u) +
                          4u * panns::kKnnRT * RS + 4u * 2u * panns::kKnnQT + 8u * panns::kKnnQT;
-    const uint32_t lds_m = 8u * (panns::kKnnQT * panns::kKnnKCap + panns::kKnnQT * 64u + 4u * 64u + panns::kKnnQT) +
-                           4u * (panns::kKnnRT * RS + panns::kKnnQT * RS + (panns::kKnnQT + 1u) * panns::kKnnRT +
-                                 panns::kKnnRT) +
-                           4u * 2u * panns::kKnnQT;
     HIP_TRY(hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    HIP_TRY(hipFuncSetAttribute(kpm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_m));
-    // candidates: k plus a margin, at most the list capacity
-    const uint32_t kc = std::min<uint32_t>((uint32_t)panns::kKnnKCap, k + 28u);
-    const bool use_mfma = method == PANNS_KNN_MFMA && kc >= k + 8u;
+    // Matrix-instruction path: a sample of S rows bounds each query's k-th
+    // distance; about k * n / S rows per query pass the filter, and a list of
+    // four times that (at least 1,024) overflows rarely.  It needs the
+    // sample to be a small share of the rows.
+    const uint32_t S = std::max<uint32_t>(4u * k + 64u, std::min<uint32_t>(32768u, h->n / 8u));
+    const uint32_t cap = std::min<uint32_t>(16384u, std::max<uint32_t>(1024u, 4u * k * ((h->n + S - 1u) / S)));
+    const bool use_mfma = (method == PANNS_KNN_MFMA || (method == PANNS_KNN_AUTO && h->n >= kKnnMfmaMinRows)) &&
+                          h->n >= 2u * S;
     float vmax2 = 0.f;
     if (use_mfma) {
         int rc = ensure_norms(h);
@@ -2260,24 +2285,46 @@ int panns_exact_knn_ex(panns_index *h, int source, uint32_t k, int method, uint3
         HIP_TRY(hipMemcpy(nv.data(), h->norms.p, (size_t)h->n * 4, hipMemcpyDeviceToHost));
         for (float x : nv) vmax2 = std::max(vmax2, x);
     }
-    // launches of at most 64k queries keep each kernel short
-    const uint32_t chunk = 65536;
-    DevBuf<uint32_t> di, df, dsel;
-    DevBuf<float> dd, dq;
+    // launches of at most 64k queries (16k with candidate lists of up to
+    // 64 KB each) keep each kernel short and the lists within 1 GiB
+    const uint32_t chunk = use_mfma ? std::min<uint32_t>(65536u, std::max<uint32_t>(64u, (1u << 28) / cap)) : 65536u;
+    DevBuf<uint32_t> di, df, dsel, dsi, dcnt, dcand;
+    DevBuf<float> dd, dq, dsd, dbound;
     HIP_TRY(di.ensure((size_t)std::min(nq, chunk) * k));
     HIP_TRY(dd.ensure((size_t)std::min(nq, chunk) * k));
     HIP_TRY(df.ensure(std::min(nq, chunk)));
+    if (use_mfma) {
+        HIP_TRY(dsi.ensure((size_t)std::min(nq, chunk) * k));
+        HIP_TRY(dsd.ensure((size_t)std::min(nq, chunk) * k));
+        HIP_TRY(dbound.ensure(std::min(nq, chunk)));
+        HIP_TRY(dcnt.ensure(std::min(nq, chunk)));
+        HIP_TRY(dcand.ensure((size_t)std::min(nq, chunk) * cap));
+    }
     std::vector<uint32_t> flags, redo;
     for (uint32_t q0 = 0; q0 < nq; q0 += chunk) {
         const uint32_t cn = std::min(chunk, nq - q0);
         const uint32_t grid = (cn + panns::kKnnQT - 1) / panns::kKnnQT;
+        const float *qc = qsrc + (size_t)q0 * D;
         // q0 offsets the chunk's query index so that exclude_self compares
         // base-row ids with global query ids.
-        if (use_mfma)
-            launch_m(h->base.p, h->norms.p, h->n, qsrc + (size_t)q0 * D, cn, kc, k, ex, q0, vmax2, di.p, dd.p, df.p,
-                     grid, lds_m, h->stream);
-        else
-            launch(h->base.p, h->n, qsrc + (size_t)q0 * D, cn, k, ex, q0, nullptr, di.p, dd.p, grid, lds, h->stream);
+        if (use_mfma) {
+            launch(h->base.p, S, qc, cn, k, ex, q0, nullptr, dsi.p, dsd.p, grid, lds, h->stream);
+            HIP_TRY(hipGetLastError());
+            mf.bound(qc, cn, dsd.p, k, vmax2, dbound.p, h->stream);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemsetAsync(dcnt.p, 0, (size_t)cn * 4, h->stream));
+            // split the rows so that the launch has at least 2,048 workgroups
+            const uint32_t qblocks = (cn + 63u) / 64u;
+            const uint32_t want = std::max<uint32_t>(1u, (2048u + qblocks - 1u) / qblocks);
+            const uint32_t per = std::max<uint32_t>(4096u, ((h->n + want - 1u) / want + 63u) & ~63u);
+            const uint32_t splits = (h->n + per - 1u) / per;
+            mf.filter(h->base.p, h->norms.p, h->n, qc, cn, dbound.p, ex, q0, per, dcand.p, cap, dcnt.p, splits,
+                      h->stream);
+            HIP_TRY(hipGetLastError());
+            mf.rerank(h->base.p, qc, cn, k, dcand.p, cap, dcnt.p, di.p, dd.p, df.p, h->stream);
+        } else {
+            launch(h->base.p, h->n, qc, cn, k, ex, q0, nullptr, di.p, dd.p, grid, lds, h->stream);
+        }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(out_ids + (size_t)q0 * k, di.p, (size_t)cn * k * 4, hipMemcpyDeviceToHost, h->stream));
         if (out_dists)
@@ -2292,8 +2339,8 @@ int panns_exact_knn_ex(panns_index *h, int source, uint32_t k, int method, uint3
             if (flags[i]) redo.push_back(q0 + i);
     }
     if (!redo.empty()) {
-        // the queries whose candidate margin did not prove the result: the
-        // exact kernel on their rows, own ids excluded as before
+        // the queries whose candidate list overflowed: the exact kernel on
+        // every row, own ids excluded as before
         if (n_recomputed) *n_recomputed = (uint32_t)redo.size();
         for (size_t r0 = 0; r0 < redo.size(); r0 += chunk) {
             const uint32_t cn = (uint32_t)std::min<size_t>(chunk, redo.size() - r0);

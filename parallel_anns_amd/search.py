@@ -359,10 +359,11 @@ class Index:
         """Exact k-NN on the GPU (reference distance, ties to the lower id).
         source="queries": of the loaded queries (ground truth); "base": of
         every base row with itself excluded (graph k-NN lists).  method
-        "valu": the exact chain for every pair ("auto" picks it, the faster
-        here); "mfma": FP32 MFMA candidates with an exact, proven re-rank.
-        stats (a dict) receives the number of queries the re-rank could not
-        prove and recomputed exactly."""
+        "valu": the exact chain for every pair; "mfma": an FP32 MFMA filter
+        bounded by the exact k-th distance over a sample of the rows, then
+        the exact chain on the candidates; "auto": mfma from 65,536 base
+        rows.  stats (a dict) receives the number of queries recomputed with
+        the exact chain over every row (candidate lists that overflowed)."""
         info = self.info()
         nq = info["num_queries"] if source == "queries" else info["num_v"]
         ids = np.empty((nq, k), np.uint32)

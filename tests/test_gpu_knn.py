@@ -65,8 +65,8 @@ def test_mfma_f32_layout():
                                          (6000, 40, 50, "queries"), (4000, 128, 120, "base"),
                                          (3000, 16, 10, "queries")])
 def test_mfma_candidates_equal_exact_chain(n, dim, k, src):
-    """MFMA candidates + exact re-rank (+ exact recompute where the margin
-    does not prove the result) give the exact kernel's ids and distance bits."""
+    """The MFMA filter bounded by a sample's exact k-th distance + the exact
+    re-rank of its candidates give the exact kernel's ids and distance bits."""
     print(f"case knn mfma n={n} dim={dim} k={k} {src}", flush=True)
     base = tools.gen_mixture(n, dim, 16, 0.3, 0, 41)
     queries = tools.gen_mixture(300, dim, 16, 0.3, 0, 42)
@@ -84,10 +84,10 @@ def test_mfma_candidates_equal_exact_chain(n, dim, k, src):
 
 
 def test_mfma_candidates_with_exact_ties():
-    """Every row repeated 100 times: the k-th and the last candidate share one
-    distance, so the margin cannot prove the result; those queries are
-    recomputed with the exact chain and the ids still equal the oracle's
-    (ties to the lower id).  Integer rows keep every distance exact."""
+    """Every row repeated 100 times: every copy of a row within the bound is a
+    candidate, and the exact re-rank orders equal distances by id, so the ids
+    equal the oracle's (ties to the lower id).  Integer rows keep every
+    distance exact."""
     rng = np.random.default_rng(9)
     uniq = rng.integers(-2, 3, (30, 32)).astype(np.float32)
     base = np.tile(uniq, (100, 1))
@@ -100,5 +100,25 @@ def test_mfma_candidates_with_exact_ties():
     gi, gd = idx.exact_knn(50, source="queries", method="mfma", stats=st)
     idx.close()
     oi, od = po.brute_force_knn(base, queries, 50)
+    assert (gi == oi).all() and (gd.view(np.uint32) == od.view(np.uint32)).all()
+
+
+def test_mfma_overflowing_lists_fall_back_to_exact_chain():
+    """3,000 copies of each of 4 rows: every copy of the nearest row passes
+    the filter, more than a candidate list holds, so those queries are
+    recomputed with the exact chain over every row; ids and distances still
+    equal the oracle's."""
+    rng = np.random.default_rng(13)
+    uniq = rng.integers(-3, 4, (4, 32)).astype(np.float32)
+    base = np.tile(uniq, (3000, 1))
+    queries = rng.integers(-3, 4, (40, 32)).astype(np.float32)
+    idx = Index(0)
+    idx.set_base(base)
+    idx.set_queries(queries)
+    st = {}
+    print("case knn mfma overflow", flush=True)
+    gi, gd = idx.exact_knn(10, source="queries", method="mfma", stats=st)
+    idx.close()
+    oi, od = po.brute_force_knn(base, queries, 10)
     assert (gi == oi).all() and (gd.view(np.uint32) == od.view(np.uint32)).all()
     assert st["recomputed"] > 0
