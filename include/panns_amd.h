@@ -297,7 +297,7 @@ int panns_plan_autotune(panns_index *h, const panns_params *p, const uint32_t *q
 int panns_exact_knn(panns_index *h, int source, uint32_t k, uint32_t *out_ids, float *out_dists);
 
 typedef enum {
-    PANNS_KNN_AUTO = 0,        /* MFMA from 65,536 base rows, EXACT_VALU below */
+    PANNS_KNN_AUTO = 0,        /* MFMA from 8,192 base rows, EXACT_VALU below */
     PANNS_KNN_EXACT_VALU = 1,  /* the exact distance chain for every pair */
     PANNS_KNN_MFMA = 2         /* FP32 MFMA filter with a proven bound, exact re-rank of the candidates */
 } panns_knn_method;

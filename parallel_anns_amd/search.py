@@ -361,7 +361,7 @@ class Index:
         every base row with itself excluded (graph k-NN lists).  method
         "valu": the exact chain for every pair; "mfma": an FP32 MFMA filter
         bounded by the exact k-th distance over a sample of the rows, then
-        the exact chain on the candidates; "auto": mfma from 65,536 base
+        the exact chain on the candidates; "auto": mfma from 8,192 base
         rows.  stats (a dict) receives the number of queries recomputed with
         the exact chain over every row (candidate lists that overflowed)."""
         info = self.info()
