@@ -2273,9 +2273,14 @@ int panns_exact_knn_ex(panns_index *h, int source, uint32_t k, int method, uint3
     // Matrix-instruction path: a sample of S rows bounds each query's k-th
     // distance; about k * n / S rows per query pass the filter, and a list of
     // four times that (at least 1,024) overflows rarely.  It needs the
-    // sample to be a small share of the rows.
+    // sample to be a small share of the rows.  The sample's own bound comes
+    // the same way from a first sample of S1 rows when S is at least 4 S1:
+    // the exact chain over S1 rows, then the filter and re-rank over S.
     const uint32_t S = std::max<uint32_t>(4u * k + 64u, std::min<uint32_t>(32768u, h->n / 8u));
-    const uint32_t cap = std::min<uint32_t>(16384u, std::max<uint32_t>(1024u, 4u * k * ((h->n + S - 1u) / S)));
+    const uint32_t S1 = std::max<uint32_t>(4u * k + 64u, std::min<uint32_t>(4096u, h->n / 64u));
+    const bool two_level = S >= 4u * S1;
+    const uint32_t cap = std::min<uint32_t>(
+        16384u, std::max<uint32_t>({1024u, 4u * k * ((h->n + S - 1u) / S), 4u * k * ((S + S1 - 1u) / S1)}));
     const bool use_mfma = (method == PANNS_KNN_MFMA || (method == PANNS_KNN_AUTO && h->n >= kKnnMfmaMinRows)) &&
                           h->n >= 2u * S;
     float vmax2 = 0.f;
@@ -2309,20 +2314,37 @@ int panns_exact_knn_ex(panns_index *h, int source, uint32_t k, int method, uint3
         // q0 offsets the chunk's query index so that exclude_self compares
         // base-row ids with global query ids.
         if (use_mfma) {
-            launch(h->base.p, S, qc, cn, k, ex, q0, nullptr, dsi.p, dsd.p, grid, lds, h->stream);
-            HIP_TRY(hipGetLastError());
-            mf.bound(qc, cn, dsd.p, k, vmax2, dbound.p, h->stream);
-            HIP_TRY(hipGetLastError());
-            HIP_TRY(hipMemsetAsync(dcnt.p, 0, (size_t)cn * 4, h->stream));
-            // split the rows so that the launch has at least 2,048 workgroups
-            const uint32_t qblocks = (cn + 63u) / 64u;
-            const uint32_t want = std::max<uint32_t>(1u, (2048u + qblocks - 1u) / qblocks);
-            const uint32_t per = std::max<uint32_t>(4096u, ((h->n + want - 1u) / want + 63u) & ~63u);
-            const uint32_t splits = (h->n + per - 1u) / per;
-            mf.filter(h->base.p, h->norms.p, h->n, qc, cn, dbound.p, ex, q0, per, dcand.p, cap, dcnt.p, splits,
-                      h->stream);
-            HIP_TRY(hipGetLastError());
-            mf.rerank(h->base.p, qc, cn, k, dcand.p, cap, dcnt.p, di.p, dd.p, df.p, h->stream);
+            // filter rows [0, rows) against the bounds from sd, re-rank into (oi, od)
+            auto pass = [&](uint32_t rows, const float *sd, uint32_t *oi, float *od) -> int {
+                mf.bound(qc, cn, sd, k, vmax2, dbound.p, h->stream);
+                HIP_TRY(hipGetLastError());
+                HIP_TRY(hipMemsetAsync(dcnt.p, 0, (size_t)cn * 4, h->stream));
+                // split the rows so that the launch has at least 2,048 workgroups
+                const uint32_t qblocks = (cn + 63u) / 64u;
+                const uint32_t want = std::max<uint32_t>(1u, (2048u + qblocks - 1u) / qblocks);
+                const uint32_t per = std::max<uint32_t>(4096u, ((rows + want - 1u) / want + 63u) & ~63u);
+                const uint32_t splits = (rows + per - 1u) / per;
+                mf.filter(h->base.p, h->norms.p, rows, qc, cn, dbound.p, ex, q0, per, dcand.p, cap, dcnt.p, splits,
+                          h->stream);
+                HIP_TRY(hipGetLastError());
+                mf.rerank(h->base.p, qc, cn, k, dcand.p, cap, dcnt.p, oi, od, df.p, h->stream);
+                HIP_TRY(hipGetLastError());
+                return PANNS_OK;
+            };
+            // the sample's exact k-th distances in dsd (a query whose first
+            // pass overflowed gets none, hence an unbounded second pass and,
+            // overflowing again, the exact recompute)
+            if (two_level) {
+                launch(h->base.p, S1, qc, cn, k, ex, q0, nullptr, di.p, dd.p, grid, lds, h->stream);
+                HIP_TRY(hipGetLastError());
+                int rc = pass(S, dd.p, dsi.p, dsd.p);
+                if (rc) return rc;
+            } else {
+                launch(h->base.p, S, qc, cn, k, ex, q0, nullptr, dsi.p, dsd.p, grid, lds, h->stream);
+                HIP_TRY(hipGetLastError());
+            }
+            int rc = pass(h->n, dsd.p, di.p, dd.p);
+            if (rc) return rc;
         } else {
             launch(h->base.p, h->n, qc, cn, k, ex, q0, nullptr, di.p, dd.p, grid, lds, h->stream);
         }

@@ -5,6 +5,7 @@
                                           [--config c1] [--steps 5]
   python -m parallel_anns_amd.profile_run --uneven M:maxL1,maxL2,...[:concurrent[:eighths[:rows[:overflow]]]]
                                           [--config c2] [--steps 5]
+  python -m parallel_anns_amd.profile_run --knn K[,method] [--config c1] [--steps 5]
 
 Loads the prepared workload from the cache (and the stop model bench.py
 saved there), then runs the fixed-L plan `steps + 1` times and, when given,
@@ -13,6 +14,10 @@ the learned-stop plan `steps + 1` times, nothing else, so that the last
 run are exactly the timed launches of the benchmark's two operating points.
 Prints one JSON line with the device times and the algorithmic bytes per
 launch.
+
+With --knn the workload's exact ground truth (K nearest base rows of every
+query; method "mfma" (default) or "valu") is computed `steps + 1` times; the
+JSON line gives each call's wall time and the products' FLOP count.
 
 With --uneven the batch is the workload's uneven one (per-query L
 log-uniform over its L range, K from its K choices, all queries) in the
@@ -36,6 +41,7 @@ def main():
     ap.add_argument("--cache-dir", default=os.environ.get("PANNS_CACHE", "/tmp/panns_cache"))
     ap.add_argument("--op", default=None)
     ap.add_argument("--uneven", default=None)
+    ap.add_argument("--knn", default=None)
     ap.add_argument("--learned-op", default=None)
     ap.add_argument("--steps", type=int, default=5)
     args = ap.parse_args()
@@ -67,6 +73,22 @@ def main():
                 "kernel_ms": round(float(np.mean(ms)), 4),
                 "algorithmic_bytes_per_launch": int(alg)}
 
+    if args.knn:
+        import time
+        f = args.knn.split(",")
+        k, method = int(f[0]), (f[1] if len(f) > 1 else "mfma")
+        walls = []
+        for _ in range(args.steps + 1):
+            st = {}
+            t0 = time.perf_counter()
+            idx.exact_knn(k, source="queries", method=method, stats=st)
+            walls.append(time.perf_counter() - t0)
+        n = ds["base"].shape[0]
+        print(json.dumps({"config": args.config, "knn": k, "method": method, "queries": nq, "rows": n,
+                          "wall_s": [round(w, 4) for w in walls[1:]], "recomputed": st.get("recomputed"),
+                          "product_flops": 2.0 * nq * n * dim}), flush=True)
+        idx.close()
+        return 0
     if args.uneven:
         print(json.dumps(uneven_run(idx, ds, args)), flush=True)
         idx.close()
