@@ -690,7 +690,10 @@ def main():
     if args.secondary != "none" and not args.no_learned and args.config == "c1":
         # after every timing of the headline, on an index of its own
         index.close()
-        secondary = secondary_line(args, args.secondary, target, args.steps, args.warmup)
+        try:
+            secondary = secondary_line(args, args.secondary, target, args.steps, args.warmup)
+        except Exception as e:  # the headline line must still be printed
+            secondary = {"workload": args.secondary, "error": f"{type(e).__name__}: {e}"}
         log(f"secondary: {secondary}")
     kernel = f"search_kernel<{dim}, {'true' if use_lrn and not lrn['fallback_to_fixed'] else 'false'}>"
     traffic, traffic_src = measured_traffic(args.config, kernel, dict(lrn if use_lrn else fixed, algorithmic_bytes=B))
