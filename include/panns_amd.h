@@ -304,11 +304,12 @@ typedef enum {
 
 /* panns_exact_knn with a choice of method; all give the same ids and
  * distances.  MFMA: the exact chain over a sample of the rows (the first
- * max(4k + 64, min(32768, n / 8))) bounds each query's k-th distance; FP32
- * matrix instructions then estimate |v|^2 - 2 q.v for every row and keep the
- * rows an fp32 error bound cannot exclude from that distance; their
- * distances are recomputed with the reference's chain and the k smallest
- * kept.  Queries whose candidate list overflows (or where the sample is too
+ * max(4k + 64, min(4096, n / 8))) bounds each query's k-th distance; FP32
+ * matrix instructions then estimate |v|^2 - 2 q.v for the rows of a larger
+ * prefix and keep the rows an fp32 error bound cannot exclude from that
+ * distance; their distances are recomputed with the reference's chain and
+ * the k smallest kept, which bounds the next, larger prefix, up to all n
+ * rows (prefixes grow by at most 32x).  Queries whose candidate list overflows (or where the sample is too
  * small to use) are recomputed with the exact chain over every row (their
  * count in *n_recomputed, nullable). */
 int panns_exact_knn_ex(panns_index *h, int source, uint32_t k, int method, uint32_t *out_ids, float *out_dists,

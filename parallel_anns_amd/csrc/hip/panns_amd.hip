@@ -2270,19 +2270,29 @@ int panns_exact_knn_ex(panns_index *h, int source, uint32_t k, int method, uint3
     const uint32_t lds = 8u * (panns::kKnnQT * panns::kKnnKCap + panns::kKnnQT * 64u + 4u * 64u) +
                          4u * panns::kKnnRT * RS + 4u * 2u * panns::kKnnQT + 8u * panns::kKnnQT;
     HIP_TRY(hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    // Matrix-instruction path: a sample of S rows bounds each query's k-th
-    // distance; about k * n / S rows per query pass the filter, and a list of
-    // four times that (at least 1,024) overflows rarely.  It needs the
-    // sample to be a small share of the rows.  The sample's own bound comes
-    // the same way from a first sample of S1 rows when S is at least 4 S1:
-    // the exact chain over S1 rows, then the filter and re-rank over S.
-    const uint32_t S = std::max<uint32_t>(4u * k + 64u, std::min<uint32_t>(32768u, h->n / 8u));
-    const uint32_t S1 = std::max<uint32_t>(4u * k + 64u, std::min<uint32_t>(4096u, h->n / 64u));
-    const bool two_level = S >= 4u * S1;
-    const uint32_t cap = std::min<uint32_t>(
-        16384u, std::max<uint32_t>({1024u, 4u * k * ((h->n + S - 1u) / S), 4u * k * ((S + S1 - 1u) / S1)}));
+    // Matrix-instruction path: the exact chain over the first S0 rows bounds
+    // each query's k-th distance; then each pass filters the first S(i+1)
+    // rows against the bound from S(i) and re-ranks the candidates, which
+    // gives the exact k-th distance over S(i+1), up to S = n.  About
+    // k * S(i+1) / S(i) rows per query pass a filter, so the sizes grow by
+    // an equal ratio of at most 32 and a list holds four times that ratio's
+    // share (at least 1,024 ids, at most 4 * 128 * 32).
+    const uint32_t S0 = std::max<uint32_t>(4u * k + 64u, std::min<uint32_t>(4096u, h->n / 8u));
+    std::vector<uint32_t> levels;
+    uint32_t cap = 1024u;
+    if (h->n >= 2u * S0) {
+        const double span = (double)h->n / (double)S0;
+        const int passes = std::max(1, (int)std::ceil(std::log(span) / std::log(32.0) - 1e-9));
+        const double ratio = std::pow(span, 1.0 / passes);
+        double cur = S0;
+        for (int i = 0; i < passes; ++i) {
+            cur = (i + 1 == passes) ? (double)h->n : cur * ratio;
+            levels.push_back(std::min<uint32_t>(h->n, (uint32_t)std::ceil(cur)));
+        }
+        cap = std::max<uint32_t>(cap, 4u * k * (uint32_t)std::ceil(ratio));
+    }
     const bool use_mfma = (method == PANNS_KNN_MFMA || (method == PANNS_KNN_AUTO && h->n >= kKnnMfmaMinRows)) &&
-                          h->n >= 2u * S;
+                          !levels.empty();
     float vmax2 = 0.f;
     if (use_mfma) {
         int rc = ensure_norms(h);
@@ -2331,20 +2341,21 @@ int panns_exact_knn_ex(panns_index *h, int source, uint32_t k, int method, uint3
                 HIP_TRY(hipGetLastError());
                 return PANNS_OK;
             };
-            // the sample's exact k-th distances in dsd (a query whose first
-            // pass overflowed gets none, hence an unbounded second pass and,
-            // overflowing again, the exact recompute)
-            if (two_level) {
-                launch(h->base.p, S1, qc, cn, k, ex, q0, nullptr, di.p, dd.p, grid, lds, h->stream);
-                HIP_TRY(hipGetLastError());
-                int rc = pass(S, dd.p, dsi.p, dsd.p);
+            // each level's exact k-th distances feed the next level's bounds
+            // (a query whose pass overflowed gets none, hence unbounded later
+            // passes and, overflowing again, the exact recompute); the last
+            // pass writes the result into (di, dd)
+            uint32_t *li[2] = {dsi.p, di.p};
+            float *ld[2] = {dsd.p, dd.p};
+            const int nl = (int)levels.size();
+            int cur = nl % 2 == 0 ? 1 : 0;  // so that the last pass lands in (di, dd)
+            launch(h->base.p, S0, qc, cn, k, ex, q0, nullptr, li[cur], ld[cur], grid, lds, h->stream);
+            HIP_TRY(hipGetLastError());
+            for (int l = 0; l < nl; ++l) {
+                int rc = pass(levels[l], ld[cur], li[1 - cur], ld[1 - cur]);
                 if (rc) return rc;
-            } else {
-                launch(h->base.p, S, qc, cn, k, ex, q0, nullptr, dsi.p, dsd.p, grid, lds, h->stream);
-                HIP_TRY(hipGetLastError());
+                cur = 1 - cur;
             }
-            int rc = pass(h->n, dsd.p, di.p, dd.p);
-            if (rc) return rc;
         } else {
             launch(h->base.p, h->n, qc, cn, k, ex, q0, nullptr, di.p, dd.p, grid, lds, h->stream);
         }
