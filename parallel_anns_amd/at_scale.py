@@ -156,8 +156,8 @@ def main():
     best = (ms_f, None, {})
     # sparser model evaluation first; a denser one (or the model at all, over
     # the fixed search) is kept only when 2 % faster, as in bench.py.  Long
-    # queues run hundreds of iterations, so strides of 16 and 8 are tried too
-    for every in (16, 8, 4, 2, 1):
+    # queues run hundreds of iterations, so strides up to 32 are tried too
+    for every in (32, 16, 8, 4, 2, 1):
         kw = dict(stop_min_iter=min_iter, stop_every=every)
         thr = learned.calibrate(idx, vq, vgt, K, vmax, M, target, L_per_query=vL, **kw)
         if thr is None:
