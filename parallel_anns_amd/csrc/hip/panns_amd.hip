@@ -490,9 +490,12 @@ int check_params(panns_index *h, const panns_params *p, uint32_t L, uint32_t K)
 // 8 * pend_cap bytes), query row (4*dim B),
 // visited table (2 B per slot), newids[nb_cap] picks[Mcap] counts[4] h1[64]
 // (4 B each).
-uint32_t lds_bytes(uint32_t Lcap, uint32_t nb_cap, uint32_t tbl_slots, uint32_t Mcap, uint32_t dim)
+uint32_t lds_bytes(uint32_t Lcap, uint32_t nb_cap, uint32_t tbl_slots, uint32_t Mcap, uint32_t dim, bool learned)
 {
-    return 8u * (Lcap + nb_cap + 64u) + 4u * dim + 2u * tbl_slots + 4u * (nb_cap + Mcap + 4u) + 4u * 64u;
+    // h1, the model's hidden layer, is the last region and only the
+    // learned-stop kernel touches it
+    return 8u * (Lcap + nb_cap + 64u) + 4u * dim + 2u * tbl_slots + 4u * (nb_cap + Mcap + 4u) +
+           (learned ? 4u * 64u : 0u);
 }
 
 // search_kernel_mw's carving: Q[Lcap] cand[W][nb_cap] bb[64] (8 B each),
@@ -662,10 +665,13 @@ int plan_geometry(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq,
     const uint32_t W = pl->waves;
     // PSS_v5 with T queues: one step's neighbours (T rows), and a worker
     // queue's entries staged for the merge into the master
-    const uint32_t nb_cap =
+    // one wave: a pick group's slots (M rows, at most kGroupSlots) plus room
+    // for the compaction staging (128 keys) -- the LDS it saves at small M
+    // lets long queues fit one more workgroup per CU
+    uint32_t nb_cap =
         pl->pss ? (std::max<uint32_t>({panns::kGroupSlots, p.num_threads * stride, (pl->maxL + 7u) & ~7u}) + 7u) & ~7u
         : W > 1 ? std::max<uint32_t>(panns::kGroupSlots, (p.M + W - 1) / W * stride)
-                : std::max<uint32_t>(panns::kGroupSlots, stride);
+                : std::max<uint32_t>({128u, stride, std::min<uint32_t>(panns::kGroupSlots, p.M * stride)});
     const uint32_t Mcap = std::max<uint32_t>(4u, (p.M + 3u) & ~3u);
     // visited table: hs 16-bit slots in nb buckets of 8; the tag of an id
     // is its scrambled value divided by nb and must stay below 2^15 (0xFFFF
@@ -721,9 +727,12 @@ int plan_geometry(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq,
                            p.M <= 64;
     pl->pend_cap = can_defer && (pl->deferred_merge == 2 || (pl->deferred_merge == 0 && pl->maxL > kDeferMinL))
                        ? kPendCap : 0u;
+    // the deferred pick stages its candidates and positions in 192 keys of cand
+    if (pl->pend_cap) nb_cap = std::max<uint32_t>(nb_cap, 192u);
     pl->lds = pl->pss ? lds_bytes_pss(Lcap, nb_cap, hs + 2u * ovf, h->dim, p.num_threads)
               : W > 1 ? lds_bytes_mw(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim, W)
-                      : lds_bytes(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim) + 8u * pl->pend_cap;
+                      : lds_bytes(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim, p.stop == PANNS_STOP_LEARNED) +
+                            8u * pl->pend_cap;
     if (pl->pend_cap)
         select_kernel(h->dim, p.stop == PANNS_STOP_LEARNED, norm, false, 1u, false, true, &fn,
                       &kp);  // the deferred-merge variant

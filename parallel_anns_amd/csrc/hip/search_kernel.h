@@ -1098,10 +1098,13 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
         if (!a.init_queue) {
             // initialize_set_L_para (PSS5LG.cpp:1055-1096): mark every init id
             // visited, compute its distance, sort the L keys.
-            for (uint32_t c0 = 0; c0 < L; c0 += 64 * kClaimChunks) {
+            // newids holds nb_cap ids, so a claim batch covers at most
+            // nb_cap / 64 chunks of init ids
+            const uint32_t ich = min((uint32_t)kClaimChunks, a.nb_cap / 64u);
+            for (uint32_t c0 = 0; c0 < L; c0 += 64u * ich) {
                 uint32_t ids[kClaimChunks];
                 uint32_t okbits = 0;
-                const uint32_t nch = min((uint32_t)kClaimChunks, (L - c0 + 63) / 64);
+                const uint32_t nch = min(ich, (L - c0 + 63) / 64);
 #pragma unroll
                 for (int u = 0; u < kClaimChunks; ++u) {
                     const uint32_t i = c0 + (uint32_t)u * 64 + lane;
