@@ -93,6 +93,12 @@ constexpr uint32_t kRanges = 8;         // work ranges (one per XCD)
 constexpr uint32_t kPivots = 256;       // locality pivots
 constexpr uint32_t kLocalityMin = 512;  // smallest batch that is reordered
 constexpr uint32_t kPendCap = 256;      // pending keys of the deferred merge
+// LDS allocation granule of a workgroup on gfx950, measured: one-wave
+// workgroups of 32,672 bytes run 4 per CU (160 KiB of LDS) and of 31,776
+// bytes 5 per CU (18 % faster on the same queries), while the runtime's
+// occupancy query reports 5 for both; 1,280 bytes (160 KiB / 128) is the
+// granule both fit
+constexpr uint32_t kLdsGranule = 1280;
 // exact k-NN: the automatic method takes the matrix-instruction path from
 // this many base rows (measured, 64-NN of every row: 1.9x faster at 8,192
 // rows, 2.0x at 49,000, 2.1x at 100,000)
@@ -143,6 +149,7 @@ struct panns_index {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int num_cu = 0;
+    uint32_t lds_per_cu = 0;
     // base and queries
     uint32_t n = 0, dim = 0, nq = 0;
     DevBuf<float> base, queries;
@@ -488,14 +495,11 @@ int check_params(panns_index *h, const panns_params *p, uint32_t L, uint32_t K)
 // Layout of one workgroup's LDS; must match search_kernel's carving:
 // Q[Lcap] cand[nb_cap] bb[64] P[pend_cap] (8 B each; the caller adds P's
 // 8 * pend_cap bytes), query row (4*dim B),
-// visited table (2 B per slot), newids[nb_cap] picks[Mcap] counts[4] h1[64]
-// (4 B each).
-uint32_t lds_bytes(uint32_t Lcap, uint32_t nb_cap, uint32_t tbl_slots, uint32_t Mcap, uint32_t dim, bool learned)
+// visited table (2 B per slot), newids[nb_cap] picks[Mcap] counts[4]
+// (4 B each).  The stop model's hidden layer shares cand.
+uint32_t lds_bytes(uint32_t Lcap, uint32_t nb_cap, uint32_t tbl_slots, uint32_t Mcap, uint32_t dim)
 {
-    // h1, the model's hidden layer, is the last region and only the
-    // learned-stop kernel touches it
-    return 8u * (Lcap + nb_cap + 64u) + 4u * dim + 2u * tbl_slots + 4u * (nb_cap + Mcap + 4u) +
-           (learned ? 4u * 64u : 0u);
+    return 8u * (Lcap + nb_cap + 64u) + 4u * dim + 2u * tbl_slots + 4u * (nb_cap + Mcap + 4u);
 }
 
 // search_kernel_mw's carving: Q[Lcap] cand[W][nb_cap] bb[64] (8 B each),
@@ -727,12 +731,12 @@ int plan_geometry(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq,
                            p.M <= 64;
     pl->pend_cap = can_defer && (pl->deferred_merge == 2 || (pl->deferred_merge == 0 && pl->maxL > kDeferMinL))
                        ? kPendCap : 0u;
-    // the deferred pick stages its candidates and positions in 192 keys of cand
-    if (pl->pend_cap) nb_cap = std::max<uint32_t>(nb_cap, 192u);
+    // the deferred pick stages its candidates and positions in 3 * Mcap keys
+    // of cand
+    if (pl->pend_cap) nb_cap = std::max<uint32_t>(nb_cap, 3u * Mcap);
     pl->lds = pl->pss ? lds_bytes_pss(Lcap, nb_cap, hs + 2u * ovf, h->dim, p.num_threads)
               : W > 1 ? lds_bytes_mw(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim, W)
-                      : lds_bytes(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim, p.stop == PANNS_STOP_LEARNED) +
-                            8u * pl->pend_cap;
+                      : lds_bytes(Lcap, nb_cap, hs + 2u * ovf, Mcap, h->dim) + 8u * pl->pend_cap;
     if (pl->pend_cap)
         select_kernel(h->dim, p.stop == PANNS_STOP_LEARNED, norm, false, 1u, false, true, &fn,
                       &kp);  // the deferred-merge variant
@@ -742,6 +746,13 @@ int plan_geometry(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq,
     int per_cu = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, 64 * W, pl->lds));
     if (per_cu <= 0) return fail(PANNS_ERR_UNSUPPORTED, "kernel does not fit on a CU with this L");
+    // the CU allocates LDS to a workgroup in granules of kLdsGranule bytes,
+    // which the occupancy query does not count: the grid holds the
+    // workgroups that are resident together
+    if (h->lds_per_cu) {
+        const uint32_t g = (pl->lds + kLdsGranule - 1u) / kLdsGranule * kLdsGranule;
+        per_cu = std::max(1, std::min(per_cu, (int)(h->lds_per_cu / std::max(g, 1u))));
+    }
     pl->grid = std::min<uint32_t>(pl->nq, (uint32_t)per_cu * (uint32_t)h->num_cu);
     if (pl->grid == 0) pl->grid = 1;
     geo->Lcap = Lcap;
@@ -1245,6 +1256,7 @@ int panns_create(int device, panns_index **out)
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     h->num_cu = prop.multiProcessorCount;
+    h->lds_per_cu = (uint32_t)prop.maxSharedMemoryPerMultiProcessor;
     // this device's allocation-free scratch (panns_search_noalloc), looked
     // up once per device; a failure here is reported by that entry point
     uint8_t *scr = nullptr;

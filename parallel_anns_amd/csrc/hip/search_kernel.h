@@ -1050,7 +1050,10 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
     uint32_t *newids = tbl + a.tbl_words;
     uint32_t *picks = newids + a.nb_cap;
     uint32_t *counts = picks + a.Mcap;     // [2] work item
-    float *h1s = reinterpret_cast<float *>(counts + 4);
+    // the stop model's hidden layer (64 floats) shares cand: the model runs
+    // after the iteration's merge, when cand holds nothing live, and
+    // mlp_eval ends with a barrier
+    float *h1s = reinterpret_cast<float *>(cand);
 
     const uint32_t stride = 1u << a.stride_log2;
     Visited vis;
@@ -1172,8 +1175,10 @@ __global__ __launch_bounds__(64, 2) void search_kernel(KArgs a)
                 // Mcur unchecked ones (from k) and P's, provided those Q
                 // entries rank below L in the union (position + SP < L);
                 // otherwise P is merged first and the plain pick runs.
-                uint64_t *ck = cand, *pk = cand + 64;
-                uint32_t *cp = reinterpret_cast<uint32_t *>(cand + 128), *pi = cp + 64;
+                // staging of at most Mcur <= Mcap picks per list (3 * Mcap
+                // keys of cand)
+                uint64_t *ck = cand, *pk = cand + a.Mcap;
+                uint32_t *cp = reinterpret_cast<uint32_t *>(cand + 2u * a.Mcap), *pi = cp + a.Mcap;
                 uint32_t mq = 0, q_last = 0;
                 while (mq < Mcur && pos < S) {
                     const uint32_t j = pos + (uint32_t)lane;
