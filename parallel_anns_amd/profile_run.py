@@ -148,6 +148,7 @@ def uneven_run(idx, ds, args):
         assert int(sel.sum()) == b["nq"], (i, b)
         b["algorithmic_bytes"] = int(4.0 * dim * nd[sel].sum() + 4.0 * (1.0 + slots) * ne[sel].sum()
                                      + (4.0 * dim * sel.sum() + 8.0 * Ks[sel].astype(np.float64).sum()))
+        b["iterations"] = int(res["niter"][sel].astype(np.int64).sum())
     alg = 4.0 * dim * c["distance_computations"] + 4.0 * (c["expansions"] + c["adjacency_slots"]) \
         + 4.0 * dim * nq + 8.0 * float(Ks.astype(np.float64).sum())
     return {"config": args.config, "steps": args.steps, "M": M, "uneven": True,

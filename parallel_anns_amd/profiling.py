@@ -6,6 +6,7 @@
   python -m parallel_anns_amd.profiling report [uneven] --trace T.db --fetch F.db --write W.db --cal-fetch CF.db
                                                 --cal-write CW.db --copy-bytes B --run run.json
                                                 --command "..." --md out.md --records out.json
+                                                [--counters P1.db,P2.db]
 
 `trace` prints per-kernel call counts and mean/min/max durations (all
 dispatches, and the last N dispatches of the selected kernel, e.g. the timed
@@ -143,6 +144,23 @@ def report_uneven(a):
                     "uneven": True, "buckets": [b["max_L"] for b in bk], "traffic_bytes": int(tot_t),
                     "algorithmic_bytes": int(alg), "run_ms": run["run_ms"], "fetch_factor": round(ff, 4),
                     "write_factor": round(wf, 4), "command": a.command})
+    if a.counters and all("iterations" in b for b in bk):
+        per = {}
+        for db in a.counters.split(","):
+            names = [r[0] for r in sqlite3.connect(db).execute("select distinct counter_name from counters_collection")]
+            for cn in names:
+                rr = _by_run(_dispatches(db, "counters_collection", match, f"where counter_name='{cn}'"), nb, steps,
+                             bk)
+                per[cn] = [mean([r[i][2] for r in rr]) for i in range(nb)]
+        lines += ["Hardware counters per search iteration (sum over the launch / the bucket's iterations; a "
+                  "--pmc pass runs the dispatches one at a time, so each bucket runs alone here):", "",
+                  "| counter | " + " | ".join(f"L ≤ {b['max_L']}" for b in bk) + " |",
+                  "|---|" + "---|" * nb,
+                  "| iterations | " + " | ".join(f"{b['iterations']:,}" for b in bk) + " |"]
+        for cn in sorted(per):
+            lines.append(f"| {cn} | " + " | ".join(f"{per[cn][i] / max(1, bk[i]['iterations']):,.1f}"
+                                                   for i in range(nb)) + " |")
+        lines.append("")
     lines += ["Kernel trace, all kernels of the traced run:", "", "```"]
     rows = kernel_durations(a.trace)
     for n in sorted({x[0] for x in rows}):
@@ -218,6 +236,8 @@ def main():
                   "--records"):
             ap.add_argument(k, required=True)
         ap.add_argument("--copy-bytes", type=int, required=True)
+        ap.add_argument("--counters", default="", help="comma-separated --pmc databases of the same command "
+                        "(uneven reports: a per-iteration table per bucket)")
         ap.add_argument("--dim", type=int, default=96)
         ap.add_argument("--title", default="search_kernel, 1M x 96, 10k queries")
         a = ap.parse_args()
