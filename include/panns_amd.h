@@ -70,7 +70,10 @@ typedef struct {
                               the per-query LDS visited table (raised when the id space needs).
                               Automatic: 8,192 up to L = 512, 10,240 above, scaled by the launch's
                               largest L when no plan options give a size (see
-                              panns_plan_options.table_eighths = 0) */
+                              panns_plan_options.table_eighths = 0); an automatic table then
+                              grows into the LDS that the 1,280-byte allocation granule leaves
+                              free at the launch's workgroups per CU (not for buckets side by
+                              side).  Results never depend on the size */
     uint32_t waves;        /* wavefronts per query: 0 or 1, else 2 or 4 (TOP_M and SCALE_M with the
                               fixed stop): the picks of an iteration are expanded by several waves
                               and merged into the master queue, the deterministic form of the
@@ -255,7 +258,9 @@ typedef struct {
                                                   buckets are full, before the global bitmap: 0 or
                                                   a power of two in [8, 4096] */
     uint32_t deferred_merge;                   /* 0: automatic (on for buckets whose largest L is above
-                                                  1,000); 1: merge every iteration's survivors into the
+                                                  1,000, and above 512 where its 2 KB of LDS cost no
+                                                  workgroup per CU and the bucket does not run side by
+                                                  side with others); 1: merge every iteration's survivors into the
                                                   queue; 2: always defer.  Deferred: survivors wait in a
                                                   sorted buffer of 256 keys and are merged into the queue
                                                   when it fills, a pick could rank outside the top L or
