@@ -81,8 +81,10 @@ typedef struct {
                               results equal the single-wave search */
     uint32_t stop_min_iter;  /* LEARNED: the model runs from this iteration on (0: from the first) */
     uint32_t stop_every;     /* LEARNED: ... at every stop_every-th iteration (0 or 1: every one) */
-    uint32_t bucket_slots;   /* visited-table buckets of 8 or 4 slots; 0 = automatic (8, or 4 when the
-                                id space needs more buckets than an 8-slot table of that size has) */
+    uint32_t bucket_slots;   /* visited-table buckets of 8, 4 or 2 slots; 0 = automatic (8, or 4 when the
+                                id space needs more buckets than an 8-slot table of that size has).
+                                The id space sets the least bucket count (ids / 2^15), so narrower
+                                buckets allow smaller tables on large id spaces */
 } panns_params;
 
 typedef struct {

@@ -10,6 +10,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "panns_amd.h"
@@ -168,6 +169,9 @@ struct panns_index {
     // bumped whenever the scratch buffers are reallocated (a captured graph
     // holds their addresses and must be captured again)
     uint64_t scratch_gen = 0;
+    // plans handed to the caller: panns_destroy detaches them, so a plan
+    // destroyed after its index frees its own resources only
+    std::unordered_set<panns_plan *> plans;
     // bumped whenever base rows, queries, graph or norms are replaced: a
     // plan holds their device addresses and its query ids were checked
     // against the query count of its build
@@ -196,7 +200,8 @@ struct panns_index {
 };
 
 struct panns_plan {
-    panns_index *h = nullptr;
+    panns_index *h = nullptr;  // null once the index is destroyed
+    int device = 0;
     panns_params p{};
     uint32_t nq = 0, maxL = 0, maxK = 0;
     DevBuf<uint32_t> qids, qL, qK, init_ids, out_ids, out_stats, out_qids;
@@ -257,6 +262,14 @@ struct panns_plan {
         for (hipStream_t st : side) (void)hipStreamDestroy(st);
     }
 };
+
+// A plan the caller will hold: registered with its index.
+static panns_plan *hand_out(panns_index *h, std::unique_ptr<panns_plan> &pl)
+{
+    pl->device = h->device;
+    h->plans.insert(pl.get());
+    return pl.release();
+}
 
 namespace {
 
@@ -487,8 +500,8 @@ int check_params(panns_index *h, const panns_params *p, uint32_t L, uint32_t K)
         return fail(PANNS_ERR_UNSUPPORTED, "the learned stop's features are squared-L2 distances; use the fixed stop");
     if (p->hash_slots && (p->hash_slots % 64 || p->hash_slots < 64 || p->hash_slots > 32768))
         return fail(PANNS_ERR_ARG, "hash_slots must be a multiple of 64 in [64, 32768]");
-    if (p->bucket_slots && p->bucket_slots != 4 && p->bucket_slots != 8)
-        return fail(PANNS_ERR_ARG, "bucket_slots must be 0 (automatic), 4 or 8");
+    if (p->bucket_slots && p->bucket_slots != 2 && p->bucket_slots != 4 && p->bucket_slots != 8)
+        return fail(PANNS_ERR_ARG, "bucket_slots must be 0 (automatic), 2, 4 or 8");
     if (p->waves > 1) {
         if (p->waves != 2 && p->waves != 4) return fail(PANNS_ERR_ARG, "waves must be 1, 2 or 4");
         if (p->mode == PANNS_MODE_PSS_V5) return fail(PANNS_ERR_UNSUPPORTED, "PSS_V5 expands one candidate per step; use waves = 1");
@@ -701,9 +714,18 @@ int plan_geometry(panns_plan *pl, const uint32_t *query_ids, const uint32_t *Lq,
     const uint64_t min_nb = (((1ull << id_bits) - 1ull) / 0x7FFEull) + 1ull;
     // buckets of 8 slots, or of 4 when an 8-slot table of this size has
     // fewer buckets than the id space needs (ids above 2^25 at 8,192 slots):
-    // the table then keeps its size instead of growing with N
-    uint32_t bslots = p.bucket_slots ? p.bucket_slots : ((hs / 8u) >= min_nb ? 8u : 4u);
-    uint32_t nb = std::max<uint32_t>(hs / bslots, (uint32_t)((min_nb + 7ull) & ~7ull));
+    // the table then keeps its size instead of growing with N.  A table of
+    // at most half the least 4-slot one (the small tables long queues ask
+    // for, on large id spaces) takes buckets of 2 slots: measured on 50M x
+    // 96, 4,112 two-slot slots instead of 8,224 four-slot ones fit 6 and 5
+    // workgroups per CU instead of 4 for L 1,368-1,683 and 1,684-2,000, -8
+    // and -6 %, though 60 % more ids spill to the bitmap.
+    const uint32_t nb_min = (uint32_t)((min_nb + 7ull) & ~7ull);
+    uint32_t bslots = p.bucket_slots ? p.bucket_slots
+                      : (hs / 8u) >= min_nb ? 8u
+                      : 2u * hs <= 4u * nb_min ? 2u
+                                               : 4u;
+    uint32_t nb = std::max<uint32_t>(hs / bslots, nb_min);
     nb = std::max<uint32_t>(nb, 8u);
     hs = nb * bslots;
     // No table (visited_table = 2 only): every visited id is claimed in the
@@ -951,7 +973,7 @@ void fill_args(panns_plan *pl, const Geometry &geo, const PlanBufs &b)
     a.ovf_slots = pl->shared_bitmap ? 0u : pl->overflow_slots;
     a.tbl_words = pl->hash_slots / 2u + a.ovf_slots;
     a.tbl_buckets = geo.nb;
-    a.tbl_bshift = geo.bslots == 8u ? 2u : 1u;
+    a.tbl_bshift = geo.bslots == 8u ? 2u : geo.bslots == 4u ? 1u : 0u;
     a.tbl_magic = (uint32_t)((1ull << 32) / geo.nb);
     a.id_bits = geo.id_bits;
     a.use_table = pl->shared_bitmap || pl->no_table ? 0u : 1u;
@@ -1311,6 +1333,11 @@ int panns_destroy(panns_index *h)
     if (!h) return PANNS_OK;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
+    for (panns_plan *pl : h->plans) {
+        pl->h = nullptr;
+        for (auto &part : pl->parts) part->h = nullptr;
+    }
+    h->plans.clear();
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1499,7 +1526,7 @@ int panns_plan_create(panns_index *h, const panns_params *p, const uint32_t *que
     pl->want_queue = true;
     rc = build_plan(pl.get(), query_ids, L_per_query, K_per_query, init_ids, n_init);
     if (rc) return rc;
-    *out = pl.release();
+    *out = hand_out(h, pl);
     return PANNS_OK;
 }
 
@@ -1540,7 +1567,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
     if (o.n_buckets == 1 && !o.use_graph && !o.concurrent) {
         rc = build_plan(pl.get(), query_ids, L_per_query, K_per_query, init_ids, n_init);
         if (rc) return rc;
-        *out = pl.release();
+        *out = hand_out(h, pl);
         return PANNS_OK;
     }
     // L and K of every query, checked as a single plan would
@@ -1641,7 +1668,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
         pl->lds = std::max(pl->lds, part->lds);
         pl->hash_slots = std::max(pl->hash_slots, part->no_table ? 0u : part->hash_slots);
     }
-    *out = pl.release();
+    *out = hand_out(h, pl);
     return PANNS_OK;
 }
 
@@ -1776,6 +1803,7 @@ int panns_plan_autotune(panns_index *h, const panns_params *p, const uint32_t *q
 int panns_plan_run(panns_plan *plan, int sync, float *kernel_ms)
 {
     if (!plan) return fail(PANNS_ERR_ARG, "null plan");
+    if (!plan->h) return fail(PANNS_ERR_STATE, "the plan's index was destroyed");
     HIP_TRY(hipSetDevice(plan->h->device));
     return run_plan(plan, sync, kernel_ms);
 }
@@ -1783,6 +1811,7 @@ int panns_plan_run(panns_plan *plan, int sync, float *kernel_ms)
 int panns_plan_fetch(panns_plan *plan, panns_outputs *out, panns_counters *cnt)
 {
     if (!plan) return fail(PANNS_ERR_ARG, "null plan");
+    if (!plan->h) return fail(PANNS_ERR_STATE, "the plan's index was destroyed");
     HIP_TRY(hipSetDevice(plan->h->device));
     float ms = 0.f;
     if (plan->ran) {
@@ -1795,8 +1824,11 @@ int panns_plan_fetch(panns_plan *plan, panns_outputs *out, panns_counters *cnt)
 int panns_plan_destroy(panns_plan *plan)
 {
     if (!plan) return PANNS_OK;
-    (void)hipSetDevice(plan->h->device);
-    (void)hipStreamSynchronize(plan->h->stream);
+    (void)hipSetDevice(plan->device);
+    if (plan->h) {
+        (void)hipStreamSynchronize(plan->h->stream);
+        plan->h->plans.erase(plan);
+    }
     delete plan;
     return PANNS_OK;
 }
@@ -2251,6 +2283,7 @@ int panns_plan_phase_cycles(panns_plan *plan, uint64_t *out8)
     if (plan && !plan->parts.empty())
         return fail(PANNS_ERR_UNSUPPORTED, "phase cycles are kept for single-bucket plans only");
     if (!plan || !out8) return fail(PANNS_ERR_ARG, "null argument");
+    if (!plan->h) return fail(PANNS_ERR_STATE, "the plan's index was destroyed");
 #ifdef PANNS_PHASE_TIMERS
     HIP_TRY(hipSetDevice(plan->h->device));
     HIP_TRY(hipStreamSynchronize(plan->h->stream));

@@ -590,7 +590,7 @@ __device__ __forceinline__ uint32_t merge_run(uint64_t *Q, uint32_t &S, uint32_t
 struct Visited {
     uint32_t *tbl;      // LDS: buckets x (1 << bshift) words
     uint32_t nb, magic, idbits, idmask, xshift, use;  // buckets, floor(2^32 / nb)
-    uint32_t bshift;    // 2: buckets of 8 slots, 1: of 4 slots
+    uint32_t bshift;    // 2: buckets of 8 slots, 1: of 4 slots, 0: of 2 slots
     uint32_t *ovf;      // LDS: overflow entries (full ids, kSent = empty), or null
     uint32_t ovf_mask;  // entries - 1
     uint32_t *bitmap;   // global
@@ -631,10 +631,12 @@ struct Visited {
                 words[1] = v.y;
                 words[2] = v.z;
                 words[3] = v.w;
-            } else {
+            } else if constexpr (NW == 2) {
                 const uint2 v = *reinterpret_cast<const uint2 *>(w);
                 words[0] = v.x;
                 words[1] = v.y;
+            } else {
+                words[0] = *w;
             }
             uint32_t hm = 0, ew = NW, eold = 0;  // ew: first word with an empty slot
 #pragma unroll
@@ -655,7 +657,9 @@ struct Visited {
     }
     __device__ __forceinline__ int probe_bucket(uint32_t b, uint32_t tag) const
     {
-        return bshift == 2 ? probe_words<4>(tbl + (b << 2), tag) : probe_words<2>(tbl + (b << 1), tag);
+        return bshift == 2   ? probe_words<4>(tbl + (b << 2), tag)
+               : bshift == 1 ? probe_words<2>(tbl + (b << 1), tag)
+                             : probe_words<1>(tbl + b, tag);
     }
 
     // 0: visited before; 1: new and now in the table; 2: in neither of its

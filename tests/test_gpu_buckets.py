@@ -168,3 +168,28 @@ def test_plans_are_refused_after_inputs_change(small_set, gpu_index, uneven):
     again.run()
     _same(again.fetch(), r0)
     again.close()
+
+
+def test_plans_outliving_their_index(small_set, uneven):
+    """An index destroyed before its plans detaches them: running or
+    fetching such a plan is an error (PANNS_ERR_STATE) and destroying it
+    frees only its own resources (it touched the freed index before)."""
+    qids, Ls, Ks = uneven
+    idx = Index(0)
+    idx.set_base(small_set["base"])
+    idx.set_queries(small_set["queries"])
+    idx.set_graph(small_set["adj"], small_set["deg"], small_set["ep"])
+    init = idx.prepare_init_ids(int(Ls.max()))
+    plain = idx.plan(qids[:16], 50, 100, 2, want_queue=False)
+    graph = idx.plan(qids, 100, int(Ls.max()), 2, init_ids=init, L_per_query=Ls, K_per_query=Ks,
+                     options=Index.plan_options(buckets=2, use_graph=True, concurrent=True), want_queue=False)
+    plain.run()
+    graph.run()
+    idx.close()
+    for pl in (plain, graph):
+        with pytest.raises(SearchError) as e:
+            pl.run()
+        assert e.value.code == -9
+        with pytest.raises(SearchError):
+            pl.fetch()
+        pl.close()

@@ -100,17 +100,18 @@ def test_small_tables_spill_and_keep_results(small_set, gpu_index, slots):
     assert (again["ids"] == res["ids"]).all() and (again["ndist"] == res["ndist"]).all()
 
 
+@pytest.mark.parametrize("bs", [4, 2])
 @pytest.mark.parametrize("slots", [64, 192, 1536, 8192])
-def test_four_slot_buckets_keep_results(small_set, gpu_index, slots):
+def test_narrow_buckets_keep_results(small_set, gpu_index, slots, bs):
     """Visited tables of 4-slot buckets (the automatic choice for id spaces
-    above 2^25 at the default size, forced here): second buckets and
-    spills, results those of the oracle."""
+    above 2^25 at the default size) and of 2-slot buckets, forced here:
+    second buckets and spills, results those of the oracle."""
     ds = small_set
     K, L, M = 100, 160, 4
     qids = np.arange(48, dtype=np.uint32)
-    _print_case(test="four_slot_buckets", hash_slots=slots)
+    _print_case(test="narrow_buckets", hash_slots=slots, bucket_slots=bs)
     init = gpu_index.prepare_init_ids(L)
-    res = gpu_index.search(qids, K, L, M, init_ids=init, hash_slots=slots, bucket_slots=4, want_queue=True)
+    res = gpu_index.search(qids, K, L, M, init_ids=init, hash_slots=slots, bucket_slots=bs, want_queue=True)
     if slots < 1536:
         assert res["counters"]["visited_spills"] > 0
     ref = oracle_run(ds, qids, K, L, M, init)
@@ -119,13 +120,14 @@ def test_four_slot_buckets_keep_results(small_set, gpu_index, slots):
         assert (res["queue_ids"][i] == ref[i]["queue"]["id"]).all()
         assert res["ndist"][i] == ref[i]["ndist"]
     for w in (2, 4):
-        mw = gpu_index.search(qids, K, L, M, init_ids=init, hash_slots=slots, bucket_slots=4, waves=w)
+        mw = gpu_index.search(qids, K, L, M, init_ids=init, hash_slots=slots, bucket_slots=bs, waves=w)
         assert (mw["ids"] == res["ids"]).all() and (mw["ndist"] == res["ndist"]).all()
     with pytest.raises(SearchError):
         gpu_index.search(qids[:2], K, L, M, init_ids=init, bucket_slots=6)
 
 
-@pytest.mark.parametrize("slots,ovf,bs", [(64, 256, 8), (192, 64, 8), (1536, 1024, 8), (128, 512, 4)])
+@pytest.mark.parametrize("slots,ovf,bs", [(64, 256, 8), (192, 64, 8), (1536, 1024, 8), (128, 512, 4),
+                                         (128, 256, 2)])
 def test_overflow_area_keeps_results(small_set, gpu_index, slots, ovf, bs):
     """Ids whose buckets are full go to the LDS overflow entries, then to the
     bitmap; the results stay the oracle's and fewer ids reach the bitmap."""

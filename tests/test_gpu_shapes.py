@@ -83,7 +83,7 @@ def uneven(c1):
     return Ls, Ks
 
 
-@pytest.mark.parametrize("bucket_slots", [8, 4])
+@pytest.mark.parametrize("bucket_slots", [8, 4, 2])
 def test_c1_uneven_L_up_to_2000(c1, uneven, bucket_slots):
     idx, g = c1["index"], c1["graph"]
     Ls, Ks = uneven

@@ -15,9 +15,11 @@ class TableModel:
     def __init__(self, slots, n, bucket_slots=0):
         self.id_bits = max(1, int(np.ceil(np.log2(n)))) if n > 1 else 1
         min_nb = ((1 << self.id_bits) - 1) // 0x7FFE + 1
-        # 8-slot buckets, or 4 when an 8-slot table this size has too few
-        bs = bucket_slots or (8 if slots // 8 >= min_nb else 4)
-        nb = max(slots // bs, (min_nb + 7) // 8 * 8, 8)
+        # 8-slot buckets, or 4 when an 8-slot table this size has too few,
+        # or 2 for a table of at most half the least 4-slot one
+        nb_min = (min_nb + 7) // 8 * 8
+        bs = bucket_slots or (8 if slots // 8 >= min_nb else 2 if 2 * slots <= 4 * nb_min else 4)
+        nb = max(slots // bs, nb_min, 8)
         self.nb, self.bucket_slots = nb, bs
         self.magic = (1 << 32) // nb
         self.idmask = (1 << self.id_bits) - 1
@@ -96,12 +98,13 @@ def test_second_buckets_take_the_overflow():
     assert len(t.bitmap) < 0.01 * len(ids)
 
 
+@pytest.mark.parametrize("bs", [4, 2])
 @pytest.mark.parametrize("slots,n,seed", [(64, 20000, 6), (1536, 1_000_000, 7), (8192, 50_000_000, 8),
-                                          (4096, 10_000_000, 9)])
-def test_four_slot_buckets_membership_equals_a_set(slots, n, seed):
+                                          (4096, 10_000_000, 9), (4096, 50_000_000, 10)])
+def test_narrow_buckets_membership_equals_a_set(slots, n, seed, bs):
     rng = np.random.default_rng(seed)
-    t = TableModel(slots, n, bucket_slots=4)
-    assert t.bucket_slots == 4
+    t = TableModel(slots, n, bucket_slots=bs)
+    assert t.bucket_slots == bs
     seen = set()
     ids = np.concatenate([rng.integers(0, n, 6000), rng.integers(0, n, 200).repeat(4)])
     rng.shuffle(ids)
@@ -120,3 +123,12 @@ def test_automatic_bucket_width_keeps_the_table_size():
     assert t.bucket_slots == 4 and t.nb * 4 <= 8192 + 64
     t8 = TableModel(8192, 50_000_000, bucket_slots=8)
     assert t8.nb * 8 > 16000
+    # 2-slot buckets: the least bucket count (2,056 at 26-bit ids) in half the
+    # slots, chosen for a table of at most half the least 4-slot one
+    t2 = TableModel(4096, 50_000_000, bucket_slots=2)
+    assert t2.nb == 2056 and t2.nb * 2 == 4112
+    assert TableModel(3840, 50_000_000).bucket_slots == 2
+    assert TableModel(4160, 50_000_000).bucket_slots == 4
+    assert TableModel(3840, 10_000_000).bucket_slots == 4  # 480 eight-slot buckets < 513
+    t = TableModel(1024, 10_000_000)
+    assert t.bucket_slots == 2 and t.nb == 520
