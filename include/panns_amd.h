@@ -115,7 +115,9 @@ typedef struct panns_plan panns_plan;
 const char *panns_last_error(void);
 
 /* Handle on HIP device `device` (ordinal).  Replaces PANNS::Searching's
- * constructor; destroy replaces ~Searching (PSS5LG.h:211-219). */
+ * constructor; destroy replaces ~Searching (PSS5LG.h:211-219).  Plans still
+ * held when their handle is destroyed are detached: running or fetching
+ * them returns PANNS_ERR_STATE, and panns_plan_destroy frees them. */
 int panns_create(int device, panns_index **out);
 int panns_destroy(panns_index *h);
 
