@@ -822,24 +822,17 @@ __device__ __forceinline__ uint32_t claim_chunks(Visited &vis, uint32_t (&ids)[k
             if (r == 2) gbits |= 1u << u;
         }
     }
-    // The bitmap path, only when some lane of the wave has an id there.
-    // Every lane issues the atomic of a chunk that any lane needs, so the
-    // batch is not split by exec-mask joins; lanes with nothing to claim OR 0
-    // into word `lane` (the bitmap's first two cache lines, padded to 64
-    // words, which stay in the L2), which changes nothing.  Pointing those
-    // lanes at their own id's word instead sent one random-line atomic to
-    // memory per idle lane: on a 50M-id set, 3,700 memory writes per query
-    // of a table plan that spills 340 ids.
+    // The bitmap path, only when some lane of the wave has an id there, and
+    // only those lanes issue the atomic: the L2 forwards every bitmap atomic
+    // to memory (measured: TCC_ATOMIC = TCC_EA0_ATOMIC), so an idle lane's
+    // OR of 0 would cost a memory request too.
     if (__ballot(gbits != 0u)) {
         uint32_t old[kClaimChunks];
 #pragma unroll
         for (int u = 0; u < kClaimChunks; ++u) {
             old[u] = 0;
             const bool g = (gbits >> u) & 1u;
-            if (__ballot(g)) {
-                const uint32_t wi = g ? ids[u] >> 5 : (uint32_t)(threadIdx.x & 63);
-                old[u] = atomicOr(&vis.bitmap[wi], g ? 1u << (ids[u] & 31) : 0u);
-            }
+            if (g) old[u] = atomicOr(&vis.bitmap[ids[u] >> 5], 1u << (ids[u] & 31));
         }
 #pragma unroll
         for (int u = 0; u < kClaimChunks; ++u) {
