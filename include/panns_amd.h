@@ -109,6 +109,9 @@ typedef struct {
     uint32_t *niter;         /* nq: iterations (picks rounds) per query */
 } panns_outputs;
 
+/* A handle and the plans made from it are used from one host thread at a
+ * time (its scratch buffers, stream and plan registry are not locked);
+ * separate handles may be used from separate threads. */
 typedef struct panns_index panns_index;
 typedef struct panns_plan panns_plan;
 
