@@ -46,16 +46,18 @@ def test_pss_v5_workers_bit_exact(small_set, gpu_index, T, X, L):
     _check(res, refs, qids)
 
 
-def test_pss_v5_workers_with_spills(small_set, gpu_index):
-    """A 64-slot visited table sends most claims to the global bitmap, whose
-    ordered atomics must keep the worker order of ownership."""
+@pytest.mark.parametrize("bs", [8, 4, 2])
+def test_pss_v5_workers_with_spills(small_set, gpu_index, bs):
+    """A 64-slot visited table (buckets of 8, 4 or 2 slots) sends most claims
+    to the global bitmap, whose ordered atomics must keep the worker order
+    of ownership."""
     ds = small_set
     K, L, T, X = 100, 120, 4, 4
     qids = np.arange(24, dtype=np.uint32)
-    _case(test="pss_workers_spills", T=T, X=X, L=L)
+    _case(test="pss_workers_spills", T=T, X=X, L=L, bucket_slots=bs)
     init = gpu_index.prepare_init_ids(L)
     res = gpu_index.search(qids, K, L, X, mode="pss_v5", num_threads=T, init_ids=init, want_queue=True,
-                           hash_slots=64)
+                           hash_slots=64, bucket_slots=bs)
     g = po.CsrGraph(ds["adj"], ds["deg"], ds["ep"])
     refs = [po.search_pss_v5(g, ds["base"], ds["queries"][q], K, L, X, T, init) for q in qids]
     _check(res, refs, qids)
