@@ -286,7 +286,7 @@ int panns_plan_create_ex(panns_index *h, const panns_params *p, const uint32_t *
  * visited-table slots, 0 for none (arrays of PANNS_MAX_BUCKETS, each nullable). */
 int panns_plan_describe_buckets(panns_plan *plan, uint32_t *n_buckets, uint32_t *max_L, uint32_t *nq,
                                 uint32_t *workgroups, uint32_t *lds_bytes, uint32_t *hash_slots);
-/* The autotuner: times candidate bucketings (1, 2, 3, 4, 6 buckets of equal
+/* The autotuner: times candidate bucketings (1, 2, 3, 4, 6, 8 buckets of equal
  * summed L, each bucket's table by its own largest L; side by side or not;
  * then one table size for all buckets, 8/8 down to 3/8 of the automatic
  * size, each with 0, 256 or 512 overflow entries; 1, 2 or 4 waves

@@ -1736,7 +1736,7 @@ int panns_plan_autotune(panns_index *h, const panns_params *p, const uint32_t *q
     };
     // stage 1: buckets of equal summed L over the whole batch (explicit
     // bounds, so they carry over from the sample), in sequence or side by side
-    static const uint32_t kCounts[] = {1, 2, 3, 4, 6};
+    static const uint32_t kCounts[] = {1, 2, 3, 4, 6, 8};
     for (uint32_t nb : kCounts) {
         panns_plan_options o{};
         uint32_t k = 0;
