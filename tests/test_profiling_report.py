@@ -1,9 +1,11 @@
-"""The uneven-launch report of parallel_anns_amd.profiling on synthetic
-rocprofv3 databases: each search-kernel dispatch is matched to its bucket
-by LDS size (launch order differs from the bucket list), FETCH_SIZE and
-WRITE_SIZE (KiB) are scaled by the copy kernel's bytes-per-counted-byte
+"""The reports of parallel_anns_amd.profiling on synthetic rocprofv3
+databases.  Uneven launches: each search-kernel dispatch is matched to its
+bucket by LDS size (launch order differs from the bucket list), FETCH_SIZE
+and WRITE_SIZE (KiB) are scaled by the copy kernel's bytes-per-counted-byte
 factors, and extra counter passes are divided by each bucket's iterations.
-Expected values are computed by hand from the numbers planted here."""
+Single launches (the records the bench's roofline traffic comes from): the
+last `steps` dispatches of each variant, warm-up ones ignored.  Expected
+values are computed by hand from the numbers planted here."""
 import argparse
 import json
 import sqlite3
@@ -112,3 +114,59 @@ def test_uneven_report_scales_matches_and_divides(dbs):
     line = next(l for l in md.splitlines() if l.startswith("| SQ_INSTS_VALU |"))
     vals = [float(x.replace(",", "")) for x in line.split("|")[2:-1]]
     assert vals == pytest.approx([SQ_VALU[1999] / 1000, SQ_VALU[417] / 4000], abs=0.05)
+
+
+def test_fixed_report_takes_the_last_steps_of_each_variant(tmp_path):
+    """report() (one launch per step, fixed-L and learned-stop variants):
+    the mean of the last `steps` dispatches of each variant, earlier
+    (warm-up) dispatches ignored, traffic from the scaled counters."""
+    steps = 3
+    fixed = KERNEL.format("false")
+    learned = "void panns::search_kernel<96, true, false, false, false>(panns::KArgs)"
+    # warm-up dispatch first (different values), then `steps` timed ones
+    spec = {fixed: ([9e9, 3.0e6, 3.2e6, 3.4e6], [1.0, 5.0e6, 5.0e6, 5.0e6], [7.0, 30e3, 30e3, 30e3]),
+            learned: ([9e9, 2.9e6, 3.0e6, 3.1e6], [1.0, 4.5e6, 4.5e6, 4.5e6], [7.0, 10e3, 10e3, 10e3])}
+    trace = str(tmp_path / "trace.db")
+    c = sqlite3.connect(trace)
+    c.execute("create table kernels (name text, duration integer, start integer)")
+    t = 0
+    for name, (dur, _, _) in spec.items():
+        for d in dur:
+            t += 1
+            c.execute("insert into kernels values (?, ?, ?)", (name, int(d), t))
+    c.commit()
+    c.close()
+    paths = {}
+    for key, counter, idx in (("fetch", "FETCH_SIZE", 1), ("write", "WRITE_SIZE", 2)):
+        rows, did = [], 0
+        for name, vals in spec.items():
+            for v in vals[idx]:
+                did += 1
+                rows.append((did, name, counter, v, 0, 0))
+        paths[key] = str(tmp_path / f"{key}.db")
+        _counters_db(paths[key], rows)
+    paths["cf"] = str(tmp_path / "cf.db")
+    _counters_db(paths["cf"], [(1, COPY, "FETCH_SIZE", 512.0, 0, 1024)])
+    paths["cw"] = str(tmp_path / "cw.db")
+    _counters_db(paths["cw"], [(1, COPY, "WRITE_SIZE", 1024.0, 0, 1024)])
+    run = {"config": "c1", "steps": steps,
+           "fixed": {"M": 4, "L": 289, "table_eighths": 5, "kernel_ms": 3.2,
+                     "algorithmic_bytes_per_launch": 22_000_000_000},
+           "learned": {"M": 4, "L": 289, "table_eighths": 4, "kernel_ms": 3.0, "threshold": 0.44,
+                       "algorithmic_bytes_per_launch": 20_000_000_000}}
+    rp = str(tmp_path / "run.json")
+    with open(rp, "w") as f:
+        f.write(json.dumps(run) + "\n")
+    a = argparse.Namespace(trace=trace, fetch=paths["fetch"], write=paths["write"], cal_fetch=paths["cf"],
+                           cal_write=paths["cw"], copy_bytes=COPY_BYTES, run=rp, command="cmd",
+                           md=str(tmp_path / "r.md"), records=str(tmp_path / "r.json"), dim=96, title="t")
+    profiling.report(a)
+    recs = {r["kernel"]: r for r in json.load(open(a.records))["records"]}
+    for name, variant, alg in ((fixed, "false", 22e9), (learned, "true", 20e9)):
+        dur, fk, wk = spec[name]
+        r = recs[f"search_kernel<96, {variant}>"]
+        assert r["trace_mean_us"] == pytest.approx(sum(dur[1:]) / 3 / 1e3, abs=0.05)
+        traffic = fk[-1] * 1024 * 2.0 + wk[-1] * 1024 * 1.0
+        assert r["traffic_bytes"] == pytest.approx(traffic, rel=1e-6)
+        assert r["algorithmic_bytes"] == alg
+    assert recs["search_kernel<96, true>"]["threshold"] == 0.44
