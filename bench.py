@@ -69,8 +69,8 @@ def measured_traffic(config, kernel, point):
     except (OSError, ValueError, KeyError):
         return None, None
     L = point.get("L_max", point.get("L"))
-    same = [r for r in recs if r["config"] == config and r["kernel"] == kernel and r["M"] == point["M"]
-            and r["L"] == L]
+    same = [r for r in recs if r.get("config") == config and r.get("kernel") == kernel
+            and r.get("M") == point["M"] and r.get("L") == L]
     for r in same:
         if abs(r.get("threshold", 0.0) - point.get("threshold", 0.0)) < 1e-6:
             return r["traffic_bytes"], f"profiles/search_kernel_traffic.json ({r['command']})"

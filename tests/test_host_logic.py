@@ -110,7 +110,10 @@ def test_measured_traffic_lookup(tmp_path, monkeypatch):
         {"config": "c1", "kernel": "search_kernel<96, true>", "M": 4, "L": 289, "traffic_bytes": 900,
          "algorithmic_bytes": 2000, "threshold": 0.5, "command": "cmd"},
         {"config": "c1", "kernel": "search_kernel<96, false>", "M": 4, "L": 289, "traffic_bytes": 1000,
-         "algorithmic_bytes": 2000, "command": "cmd"}]}))
+         "algorithmic_bytes": 2000, "command": "cmd"},
+        # a record of an uneven launch (buckets, no single L) is skipped
+        {"config": "c1", "kernel": "search_kernel<96, false>", "M": 4, "uneven": True, "buckets": [417, 1999],
+         "traffic_bytes": 5, "algorithmic_bytes": 7, "command": "uneven"}]}))
     monkeypatch.setattr(b, "TRAFFIC_FILE", str(f))
     t, src = b.measured_traffic("c1", "search_kernel<96, true>", {"M": 4, "L_max": 289, "threshold": 0.5})
     assert t == 900 and "cmd" in src
@@ -119,6 +122,8 @@ def test_measured_traffic_lookup(tmp_path, monkeypatch):
                                 {"M": 4, "L_max": 289, "threshold": 0.4, "algorithmic_bytes": 3000})
     assert t == 1350 and src.startswith("scaled")
     assert b.measured_traffic("c1", "search_kernel<96, false>", {"M": 8, "L": 289}) == (None, None)
+    t, src = b.measured_traffic("c1", "search_kernel<96, false>", {"M": 4, "L": 289})
+    assert t == 1000 and "uneven" not in src
 
 
 def test_deficit_distribution():
