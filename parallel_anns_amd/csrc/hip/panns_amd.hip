@@ -537,9 +537,8 @@ uint32_t lds_bytes_pss(uint32_t Lcap, uint32_t nb_cap, uint32_t tbl_slots, uint3
            4u * (panns::kWsN * panns::kPssMaxQueues + 4u);
 }
 
-// One N-bit bitmap per persistent workgroup, padded to at least 64 words:
-// in the claim's bitmap path every lane of the wave issues an atomic, and a
-// lane with no id ORs 0 into word `lane` of its own workgroup's bitmap.
+// One N-bit bitmap per persistent workgroup, at least 64 words long, so even
+// for a tiny id space no two workgroups' bitmaps share a cache line.
 uint64_t bitmap_words_for(uint32_t n) { return std::max<uint64_t>(((uint64_t)n + 31) / 32, 64); }
 
 int ensure_scratch(panns_index *h, uint32_t grid)
@@ -2014,8 +2013,7 @@ int panns_subsearch_with_top_m(panns_index *h, uint32_t M_max, uint32_t query_id
     }
     HIP_TRY(hipSetDevice(h->device));
     const size_t words32 = ((size_t)h->n + 31) / 32;
-    // padded as the per-workgroup bitmaps are (bitmap_words_for): lanes with
-    // no id OR 0 into word `lane`
+    // sized as the per-workgroup bitmaps are (bitmap_words_for)
     const size_t bm_words = bitmap_words_for(h->n);
     DevBuf<uint32_t> bm;
     HIP_TRY(bm.ensure(bm_words));
